@@ -1,0 +1,181 @@
+/*
+ * rh_gpu.h -- C ABI of the MI355X-native RHybrid hot path.
+ *
+ * One context (rh_ctx) owns the device state of one simulation domain on one
+ * GPU: the grid arrays, the particle stores of every species and the scratch
+ * of the sort and deposit.  All calls enqueue on the context's HIP stream;
+ * upload/download/diagnostics calls synchronise.  Every function returns
+ * RH_OK (0) or a negative rh_status; rh_last_error() gives the message.  The
+ * library never aborts.
+ *
+ * The operators replace the reference's Corsair operator surface one for one
+ * (file:line in the reference tree are given at each declaration).  Arrays
+ * cross the boundary in the reference's layouts: FP64, cell n = i + nx*(j +
+ * ny*k), vector components interleaved n*3+l (src/user.cpp:839-927), and
+ * particles as (cell, AoS state[7] = {x,y,z,vx,vy,vz,w}) with cell-local
+ * coordinates in metres (src/particle_definition.h:29-38).
+ */
+#ifndef RH_GPU_H
+#define RH_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rh_ctx rh_ctx;
+
+typedef enum rh_status {
+    RH_OK = 0,
+    RH_ERR_ARG = -1,       /* bad argument (null pointer, id or size out of range) */
+    RH_ERR_HIP = -2,       /* a HIP runtime call failed */
+    RH_ERR_STATE = -3,     /* call not valid in the current state */
+    RH_ERR_CAPACITY = -4,  /* a particle store or index would overflow */
+    RH_ERR_NO_DEVICE = -5  /* no usable GPU */
+} rh_status;
+
+typedef enum rh_precision { RH_FP32 = 0, RH_FP64 = 1 } rh_precision;
+
+/* Grid of nx*ny*nz cells including the exterior layer of non-periodic
+ * dimensions (the LogicallyCartesian grids of the reference's example
+ * configs); cubic cells of edge dx; (x0,y0,z0) is the lower corner of cell
+ * (0,0,0).  tile[] is the cell tile of the sort and deposit (0 = choose);
+ * each must divide its dimension. */
+typedef struct rh_grid_desc {
+    int32_t nx, ny, nz;
+    int32_t periodic[3];
+    double dx;
+    double x0, y0, z0;
+    int32_t tile[3];
+} rh_grid_desc;
+
+/* The Hybrid:: parameters the path reads (src/hybrid.h:253-300), in SI, as
+ * src/user.cpp:1565-1807 derives them. */
+typedef struct rh_params {
+    double dt;
+    int32_t use_hall;             /* Hybrid::useHallElectricField */
+    int32_t use_e_pressure;       /* Hybrid::useElectronPressure */
+    int32_t isothermal;           /* Hybrid::useIsothermalElectrons */
+    double e_pressure_coeff;      /* Hybrid::electronPressureCoeff */
+    double max_ion_speed2;        /* Hybrid::maxIonSpeed2 */
+    double max_electron_speed2;   /* Hybrid::maxElectronSpeed2 */
+    double min_rho;               /* Hybrid::minIonChargeDensity [C/m^3] */
+    double max_E2;                /* Hybrid::maxElectricField2 (0: off) */
+    double max_whistler;          /* Hybrid::maxWhistlerSpeed (0: off) */
+    int32_t n_linear_smoothings;  /* Hybrid::N_linearSmoothingsNodeE */
+    int32_t use_gaussian;         /* Hybrid::useGaussianSmoothingNodeE */
+    double gauss_coeffs[4];       /* Hybrid::gaussianSmoothingCoeffsNodeE */
+    double imf[3];                /* Hybrid::IMFB{x,y,z} */
+    int32_t imf_cell[6];          /* Hybrid::IMFBoundaryCellB, +x -x +y -y +z -z */
+    int32_t imf_face[6];          /* Hybrid::IMFBoundaryFaceB */
+    double obstacle_ue[3];        /* Hybrid::fieldObstacleUe */
+    int32_t initial_flow_through; /* Hybrid::useInitialFlowThrough */
+    int32_t use_gravity;          /* Hybrid::useGravity */
+    double GMdt;                  /* Hybrid::GMdt */
+} rh_params;
+
+/* Species (src/particle_species.h:30-46): charge and mass in SI, the
+ * macroparticle weight, squared obstacle radius (<0: none), the accumulate and
+ * accelerate flags, and the particle capacity to reserve (0: 1.5x the first
+ * upload). */
+typedef struct rh_species_desc {
+    double q, m, w;
+    double R2_obstacle;
+    int32_t accumulate, accelerate;
+    int64_t capacity;
+} rh_species_desc;
+
+/* Field ids: the grid arrays of src/user.cpp:839-927. */
+enum rh_field_id {
+    RH_FACE_B = 0, RH_CELL_B, RH_CELL_J, RH_CELL_UE, RH_CELL_JI, RH_CELL_EP, RH_CELL_RHOQ,
+    RH_NODE_RHOQ, RH_NODE_E, RH_NODE_B, RH_NODE_J, RH_NODE_UE, RH_NODE_JI, RH_NODE_ETA,
+    RH_N_FIELDS
+};
+/* Inner-boundary flag ids (src/user.cpp:1882-1912). */
+enum rh_flag_id {
+    RH_INNER_FIELD = 0, RH_INNER_NODE, RH_INNER_PARTICLE, RH_INNER_CELLEP, RH_N_FLAGS
+};
+/* Constraint counters, as Hybrid::logCounter* (src/hybrid.h:306-330). */
+enum rh_counter_id {
+    RH_CNT_MAX_VI = 0, RH_CNT_MIN_CELL_RHO, RH_CNT_MAX_CELL_UE, RH_CNT_MIN_NODE_RHO,
+    RH_CNT_MAX_NODE_UE, RH_CNT_MAX_E, RH_CNT_MAX_VW, RH_CNT_DROPPED, RH_N_COUNTERS
+};
+
+/* ---- context -------------------------------------------------------- */
+
+/* Replaces userEarlyInitialization + the array allocation of
+ * userLateInitialization (src/user.cpp:160-185, 839-1050): picks the device
+ * (the current HIP device), allocates grid arrays (zeroed, as
+ * src/user.cpp:1853-1870) and creates the stream. */
+int rh_create(const rh_grid_desc* grid, const rh_params* params, int32_t precision, rh_ctx** out);
+int rh_destroy(rh_ctx* ctx);
+const char* rh_last_error(const rh_ctx* ctx);
+/* Species::readParameters + particle list creation (src/particle_species.cpp:38-107). */
+int rh_add_species(rh_ctx* ctx, const rh_species_desc* desc, int32_t* id);
+
+/* ---- data in and out (ParGrid getUserData* accessors) ---------------- */
+
+int rh_upload_field(rh_ctx* ctx, int32_t field_id, const double* data);
+int rh_download_field(rh_ctx* ctx, int32_t field_id, double* data);
+int rh_upload_flags(rh_ctx* ctx, int32_t flag_id, const uint8_t* data);
+/* Replace the particles of a species; cells must be valid, coordinates are
+ * cell-local metres in [0,dx), w is ignored (the species weight is used). */
+int rh_upload_particles(rh_ctx* ctx, int32_t species, int64_t n, const int64_t* cell, const double* state7);
+int rh_num_particles(rh_ctx* ctx, int32_t species, int64_t* n);
+/* Live particles in the store's order: by tile, then by cell inside the tile,
+ * then by half-cell octant; w is the species weight. */
+int rh_download_particles(rh_ctx* ctx, int32_t species, int64_t capacity, int64_t* cell, double* state7);
+
+/* ---- the operators ---------------------------------------------------- */
+
+/* setupGetFields (src/hybrid_propagator.cpp:1861-1876): faceB halo exchange;
+ * nothing to exchange inside one GPU. */
+int rh_setup_get_fields(rh_ctx* ctx);
+/* BorisBuneman::propagateCell for every cell of a species
+ * (src/particle_propagator_boris_buneman.h:196-207, :64-194). */
+int rh_propagate(rh_ctx* ctx, int32_t species);
+/* ParticleListSkeleton::waitParticleSends (src/user.cpp:123, Corsair): the
+ * cell migration, done as a deterministic counting sort by cell. */
+int rh_wait_particle_sends(rh_ctx* ctx, int32_t species);
+/* Accumulator::clearAccumulationArrays (src/particle_accumulator.cpp:337-362). */
+int rh_clear_accumulation_arrays(rh_ctx* ctx);
+/* Accumulator::accumulateBoundaryCells + accumulateInnerCells
+ * (src/particle_accumulator.h:36-37, .cpp:202-254). */
+int rh_accumulate(rh_ctx* ctx, int32_t species);
+/* Accumulator::sendUpdates + wait + addRemoteUpdates
+ * (src/particle_accumulator.cpp:381-409, 260-335): the fixed-order FP64 sum of
+ * tile ghost contributions, /dV and the rho floor. */
+int rh_add_remote_updates(rh_ctx* ctx);
+/* ParticleBoundaryCondHybrid::apply (src/particle_boundary_cond_hybrid.h:55-187). */
+int rh_apply_boundary_conditions(rh_ctx* ctx, int32_t species);
+/* propagateB (src/hybrid_propagator.cpp:42-440). */
+int rh_propagate_B(rh_ctx* ctx);
+/* propagate() (src/user.cpp:118-134) without logging or injection, nsteps times. */
+int rh_step(rh_ctx* ctx, int32_t nsteps);
+
+/* ---- diagnostics ------------------------------------------------------ */
+
+/* logCalcParticle sums in FP64 (src/diagnostics.h:209-220):
+ * out = {sum w, sum w vx, sum w vy, sum w vz, sum w v^2}. */
+int rh_diagnostics(rh_ctx* ctx, int32_t species, double out[5]);
+int rh_counters(rh_ctx* ctx, double out[RH_N_COUNTERS]);
+int rh_synchronize(rh_ctx* ctx);
+
+/* ---- measurement -------------------------------------------------------- */
+
+/* Per-kernel timing with HIP events on the context's stream (the profile::
+ * regions of src/hybrid_propagator.cpp:33-38).  When enabled, every launch is
+ * bracketed by events; rh_kernel_times returns, for the kernel name, the
+ * accumulated milliseconds and the number of launches since the last reset. */
+int rh_set_timing(rh_ctx* ctx, int32_t enable);
+int rh_kernel_times(rh_ctx* ctx, const char* kernel, double* ms, int64_t* launches);
+int rh_reset_timing(rh_ctx* ctx);
+/* Sorted-store layout facts for roofline accounting: out = {n_tiles,
+ * tile_x, tile_y, tile_z, n_cells, particle bytes per particle}. */
+int rh_layout_info(rh_ctx* ctx, int64_t out[6]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,789 @@
+// rh_capi.hip -- the host side: a simulation context per domain, the
+// operators of the reference's step as sequences of kernel launches, and the
+// extern "C" entry points declared in include/rh_gpu.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/rh_gpu.h"
+#include "rh_fields.hpp"
+#include "rh_geom.hpp"
+#include "rh_particles.hpp"
+
+namespace rh {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct ArgError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct CapError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct StateError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define RH_HIP(call)                                                                               \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            throw HipError(std::string(#call) + ": " + hipGetErrorString(e_));                     \
+    } while (0)
+
+static const int FIELD_DIM[RH_N_FIELDS] = {3, 3, 3, 3, 3, 3, 1, 1, 3, 3, 3, 3, 3, 1};
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { free(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    void alloc(size_t count) {
+        free();
+        if (count == 0) count = 1;
+        RH_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void free() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { free(); }
+};
+
+struct Timer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    double ms = 0;
+    int64_t launches = 0;
+};
+
+class Ctx {
+public:
+    virtual ~Ctx() = default;
+    std::string err;
+    virtual int add_species(const rh_species_desc* d) = 0;
+    virtual void upload_field(int id, const double* data) = 0;
+    virtual void download_field(int id, double* data) = 0;
+    virtual void upload_flags(int id, const uint8_t* data) = 0;
+    virtual void upload_particles(int s, int64_t n, const int64_t* cell, const double* st) = 0;
+    virtual int64_t num_particles(int s) = 0;
+    virtual void download_particles(int s, int64_t cap, int64_t* cell, double* st) = 0;
+    virtual void propagate(int s) = 0;
+    virtual void wait_particle_sends(int s) = 0;
+    virtual void clear_accumulation() = 0;
+    virtual void accumulate(int s) = 0;
+    virtual void add_remote_updates() = 0;
+    virtual void apply_bcs(int s) = 0;
+    virtual void propagate_B() = 0;
+    virtual void step(int n) = 0;
+    virtual void diagnostics(int s, double* out) = 0;
+    virtual void counters(double* out) = 0;
+    virtual void synchronize() = 0;
+    virtual void set_timing(bool on) = 0;
+    virtual void kernel_time(const char* name, double* ms, int64_t* launches) = 0;
+    virtual void reset_timing() = 0;
+    virtual void layout(int64_t* out) = 0;
+};
+
+template <typename Real>
+class Sim final : public Ctx {
+public:
+    struct Species {
+        rh_species_desc d;
+        size_t cap = 0;
+        DevBuf<Real> a[6], b[6];
+        DevBuf<uint32_t> cell, packed, sub, sub_new, cnt_tile;
+        bool pushed = false;  // B/packed hold a push not yet sorted
+        PArrays<Real> A() { return {a[0].p, a[1].p, a[2].p, a[3].p, a[4].p, a[5].p, cell.p}; }
+        PArrays<Real> B() { return {b[0].p, b[1].p, b[2].p, b[3].p, b[4].p, b[5].p, nullptr}; }
+    };
+
+    Geom g{};
+    rh_params P{};
+    hipStream_t stream = nullptr;
+    DevBuf<Real> fld[RH_N_FIELDS];
+    DevBuf<Real> tmpv;            // old nodeE for the gaussian smoothing
+    DevBuf<uint8_t> flg[RH_N_FLAGS];
+    DevBuf<int> ext_cells, bc_cells;
+    int n_ext = 0, n_bc = 0;
+    DevBuf<double> tile_acc;
+    DevBuf<unsigned long long> dcount;
+    DevBuf<double> diag_partial;
+    std::vector<std::unique_ptr<Species>> sp;
+    std::vector<DevBuf<uint32_t>> scan_lv;  // per level: block sums
+    DevBuf<uint32_t> zero_u32;
+    int kbits = 12;
+    bool timing = false;
+    std::map<std::string, Timer> timers;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<uint8_t> h_inner_particle;
+
+    Sim(const rh_grid_desc* gd, const rh_params* pr) {
+        if (gd->nx < 3 || gd->ny < 3 || gd->nz < 3) throw ArgError("each grid dimension needs >= 3 cells");
+        if (!(gd->dx > 0)) throw ArgError("dx must be positive");
+        int dev_count = 0;
+        if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) throw HipError("no GPU visible");
+        P = *pr;
+        g.nx = gd->nx; g.ny = gd->ny; g.nz = gd->nz;
+        g.nc = g.nx * g.ny * g.nz;
+        for (int d = 0; d < 3; ++d) g.per[d] = gd->periodic[d] ? 1 : 0;
+        g.dx = gd->dx; g.x0 = gd->x0; g.y0 = gd->y0; g.z0 = gd->z0;
+        choose_tiles(gd->tile);
+        RH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (int f = 0; f < RH_N_FIELDS; ++f) {
+            fld[f].alloc((size_t)g.nc * FIELD_DIM[f]);
+            fill(fld[f].p, Real(0), fld[f].n);
+        }
+        tmpv.alloc((size_t)g.nc * 3);
+        for (int f = 0; f < RH_N_FLAGS; ++f) {
+            flg[f].alloc(g.nc);
+            RH_HIP(hipMemsetAsync(flg[f].p, 0, g.nc, stream));
+        }
+        h_inner_particle.assign(g.nc, 0);
+        // exterior cells: missing any neighbour
+        std::vector<int> ext;
+        for (int n = 0; n < g.nc; ++n) {
+            int i, j, k;
+            cell_ijk(g, n, i, j, k);
+            if (!all_nbrs(g, i, j, k)) ext.push_back(n);
+        }
+        n_ext = (int)ext.size();
+        ext_cells.alloc(std::max<size_t>(ext.size(), 1));
+        if (n_ext) RH_HIP(hipMemcpy(ext_cells.p, ext.data(), ext.size() * sizeof(int), hipMemcpyHostToDevice));
+        rebuild_bc_cells();
+        tile_acc.alloc((size_t)g.ntiles * g.next * 4);
+        fill(tile_acc.p, 0.0, tile_acc.n);
+        dcount.alloc(C_N);
+        RH_HIP(hipMemsetAsync(dcount.p, 0, sizeof(unsigned long long) * C_N, stream));
+        diag_partial.alloc(DIAG_BLOCKS * 5);
+        zero_u32.alloc(1);
+        RH_HIP(hipMemsetAsync(zero_u32.p, 0, sizeof(uint32_t), stream));
+        // scan levels
+        size_t n = (size_t)g.nkeys;
+        while (n > (size_t)SCAN_TILE) {
+            const size_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+            scan_lv.emplace_back();
+            scan_lv.back().alloc(nb + 1);
+            scan_lv.emplace_back();
+            scan_lv.back().alloc(nb + 1);
+            n = nb;
+        }
+        kbits = 1;
+        while ((1 << kbits) < g.extk) ++kbits;
+        RH_HIP(hipStreamSynchronize(stream));
+    }
+
+    ~Sim() override {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& t : timers)
+            for (auto& pe : t.second.pending) { (void)hipEventDestroy(pe.first); (void)hipEventDestroy(pe.second); }
+        for (auto e : event_pool) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void choose_tiles(const int32_t* req) {
+        int t[3] = {req ? req[0] : 0, req ? req[1] : 0, req ? req[2] : 0};
+        const int dims[3] = {g.nx, g.ny, g.nz};
+        const int max_cells = sizeof(Real) == 4 ? 64 : 32;  // deposit LDS <= 64 KiB
+        if (t[0] <= 0 || t[1] <= 0 || t[2] <= 0) {
+            for (int d = 0; d < 3; ++d) {
+                t[d] = 4;
+                while (t[d] > 1 && dims[d] % t[d]) t[d] /= 2;
+            }
+            auto ntiles = [&]() { return (dims[0] / t[0]) * (dims[1] / t[1]) * (dims[2] / t[2]); };
+            while (t[0] * t[1] * t[2] > max_cells) {
+                int d = 0;
+                for (int q = 1; q < 3; ++q) if (t[q] > t[d]) d = q;
+                t[d] /= 2;
+            }
+            // enough tiles to fill the chip when the grid is small
+            while (ntiles() < 4096) {
+                int d = -1;
+                for (int q = 0; q < 3; ++q)
+                    if (t[q] > 2 && (d < 0 || t[q] > t[d])) d = q;
+                if (d < 0) break;
+                t[d] /= 2;
+            }
+        }
+        for (int d = 0; d < 3; ++d)
+            if (t[d] < 1 || dims[d] % t[d]) throw ArgError("tile shape must divide the grid");
+        if (t[0] * t[1] * t[2] > max_cells) throw ArgError("tile too large for this precision");
+        g.tx = t[0]; g.ty = t[1]; g.tz = t[2];
+        g.tc = g.tx * g.ty * g.tz;
+        g.ntx = g.nx / g.tx; g.nty = g.ny / g.ty; g.ntz = g.nz / g.tz;
+        g.ntiles = g.ntx * g.nty * g.ntz;
+        g.ex = g.tx + 2; g.ey = g.ty + 2; g.ez = g.tz + 2;
+        g.next = g.ex * g.ey * g.ez;
+        g.extk = g.next * 8;
+        if (g.extk > MAX_EXTK) throw ArgError("tile extended region too large");
+        const uint64_t nk = (uint64_t)g.ntiles * g.tc * 8;
+        if (nk >= 0xFFFFFFFFull) throw ArgError("grid too large for 32-bit subcell keys");
+        g.nkeys = (uint32_t)nk;
+    }
+
+    void rebuild_bc_cells() {
+        std::vector<int> cells;
+        for (int n = 0; n < g.nc; ++n) {
+            int i, j, k;
+            cell_ijk(g, n, i, j, k);
+            if (!all_nbrs(g, i, j, k) || h_inner_particle[n]) cells.push_back(n);
+        }
+        n_bc = (int)cells.size();
+        bc_cells.alloc(std::max<size_t>(cells.size(), 1));
+        if (n_bc) RH_HIP(hipMemcpy(bc_cells.p, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+
+    // ---- launch helpers -------------------------------------------------
+
+    hipEvent_t get_event() {
+        if (!event_pool.empty()) {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        RH_HIP(hipEventCreate(&e));
+        return e;
+    }
+
+    void resolve_timers() {
+        RH_HIP(hipStreamSynchronize(stream));
+        for (auto& kv : timers) {
+            for (auto& pe : kv.second.pending) {
+                float ms = 0;
+                RH_HIP(hipEventElapsedTime(&ms, pe.first, pe.second));
+                kv.second.ms += ms;
+                event_pool.push_back(pe.first);
+                event_pool.push_back(pe.second);
+            }
+            kv.second.pending.clear();
+        }
+    }
+
+    template <typename K, typename... Args>
+    void launch(const char* name, K kernel, unsigned grid, unsigned block, size_t shmem, Args... args) {
+        if (grid == 0) return;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timing) {
+            e0 = get_event();
+            e1 = get_event();
+            RH_HIP(hipEventRecord(e0, stream));
+        }
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, stream, args...);
+        RH_HIP(hipGetLastError());
+        if (timing) {
+            RH_HIP(hipEventRecord(e1, stream));
+            Timer& t = timers[name];
+            t.pending.emplace_back(e0, e1);
+            t.launches++;
+            if (t.pending.size() > 4096) resolve_timers();
+        }
+    }
+
+    template <typename T>
+    void fill(T* p, T v, size_t n) {
+        if (n == 0) return;
+        const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+        launch("fill", k_fill<T>, blocks, 256, 0, p, v, n);
+    }
+
+    static unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+    V3<Real> v3(int id) {
+        Real* p = fld[id].p;
+        return {p, p + g.nc, p + 2 * (size_t)g.nc};
+    }
+
+    Species& S(int s) {
+        if (s < 0 || s >= (int)sp.size()) throw ArgError("bad species id");
+        return *sp[s];
+    }
+
+    // ---- set-up and data movement ----------------------------------------
+
+    int add_species(const rh_species_desc* d) override {
+        auto s = std::make_unique<Species>();
+        s->d = *d;
+        s->sub.alloc((size_t)g.nkeys + 1);
+        s->sub_new.alloc((size_t)g.nkeys + 1);
+        RH_HIP(hipMemsetAsync(s->sub.p, 0, sizeof(uint32_t) * ((size_t)g.nkeys + 1), stream));
+        s->cnt_tile.alloc((size_t)g.ntiles * g.extk);
+        ensure_capacity(*s, (size_t)std::max<int64_t>(d->capacity, 0));
+        sp.push_back(std::move(s));
+        RH_HIP(hipStreamSynchronize(stream));
+        return (int)sp.size() - 1;
+    }
+
+    void ensure_capacity(Species& s, size_t cap) {
+        if (cap <= s.cap && s.cap > 0) return;
+        if (cap >= 0xFFFFFFFFull) throw CapError("particle capacity exceeds 32-bit indices");
+        cap = std::max<size_t>(cap, 64);
+        for (int q = 0; q < 6; ++q) { s.a[q].alloc(cap); s.b[q].alloc(cap); }
+        s.cell.alloc(cap);
+        s.packed.alloc(cap);
+        s.cap = cap;
+    }
+
+    void upload_field(int id, const double* data) override {
+        if (id < 0 || id >= RH_N_FIELDS || !data) throw ArgError("bad field id or null data");
+        const int dim = FIELD_DIM[id];
+        std::vector<Real> h((size_t)g.nc * dim);
+        for (int l = 0; l < dim; ++l)
+            for (int n = 0; n < g.nc; ++n) h[(size_t)l * g.nc + n] = (Real)data[(size_t)n * dim + l];
+        RH_HIP(hipMemcpyAsync(fld[id].p, h.data(), h.size() * sizeof(Real), hipMemcpyHostToDevice, stream));
+        RH_HIP(hipStreamSynchronize(stream));
+    }
+
+    void download_field(int id, double* data) override {
+        if (id < 0 || id >= RH_N_FIELDS || !data) throw ArgError("bad field id or null data");
+        const int dim = FIELD_DIM[id];
+        std::vector<Real> h((size_t)g.nc * dim);
+        RH_HIP(hipMemcpyAsync(h.data(), fld[id].p, h.size() * sizeof(Real), hipMemcpyDeviceToHost, stream));
+        RH_HIP(hipStreamSynchronize(stream));
+        for (int l = 0; l < dim; ++l)
+            for (int n = 0; n < g.nc; ++n) data[(size_t)n * dim + l] = (double)h[(size_t)l * g.nc + n];
+    }
+
+    void upload_flags(int id, const uint8_t* data) override {
+        if (id < 0 || id >= RH_N_FLAGS || !data) throw ArgError("bad flag id or null data");
+        RH_HIP(hipMemcpyAsync(flg[id].p, data, g.nc, hipMemcpyHostToDevice, stream));
+        RH_HIP(hipStreamSynchronize(stream));
+        if (id == RH_INNER_PARTICLE) {
+            for (int n = 0; n < g.nc; ++n) h_inner_particle[n] = data[n] ? 1 : 0;
+            rebuild_bc_cells();
+        }
+    }
+
+    static Real to_unit(double x, double dx) {
+        Real r = (Real)(x / dx);
+        if (r >= Real(1)) r = std::nextafter(Real(1), Real(0));
+        if (r < Real(0)) r = Real(0);
+        return r;
+    }
+
+    void upload_particles(int s, int64_t n, const int64_t* cell, const double* st) override {
+        Species& S_ = S(s);
+        if (n < 0 || (n > 0 && (!cell || !st))) throw ArgError("bad particle arrays");
+        if ((uint64_t)n >= 0xFFFFFFFFull) throw CapError("too many particles for 32-bit indices");
+        // keys and a stable counting sort on the host
+        std::vector<uint32_t> key((size_t)n);
+        std::vector<Real> u[3];
+        for (int q = 0; q < 3; ++q) u[q].resize((size_t)n);
+        for (int64_t p = 0; p < n; ++p) {
+            if (cell[p] < 0 || cell[p] >= g.nc) throw ArgError("particle cell out of range");
+            int i, j, k;
+            cell_ijk(g, (int)cell[p], i, j, k);
+            const Real X = to_unit(st[7 * p + 0], g.dx), Y = to_unit(st[7 * p + 1], g.dx), Z = to_unit(st[7 * p + 2], g.dx);
+            u[0][p] = X; u[1][p] = Y; u[2][p] = Z;
+            const int o = (X >= Real(0.5)) + 2 * (Y >= Real(0.5)) + 4 * (Z >= Real(0.5));
+            key[p] = cell_key(g, i, j, k) * 8u + (uint32_t)o;
+        }
+        std::vector<uint32_t> sub((size_t)g.nkeys + 1, 0);
+        for (int64_t p = 0; p < n; ++p) sub[key[p] + 1]++;
+        for (size_t k2 = 0; k2 < g.nkeys; ++k2) sub[k2 + 1] += sub[k2];
+        std::vector<uint32_t> fillp(sub.begin(), sub.end() - 1);
+        std::vector<Real> h[6];
+        for (int q = 0; q < 6; ++q) h[q].resize((size_t)n);
+        std::vector<uint32_t> hc((size_t)n);
+        for (int64_t p = 0; p < n; ++p) {
+            const uint32_t d = fillp[key[p]]++;
+            h[0][d] = u[0][p]; h[1][d] = u[1][p]; h[2][d] = u[2][p];
+            h[3][d] = (Real)st[7 * p + 3]; h[4][d] = (Real)st[7 * p + 4]; h[5][d] = (Real)st[7 * p + 5];
+            hc[d] = (uint32_t)cell[p];
+        }
+        size_t cap = (size_t)std::max<int64_t>(S_.d.capacity, 0);
+        if (cap < (size_t)n) cap = (size_t)n + (size_t)n / 2;
+        ensure_capacity(S_, cap);
+        for (int q = 0; q < 6; ++q)
+            RH_HIP(hipMemcpyAsync(S_.a[q].p, h[q].data(), sizeof(Real) * n, hipMemcpyHostToDevice, stream));
+        RH_HIP(hipMemcpyAsync(S_.cell.p, hc.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, stream));
+        RH_HIP(hipMemcpyAsync(S_.sub.p, sub.data(), sizeof(uint32_t) * sub.size(), hipMemcpyHostToDevice, stream));
+        RH_HIP(hipStreamSynchronize(stream));
+        S_.pushed = false;
+    }
+
+    // live particles (those removed by the boundary conditions stay stored,
+    // marked, until the next sort)
+    int64_t num_particles(int s) override {
+        double t[5];
+        diag_raw(s, t);
+        return (int64_t)t[0];
+    }
+
+    void download_particles(int s, int64_t capn, int64_t* cell, double* st) override {
+        Species& S_ = S(s);
+        uint32_t np = 0;
+        RH_HIP(hipMemcpyAsync(&np, S_.sub.p + g.nkeys, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        RH_HIP(hipStreamSynchronize(stream));
+        std::vector<Real> h[6];
+        std::vector<uint32_t> hc(np);
+        for (int q = 0; q < 6; ++q) {
+            h[q].resize(np);
+            if (np) RH_HIP(hipMemcpy(h[q].data(), S_.a[q].p, sizeof(Real) * np, hipMemcpyDeviceToHost));
+        }
+        if (np) RH_HIP(hipMemcpy(hc.data(), S_.cell.p, sizeof(uint32_t) * np, hipMemcpyDeviceToHost));
+        int64_t out = 0;
+        for (uint32_t p = 0; p < np; ++p) {
+            if (hc[p] == DEAD_CELL) continue;
+            if (out >= capn) throw CapError("download buffer too small");
+            if (cell) cell[out] = hc[p];
+            if (st) {
+                double* q = st + 7 * out;
+                q[0] = (double)h[0][p] * g.dx; q[1] = (double)h[1][p] * g.dx; q[2] = (double)h[2][p] * g.dx;
+                q[3] = (double)h[3][p]; q[4] = (double)h[4][p]; q[5] = (double)h[5][p];
+                q[6] = S_.d.w;
+            }
+            ++out;
+        }
+    }
+
+    // ---- the operators ------------------------------------------------------
+
+    void propagate(int s) override {
+        Species& S_ = S(s);
+        if (S_.pushed) throw StateError("propagate called twice without wait_particle_sends");
+        PushParams pp;
+        pp.dt = P.dt;
+        pp.dt_dx = P.dt / g.dx;
+        pp.qmdt2 = 0.5 * S_.d.q * P.dt / S_.d.m;
+        pp.max_vi2 = P.max_ion_speed2;
+        pp.GMdt = P.GMdt;
+        pp.accelerate = S_.d.accelerate;
+        pp.use_e_pressure = P.use_e_pressure;
+        pp.use_gravity = P.use_gravity;
+        pp.initial_flow_through = P.initial_flow_through;
+        const V3<Real> fb = v3(RH_FACE_B), ue = v3(RH_CELL_UE), ep = v3(RH_CELL_EP);
+        PushFields<Real> pf{fb.x, fb.y, fb.z, ue.x, ue.y, ue.z, ep.x, ep.y, ep.z};
+        launch("push", k_push_rank<Real>, g.ntiles, 64, sizeof(uint32_t) * g.extk, g, pp, pf, S_.A(), S_.B(),
+               S_.packed.p, (const uint32_t*)S_.sub.p, S_.cnt_tile.p, dcount.p, kbits);
+        S_.pushed = true;
+    }
+
+    void scan(const uint32_t* in, uint32_t n, uint32_t* out, size_t level) {
+        if (n <= (uint32_t)SCAN_TILE) {
+            launch("scan", k_scan_apply, 1, SCAN_THREADS, 0, in, n, (const uint32_t*)zero_u32.p, out);
+            return;
+        }
+        const uint32_t nb = cdiv(n, SCAN_TILE);
+        uint32_t* bsum = scan_lv[level].p;
+        uint32_t* boff = scan_lv[level + 1].p;
+        launch("scan", k_scan_reduce, nb, SCAN_THREADS, 0, in, n, bsum);
+        scan(bsum, nb, boff, level + 2);
+        launch("scan", k_scan_apply, nb, SCAN_THREADS, 0, in, n, (const uint32_t*)boff, out);
+    }
+
+    void wait_particle_sends(int s) override {
+        Species& S_ = S(s);
+        if (!S_.pushed) throw StateError("wait_particle_sends without a preceding propagate");
+        const unsigned kb = cdiv(g.nkeys, 256);
+        launch("sort_count", k_slot_offsets<false>, kb, 256, 0, g, S_.cnt_tile.p, S_.sub_new.p);
+        scan(S_.sub_new.p, g.nkeys, S_.sub_new.p, 0);
+        launch("sort_base", k_slot_offsets<true>, kb, 256, 0, g, S_.cnt_tile.p, S_.sub_new.p);
+        launch("scatter", k_scatter<Real>, g.ntiles, 64, 0, g, S_.B(), (const uint32_t*)S_.packed.p,
+               (const uint32_t*)S_.cnt_tile.p, (const uint32_t*)S_.sub.p, S_.A(), (uint32_t)S_.cap, dcount.p);
+        std::swap(S_.sub.p, S_.sub_new.p);
+        S_.pushed = false;
+    }
+
+    void clear_accumulation() override { fill(tile_acc.p, 0.0, tile_acc.n); }
+
+    void accumulate(int s) override {
+        Species& S_ = S(s);
+        if (!S_.d.accumulate) return;
+        if (S_.pushed) throw StateError("accumulate between propagate and wait_particle_sends");
+        const int NS = g.tc * 8;
+        launch("deposit", k_deposit<Real>, g.ntiles, NS, sizeof(Real) * NS * 32, g, S_.A(),
+               (const uint32_t*)S_.sub.p, S_.d.w * S_.d.q, tile_acc.p);
+    }
+
+    void add_remote_updates() override {
+        const V3<Real> ji = v3(RH_CELL_JI);
+        launch("fold", k_fold<Real>, cdiv(g.nc, 256), 256, 0, g, (const double*)tile_acc.p,
+               1.0 / (g.dx * g.dx * g.dx), P.min_rho, fld[RH_CELL_RHOQ].p, ji.x, ji.y, ji.z, dcount.p);
+    }
+
+    void apply_bcs(int s) override {
+        Species& S_ = S(s);
+        if (S_.pushed) throw StateError("boundary conditions between propagate and wait_particle_sends");
+        launch("boundary", k_boundary<Real>, cdiv(n_bc, 256), 256, 0, g, (const int*)bc_cells.p, n_bc,
+               (const uint32_t*)S_.sub.p, S_.A(), S_.d.R2_obstacle);
+    }
+
+    void neumann_cell(int id, int dim) {
+        launch("neumann", k_neumann_cell<Real>, cdiv(n_ext, 256), 256, 0, g, (const int*)ext_cells.p, n_ext,
+               fld[id].p, dim);
+    }
+
+    void cell2node(int from, int to, int dim) {
+        launch("cell2node", k_cell2node<Real>, cdiv(g.nc, 256), 256, 0, g, (const Real*)fld[from].p, fld[to].p, dim);
+    }
+
+    void propagate_B() override {
+        const unsigned cb = cdiv(g.nc, 256), eb = cdiv(n_ext, 256);
+        const int imf_cell = P.imf_cell[0] | P.imf_cell[1] << 1 | P.imf_cell[2] << 2 | P.imf_cell[3] << 3 |
+                             P.imf_cell[4] << 4 | P.imf_cell[5] << 5;
+        const int imf_face = P.imf_face[0] | P.imf_face[1] << 1 | P.imf_face[2] << 2 | P.imf_face[3] << 3 |
+                             P.imf_face[4] << 4 | P.imf_face[5] << 5;
+        const Real bx = (Real)P.imf[0], by = (Real)P.imf[1], bz = (Real)P.imf[2];
+        launch("face2cell", k_face2cell<Real>, cb, 256, 0, g, v3(RH_FACE_B), v3(RH_CELL_B));
+        neumann_cell(RH_CELL_B, 3);
+        neumann_cell(RH_CELL_JI, 3);
+        neumann_cell(RH_CELL_RHOQ, 1);
+        if (imf_cell) launch("imf", k_imf<Real>, eb, 256, 0, g, (const int*)ext_cells.p, n_ext, v3(RH_CELL_B), 0, imf_cell, bx, by, bz);
+        cell2node(RH_CELL_B, RH_NODE_B, 3);
+        cell2node(RH_CELL_RHOQ, RH_NODE_RHOQ, 1);
+        cell2node(RH_CELL_JI, RH_NODE_JI, 3);
+        launch("neumann", k_neumann_face<Real>, eb, 256, 0, g, (const int*)ext_cells.p, n_ext, v3(RH_FACE_B));
+        if (imf_face) launch("imf", k_imf<Real>, eb, 256, 0, g, (const int*)ext_cells.p, n_ext, v3(RH_FACE_B), 1, imf_face, bx, by, bz);
+        launch("node_J", k_node_J<Real>, cb, 256, 0, g, v3(RH_FACE_B), v3(RH_NODE_B), (const Real*)fld[RH_NODE_RHOQ].p,
+               v3(RH_NODE_J), (Real)P.max_whistler, dcount.p);
+        launch("node2cell", k_node2cell<Real>, cb, 256, 0, g, v3(RH_NODE_J), v3(RH_CELL_J));
+        launch("cell_Ue", k_ue<Real, false>, cb, 256, 0, g, v3(RH_CELL_J), v3(RH_CELL_JI), fld[RH_CELL_RHOQ].p,
+               (const uint8_t*)flg[RH_INNER_FIELD].p, v3(RH_CELL_UE), P.use_hall, (Real)P.obstacle_ue[0],
+               (Real)P.obstacle_ue[1], (Real)P.obstacle_ue[2], (Real)P.max_electron_speed2, (Real)P.min_rho, dcount.p);
+        neumann_cell(RH_CELL_UE, 3);
+        launch("node_Ue", k_ue<Real, true>, cb, 256, 0, g, v3(RH_NODE_J), v3(RH_NODE_JI), fld[RH_NODE_RHOQ].p,
+               (const uint8_t*)flg[RH_INNER_NODE].p, v3(RH_NODE_UE), P.use_hall, (Real)P.obstacle_ue[0],
+               (Real)P.obstacle_ue[1], (Real)P.obstacle_ue[2], (Real)P.max_electron_speed2, (Real)P.min_rho, dcount.p);
+        launch("upwind_B", k_upwind_B<Real>, cb, 256, 0, g, v3(RH_CELL_B), v3(RH_NODE_UE), v3(RH_NODE_B));
+        launch("node_E", k_node_E<Real>, cb, 256, 0, g, v3(RH_NODE_UE), v3(RH_NODE_B), (const Real*)fld[RH_NODE_ETA].p,
+               v3(RH_NODE_J), v3(RH_NODE_E), (Real)P.max_E2, dcount.p);
+        for (int it = 0; it < P.n_linear_smoothings; ++it) {
+            launch("node2cell", k_node2cell<Real>, cb, 256, 0, g, v3(RH_NODE_E), v3(RH_CELL_J));
+            neumann_cell(RH_CELL_J, 3);
+            cell2node(RH_CELL_J, RH_NODE_E, 3);
+            fill(fld[RH_CELL_J].p, Real(0), fld[RH_CELL_J].n);
+        }
+        if (P.use_gaussian) {
+            RH_HIP(hipMemcpyAsync(tmpv.p, fld[RH_NODE_E].p, sizeof(Real) * 3 * g.nc, hipMemcpyDeviceToDevice, stream));
+            V3<Real> old{tmpv.p, tmpv.p + g.nc, tmpv.p + 2 * (size_t)g.nc};
+            launch("node_avg", k_node_avg<Real>, cb, 256, 0, g, old, v3(RH_NODE_E), (Real)P.gauss_coeffs[0],
+                   (Real)P.gauss_coeffs[1], (Real)P.gauss_coeffs[2], (Real)P.gauss_coeffs[3]);
+        }
+        launch("face_curl", k_face_curl<Real>, cb, 256, 0, g, v3(RH_NODE_E), v3(RH_FACE_B), (Real)P.dt);
+        if (P.use_e_pressure)
+            launch("cell_Ep", k_cell_Ep<Real>, cb, 256, 0, g, (const Real*)fld[RH_NODE_RHOQ].p,
+                   (const Real*)fld[RH_CELL_RHOQ].p, (const uint8_t*)flg[RH_INNER_CELLEP].p, v3(RH_CELL_EP),
+                   (Real)P.e_pressure_coeff, P.isothermal);
+    }
+
+    void step(int n) override {
+        for (int it = 0; it < n; ++it) {
+            for (size_t s = 0; s < sp.size(); ++s) propagate((int)s);
+            clear_accumulation();
+            for (size_t s = 0; s < sp.size(); ++s) wait_particle_sends((int)s);
+            for (size_t s = 0; s < sp.size(); ++s) accumulate((int)s);
+            add_remote_updates();
+            for (size_t s = 0; s < sp.size(); ++s) apply_bcs((int)s);
+            propagate_B();
+        }
+    }
+
+    void diag_raw(int s, double* t) {
+        Species& S_ = S(s);
+        if (S_.pushed) throw StateError("diagnostics between propagate and wait_particle_sends");
+        launch("diag", k_diag<Real>, DIAG_BLOCKS, 256, 0, S_.A(), (const uint32_t*)(S_.sub.p + g.nkeys), diag_partial.p);
+        std::vector<double> h(DIAG_BLOCKS * 5);
+        RH_HIP(hipMemcpyAsync(h.data(), diag_partial.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
+        RH_HIP(hipStreamSynchronize(stream));
+        for (int q = 0; q < 5; ++q) t[q] = 0;
+        for (int b = 0; b < DIAG_BLOCKS; ++b)
+            for (int q = 0; q < 5; ++q) t[q] += h[b * 5 + q];
+    }
+
+    void diagnostics(int s, double* out) override {
+        double t[5];
+        diag_raw(s, t);
+        for (int q = 0; q < 5; ++q) out[q] = t[q] * S(s).d.w;
+    }
+
+    void counters(double* out) override {
+        unsigned long long h[C_N];
+        RH_HIP(hipMemcpyAsync(h, dcount.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+        RH_HIP(hipStreamSynchronize(stream));
+        for (int q = 0; q < RH_N_COUNTERS; ++q) out[q] = (double)h[q];
+        if (h[C_RANK_OVERFLOW]) throw CapError("particle sort overflow (a subcell slot exceeded 2^20 particles in one tile)");
+    }
+
+    void synchronize() override { RH_HIP(hipStreamSynchronize(stream)); }
+
+    void set_timing(bool on) override {
+        if (!on && timing) resolve_timers();
+        timing = on;
+    }
+
+    void kernel_time(const char* name, double* ms, int64_t* launches) override {
+        resolve_timers();
+        auto it = timers.find(name ? name : "");
+        if (ms) *ms = it == timers.end() ? 0.0 : it->second.ms;
+        if (launches) *launches = it == timers.end() ? 0 : it->second.launches;
+    }
+
+    void reset_timing() override {
+        resolve_timers();
+        timers.clear();
+    }
+
+    void layout(int64_t* out) override {
+        out[0] = g.ntiles; out[1] = g.tx; out[2] = g.ty; out[3] = g.tz; out[4] = g.nc;
+        out[5] = (int64_t)(6 * sizeof(Real) + sizeof(uint32_t));
+    }
+};
+
+}  // namespace rh
+
+struct rh_ctx {
+    std::unique_ptr<rh::Ctx> impl;
+    std::string err;
+};
+
+template <typename F>
+static int guarded(rh_ctx* ctx, F&& f) {
+    if (!ctx || !ctx->impl) return RH_ERR_ARG;
+    try {
+        f(*ctx->impl);
+        return RH_OK;
+    } catch (const rh::ArgError& e) {
+        ctx->err = e.what();
+        return RH_ERR_ARG;
+    } catch (const rh::CapError& e) {
+        ctx->err = e.what();
+        return RH_ERR_CAPACITY;
+    } catch (const rh::StateError& e) {
+        ctx->err = e.what();
+        return RH_ERR_STATE;
+    } catch (const rh::HipError& e) {
+        ctx->err = e.what();
+        return RH_ERR_HIP;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return RH_ERR_HIP;
+    }
+}
+
+extern "C" {
+
+int rh_create(const rh_grid_desc* grid, const rh_params* params, int32_t precision, rh_ctx** out) {
+    if (!grid || !params || !out) return RH_ERR_ARG;
+    *out = nullptr;
+    auto* c = new rh_ctx();
+    try {
+        if (precision == RH_FP32) c->impl.reset(new rh::Sim<float>(grid, params));
+        else if (precision == RH_FP64) c->impl.reset(new rh::Sim<double>(grid, params));
+        else throw rh::ArgError("precision must be RH_FP32 or RH_FP64");
+    } catch (const rh::ArgError& e) {
+        delete c;
+        return RH_ERR_ARG;
+    } catch (const std::exception& e) {
+        int dev = 0;
+        const bool nodev = hipGetDeviceCount(&dev) != hipSuccess || dev == 0;
+        delete c;
+        return nodev ? RH_ERR_NO_DEVICE : RH_ERR_HIP;
+    }
+    *out = c;
+    return RH_OK;
+}
+
+int rh_destroy(rh_ctx* ctx) {
+    if (!ctx) return RH_ERR_ARG;
+    delete ctx;
+    return RH_OK;
+}
+
+const char* rh_last_error(const rh_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int rh_add_species(rh_ctx* ctx, const rh_species_desc* desc, int32_t* id) {
+    if (!desc || !id) return RH_ERR_ARG;
+    return guarded(ctx, [&](rh::Ctx& c) { *id = c.add_species(desc); });
+}
+
+int rh_upload_field(rh_ctx* ctx, int32_t f, const double* d) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.upload_field(f, d); });
+}
+int rh_download_field(rh_ctx* ctx, int32_t f, double* d) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.download_field(f, d); });
+}
+int rh_upload_flags(rh_ctx* ctx, int32_t f, const uint8_t* d) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.upload_flags(f, d); });
+}
+int rh_upload_particles(rh_ctx* ctx, int32_t s, int64_t n, const int64_t* cell, const double* st) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.upload_particles(s, n, cell, st); });
+}
+int rh_num_particles(rh_ctx* ctx, int32_t s, int64_t* n) {
+    if (!n) return RH_ERR_ARG;
+    return guarded(ctx, [&](rh::Ctx& c) { *n = c.num_particles(s); });
+}
+int rh_download_particles(rh_ctx* ctx, int32_t s, int64_t cap, int64_t* cell, double* st) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.download_particles(s, cap, cell, st); });
+}
+int rh_setup_get_fields(rh_ctx* ctx) {
+    return guarded(ctx, [&](rh::Ctx&) {});
+}
+int rh_propagate(rh_ctx* ctx, int32_t s) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.propagate(s); });
+}
+int rh_wait_particle_sends(rh_ctx* ctx, int32_t s) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.wait_particle_sends(s); });
+}
+int rh_clear_accumulation_arrays(rh_ctx* ctx) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.clear_accumulation(); });
+}
+int rh_accumulate(rh_ctx* ctx, int32_t s) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.accumulate(s); });
+}
+int rh_add_remote_updates(rh_ctx* ctx) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.add_remote_updates(); });
+}
+int rh_apply_boundary_conditions(rh_ctx* ctx, int32_t s) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.apply_bcs(s); });
+}
+int rh_propagate_B(rh_ctx* ctx) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.propagate_B(); });
+}
+int rh_step(rh_ctx* ctx, int32_t n) {
+    if (n < 0) return RH_ERR_ARG;
+    return guarded(ctx, [&](rh::Ctx& c) { c.step(n); });
+}
+int rh_diagnostics(rh_ctx* ctx, int32_t s, double out[5]) {
+    if (!out) return RH_ERR_ARG;
+    return guarded(ctx, [&](rh::Ctx& c) { c.diagnostics(s, out); });
+}
+int rh_counters(rh_ctx* ctx, double out[RH_N_COUNTERS]) {
+    if (!out) return RH_ERR_ARG;
+    return guarded(ctx, [&](rh::Ctx& c) { c.counters(out); });
+}
+int rh_synchronize(rh_ctx* ctx) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.synchronize(); });
+}
+int rh_set_timing(rh_ctx* ctx, int32_t on) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.set_timing(on != 0); });
+}
+int rh_kernel_times(rh_ctx* ctx, const char* kernel, double* ms, int64_t* launches) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.kernel_time(kernel, ms, launches); });
+}
+int rh_reset_timing(rh_ctx* ctx) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.reset_timing(); });
+}
+int rh_layout_info(rh_ctx* ctx, int64_t out[6]) {
+    if (!out) return RH_ERR_ARG;
+    return guarded(ctx, [&](rh::Ctx& c) { c.layout(out); });
+}
+
+}  // extern "C"

@@ -1,0 +1,405 @@
+// rh_fields.hpp -- device kernels of propagateB (src/hybrid_propagator.cpp:
+// 42-440), variant USE_NODE_UE + USE_EDGE_J (src/Makefile:1-7).
+//
+// Grid arrays are structure of arrays: a vector field is one allocation of
+// 3*nc with component l at [l*nc + n].  One thread per cell; neighbours are
+// addressed through the periodic/missing topology of rh_geom.hpp, so every
+// kernel keeps the reference's rule for which cells it updates:
+//   cell-centred kernels only in cells with all 26 neighbours;
+//   node/face kernels skip cells missing a +x/+y/+z neighbour.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "rh_geom.hpp"
+#include "rh_particles.hpp"
+
+namespace rh {
+
+template <typename Real>
+struct V3 {
+    Real *x, *y, *z;
+    __device__ __forceinline__ Real& c(int l, int n) const { return l == 0 ? x[n] : (l == 1 ? y[n] : z[n]); }
+};
+
+#define RH_CELL_PROLOGUE                          \
+    const int n = blockIdx.x * 256 + threadIdx.x; \
+    if (n >= g.nc) return;                        \
+    int i, j, k;                                  \
+    cell_ijk(g, n, i, j, k);
+
+// face2Cell (:1043-1056)
+template <typename Real>
+__global__ __launch_bounds__(256) void k_face2cell(Geom g, V3<Real> face, V3<Real> cell) {
+    RH_CELL_PROLOGUE
+    if (!all_nbrs(g, i, j, k)) return;
+    cell.x[n] = Real(0.5) * (face.x[n] + face.x[nbr(g, i, j, k, -1, 0, 0)]);
+    cell.y[n] = Real(0.5) * (face.y[n] + face.y[nbr(g, i, j, k, 0, -1, 0)]);
+    cell.z[n] = Real(0.5) * (face.z[n] + face.z[nbr(g, i, j, k, 0, 0, -1)]);
+}
+
+// cell2Node (:1059-1089): node = mean of the 8 cells around the upper corner
+template <typename Real>
+__global__ __launch_bounds__(256) void k_cell2node(Geom g, const Real* __restrict__ cell, Real* __restrict__ node, int dim) {
+    RH_CELL_PROLOGUE
+    if (upper_missing(g, i, j, k)) return;
+    const int c001 = nbr(g, i, j, k, 0, 0, 1), c010 = nbr(g, i, j, k, 0, 1, 0), c100 = nbr(g, i, j, k, 1, 0, 0);
+    const int c011 = nbr(g, i, j, k, 0, 1, 1), c110 = nbr(g, i, j, k, 1, 1, 0), c101 = nbr(g, i, j, k, 1, 0, 1);
+    const int c111 = nbr(g, i, j, k, 1, 1, 1);
+    for (int l = 0; l < dim; ++l) {
+        const Real* a = cell + (size_t)l * g.nc;
+        node[(size_t)l * g.nc + n] =
+            Real(0.125) * (a[n] + a[c001] + a[c010] + a[c100] + a[c011] + a[c110] + a[c101] + a[c111]);
+    }
+}
+
+// node2Cell (:1092-1112): cell = mean of its 8 corner nodes
+template <typename Real>
+__global__ __launch_bounds__(256) void k_node2cell(Geom g, V3<Real> node, V3<Real> cell) {
+    RH_CELL_PROLOGUE
+    if (!all_nbrs(g, i, j, k)) return;
+    const int m100 = nbr(g, i, j, k, -1, 0, 0), m110 = nbr(g, i, j, k, -1, -1, 0), m010 = nbr(g, i, j, k, 0, -1, 0);
+    const int m001 = nbr(g, i, j, k, 0, 0, -1), m011 = nbr(g, i, j, k, 0, -1, -1), m101 = nbr(g, i, j, k, -1, 0, -1);
+    const int m111 = nbr(g, i, j, k, -1, -1, -1);
+    for (int l = 0; l < 3; ++l) {
+        const Real* a = l == 0 ? node.x : (l == 1 ? node.y : node.z);
+        (l == 0 ? cell.x : (l == 1 ? cell.y : cell.z))[n] =
+            Real(0.125) * (a[n] + a[m100] + a[m110] + a[m010] + a[m001] + a[m011] + a[m101] + a[m111]);
+    }
+}
+
+// neumannCell (:443-655).  Walls, then edges, then corners overwrite one
+// another, so an exterior cell ends with the value of the neighbour lying
+// opposite to all of its missing directions; that neighbour is never an
+// exterior cell, which makes the cells independent.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_neumann_cell(Geom g, const int* __restrict__ ext, int next,
+                                                     Real* __restrict__ D, int dim) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= next) return;
+    const int n = ext[q];
+    int i, j, k;
+    cell_ijk(g, n, i, j, k);
+    const int sx = has_neg(i, g.per[0]) ? (has_pos(i, g.nx, g.per[0]) ? 0 : -1) : 1;
+    const int sy = has_neg(j, g.per[1]) ? (has_pos(j, g.ny, g.per[1]) ? 0 : -1) : 1;
+    const int sz = has_neg(k, g.per[2]) ? (has_pos(k, g.nz, g.per[2]) ? 0 : -1) : 1;
+    const int src = nbr(g, i, j, k, sx, sy, sz);
+    if (src < 0) return;
+    for (int l = 0; l < dim; ++l) D[(size_t)l * g.nc + n] = D[(size_t)l * g.nc + src];
+}
+
+// neumannFace (:659-908): per-component copy rules in the reference's order;
+// for each component the last applicable rule wins.
+struct FaceRule { uint8_t need; int8_t comp, a, b, c; };
+// need bits: 1 -x, 2 +x, 4 -y, 8 +y, 16 -z, 32 +z (neighbours that must be missing)
+__constant__ FaceRule c_face_rules[48] = {
+    {1, 1, 1, 0, 0}, {1, 2, 1, 0, 0}, {8, 0, 0, -1, 0}, {8, 2, 0, -1, 0},
+    {4, 0, 0, 1, 0}, {4, 2, 0, 1, 0}, {32, 0, 0, 0, -1}, {32, 1, 0, 0, -1},
+    {16, 0, 0, 0, 1}, {16, 1, 0, 0, 1}, {2, 1, -1, 0, 0}, {2, 2, -1, 0, 0},
+    {1 | 4, 0, 0, 1, 0}, {1 | 4, 1, 1, 0, 0}, {1 | 4, 2, 1, 1, 0},
+    {1 | 8, 0, 0, -1, 0}, {1 | 8, 2, 1, -1, 0},
+    {2 | 4, 1, -1, 0, 0}, {2 | 4, 2, -1, 1, 0},
+    {2 | 8, 2, -1, -1, 0},
+    {1 | 16, 0, 0, 0, 1}, {1 | 16, 1, 1, 0, 1}, {1 | 16, 2, 1, 0, 0},
+    {1 | 32, 0, 0, 0, -1}, {1 | 32, 1, 1, 0, -1},
+    {2 | 16, 1, -1, 0, 1}, {2 | 16, 2, -1, 0, 0},
+    {2 | 32, 1, -1, 0, -1},
+    {4 | 16, 0, 0, 1, 1}, {4 | 16, 1, 0, 0, 1}, {4 | 16, 2, 0, 1, 0},
+    {4 | 32, 0, 0, 1, -1}, {4 | 32, 1, 0, 0, -1},
+    {8 | 16, 0, 0, -1, 1}, {8 | 16, 2, 0, -1, 0},
+    {8 | 32, 0, 0, -1, -1},
+    {1 | 4 | 16, 0, 0, 1, 1}, {1 | 4 | 16, 1, 1, 0, 1}, {1 | 4 | 16, 2, 1, 1, 0},
+    {1 | 4 | 32, 0, 0, 1, -1}, {1 | 4 | 32, 1, 1, 0, -1},
+    {1 | 8 | 16, 0, 0, -1, 1}, {1 | 8 | 16, 2, 1, -1, 0},
+    {1 | 8 | 32, 0, 0, -1, -1},
+    {2 | 4 | 16, 1, -1, 0, 1}, {2 | 4 | 16, 2, -1, 1, 0},
+    {2 | 4 | 32, 1, -1, 0, -1},
+    {2 | 8 | 16, 2, -1, -1, 0},
+};
+
+__device__ __forceinline__ int missing_mask(const Geom& g, int i, int j, int k) {
+    int m = 0;
+    if (!has_neg(i, g.per[0])) m |= 1;
+    if (!has_pos(i, g.nx, g.per[0])) m |= 2;
+    if (!has_neg(j, g.per[1])) m |= 4;
+    if (!has_pos(j, g.ny, g.per[1])) m |= 8;
+    if (!has_neg(k, g.per[2])) m |= 16;
+    if (!has_pos(k, g.nz, g.per[2])) m |= 32;
+    return m;
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_neumann_face(Geom g, const int* __restrict__ ext, int next, V3<Real> F) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= next) return;
+    const int n = ext[q];
+    int i, j, k;
+    cell_ijk(g, n, i, j, k);
+    const int miss = missing_mask(g, i, j, k);
+    for (int l = 0; l < 3; ++l) {
+        for (int r = 47; r >= 0; --r) {
+            const FaceRule R = c_face_rules[r];
+            if (R.comp != l || (miss & R.need) != R.need) continue;
+            const int src = nbr(g, i, j, k, R.a, R.b, R.c);
+            if (src >= 0) F.c(l, n) = F.c(l, src);
+            break;
+        }
+    }
+}
+
+// setIMFCell / setIMFFace (:910-1040); wall order +x -x +y -y +z -z
+template <typename Real>
+__global__ __launch_bounds__(256) void k_imf(Geom g, const int* __restrict__ ext, int next, V3<Real> D,
+                                            int face, int flags, Real bx, Real by, Real bz) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= next) return;
+    const int n = ext[q];
+    int i, j, k;
+    cell_ijk(g, n, i, j, k);
+    const int miss = missing_mask(g, i, j, k);
+    // missing-mask bit of each wall in the +x -x +y -y +z -z order
+    const int wall_bit[6] = {2, 1, 8, 4, 32, 16};
+    for (int w = 0; w < 6; ++w) {
+        if (!((flags >> w) & 1) || !(miss & wall_bit[w])) continue;
+        const int axis = w / 2;
+        if (!face || axis != 0) D.x[n] = bx;
+        if (!face || axis != 1) D.y[n] = by;
+        if (!face || axis != 2) D.z[n] = bz;
+    }
+}
+
+// calcNodeJ (:1449-1522): nodeJ = curl(faceB)/mu0 on edges averaged to the
+// node, divided by the whistler-limiter factor d.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_node_J(Geom g, V3<Real> F, V3<Real> nodeB, const Real* __restrict__ nodeRho,
+                                               V3<Real> nodeJ, Real max_vw, unsigned long long* counters) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    bool limited = false;
+    if (n < g.nc) {
+        int i, j, k;
+        cell_ijk(g, n, i, j, k);
+        if (!upper_missing(g, i, j, k)) {
+            const int c100 = nbr(g, i, j, k, 1, 0, 0), c010 = nbr(g, i, j, k, 0, 1, 0), c001 = nbr(g, i, j, k, 0, 0, 1);
+            const int c110 = nbr(g, i, j, k, 1, 1, 0), c101 = nbr(g, i, j, k, 1, 0, 1), c011 = nbr(g, i, j, k, 0, 1, 1);
+            const Real ex1 = -F.z[n] + F.y[n] + F.z[c010] - F.y[c001];
+            const Real ex2 = -F.z[c100] + F.y[c100] + F.z[c110] - F.y[c101];
+            const Real ey1 = +F.z[n] + F.x[c001] - F.z[c100] - F.x[n];
+            const Real ey2 = +F.z[c010] + F.x[c011] - F.z[c110] - F.x[c010];
+            const Real ez1 = -F.y[n] + F.x[n] + F.y[c100] - F.x[c010];
+            const Real ez2 = -F.y[c001] + F.x[c001] + F.y[c101] - F.x[c011];
+            const Real e = Real(1.60217653e-19);
+            const Real mu0 = Real(4.0e-7 * 3.14159265358979323846);
+            Real d = Real(1), vw = Real(0);
+            const Real ne = nodeRho[n] / e;
+            const Real Btot = sqrt(nodeB.x[n] * nodeB.x[n] + nodeB.y[n] * nodeB.y[n] + nodeB.z[n] * nodeB.z[n]);
+            const Real dx = (Real)g.dx;
+            if (ne > Real(0) && dx > Real(0))
+                vw = Real(2) * Btot * Real(3.14159265358979323846) / (mu0 * ne * e * dx);
+            if (vw > max_vw && max_vw > Real(0)) { d = vw / max_vw; limited = true; }
+            const Real a = Real(0.5) / (dx * mu0 * d);
+            nodeJ.x[n] = (ex1 + ex2) * a;
+            nodeJ.y[n] = (ey1 + ey2) * a;
+            nodeJ.z[n] = (ez1 + ez2) * a;
+        }
+    }
+    count_event(counters, C_MAX_VW, limited);
+}
+
+// calcCellUe (:1342-1387) and calcNodeUe (:1524-1584; NODE clamps rho in place)
+template <typename Real, bool NODE>
+__global__ __launch_bounds__(256) void k_ue(Geom g, V3<Real> J, V3<Real> Ji, Real* __restrict__ rho,
+                                           const uint8_t* __restrict__ inner, V3<Real> Ue, int use_hall,
+                                           Real ox_, Real oy_, Real oz_, Real max_ue2, Real min_rho,
+                                           unsigned long long* counters) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    bool c_ue = false, c_rho = false;
+    if (n < g.nc) {
+        int i, j, k;
+        cell_ijk(g, n, i, j, k);
+        const bool run = NODE ? !upper_missing(g, i, j, k) : all_nbrs(g, i, j, k);
+        if (run) {
+            if (inner[n]) {
+                Ue.x[n] = ox_; Ue.y[n] = oy_; Ue.z[n] = oz_;
+            } else {
+                Real r = rho[n];
+                if (NODE && r < min_rho) { r = min_rho; rho[n] = r; c_rho = true; }
+                Real u[3];
+                const Real jj[3] = {J.x[n], J.y[n], J.z[n]};
+                const Real ji[3] = {Ji.x[n], Ji.y[n], Ji.z[n]};
+                for (int l = 0; l < 3; ++l) {
+                    if (fabs(r) > Real(0)) u[l] = use_hall ? -(jj[l] - ji[l]) / r : ji[l] / r;
+                    else u[l] = Real(0);
+                }
+                const Real u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+                if (u2 > max_ue2) {
+                    const Real norm = sqrt(max_ue2 / u2);
+                    u[0] *= norm; u[1] *= norm; u[2] *= norm;
+                    c_ue = true;
+                }
+                Ue.x[n] = u[0]; Ue.y[n] = u[1]; Ue.z[n] = u[2];
+            }
+        }
+    }
+    count_event(counters, NODE ? C_MAX_NODE_UE : C_MAX_CELL_UE, c_ue);
+    if (NODE) count_event(counters, C_MIN_NODE_RHO, c_rho);
+}
+
+// upwindNodeB (:1268-1339): inverse-distance weights of the 8 cells around
+// the node, seen from a point displaced by -Ue/|Ue|/2 (default (-1/2,0,0)).
+template <typename Real>
+__global__ __launch_bounds__(256) void k_upwind_B(Geom g, V3<Real> cellB, V3<Real> Ue, V3<Real> nodeB) {
+    RH_CELL_PROLOGUE
+    if (upper_missing(g, i, j, k)) return;
+    Real xu = Real(-0.5), yu = Real(0), zu = Real(0);
+    const Real ue = sqrt(Ue.x[n] * Ue.x[n] + Ue.y[n] * Ue.y[n] + Ue.z[n] * Ue.z[n]);
+    if (ue > Real(0)) {
+        const Real a = Real(-0.5) / ue;
+        xu = Ue.x[n] * a; yu = Ue.y[n] * a; zu = Ue.z[n] * a;
+    }
+    // reference order 111,112,121,211,122,221,212,222 (offsets 0 or +1)
+    const int off[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}, {0, 1, 1}, {1, 1, 0}, {1, 0, 1}, {1, 1, 1}};
+    Real w[8];
+    int c[8];
+    Real wsum = Real(0);
+    for (int q = 0; q < 8; ++q) {
+        const Real xc = off[q][0] ? Real(0.5) : Real(-0.5);
+        const Real yc = off[q][1] ? Real(0.5) : Real(-0.5);
+        const Real zc = off[q][2] ? Real(0.5) : Real(-0.5);
+        w[q] = Real(1) / sqrt((xc - xu) * (xc - xu) + (yc - yu) * (yc - yu) + (zc - zu) * (zc - zu));
+        c[q] = nbr(g, i, j, k, off[q][0], off[q][1], off[q][2]);
+        wsum += w[q];
+    }
+    if (wsum > Real(0)) {
+        Real bx = 0, by = 0, bz = 0;
+        for (int q = 0; q < 8; ++q) {
+            bx += w[q] * cellB.x[c[q]];
+            by += w[q] * cellB.y[c[q]];
+            bz += w[q] * cellB.z[c[q]];
+        }
+        nodeB.x[n] = bx / wsum; nodeB.y[n] = by / wsum; nodeB.z[n] = bz / wsum;
+    } else {
+        nodeB.x[n] = nodeB.y[n] = nodeB.z[n] = Real(0);
+    }
+}
+
+// calcNodeE (:1391-1446): E = -Ue x B + eta J, |E| <= Emax
+template <typename Real>
+__global__ __launch_bounds__(256) void k_node_E(Geom g, V3<Real> Ue, V3<Real> B, const Real* __restrict__ eta,
+                                               V3<Real> J, V3<Real> E, Real max_E2, unsigned long long* counters) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    bool c = false;
+    if (n < g.nc) {
+        int i, j, k;
+        cell_ijk(g, n, i, j, k);
+        if (!upper_missing(g, i, j, k)) {
+            Real ex = -(Ue.y[n] * B.z[n] - Ue.z[n] * B.y[n]);
+            Real ey = -(Ue.z[n] * B.x[n] - Ue.x[n] * B.z[n]);
+            Real ez = -(Ue.x[n] * B.y[n] - Ue.y[n] * B.x[n]);
+            ex += eta[n] * J.x[n];
+            ey += eta[n] * J.y[n];
+            ez += eta[n] * J.z[n];
+            if (max_E2 > Real(0)) {
+                const Real e2 = ex * ex + ey * ey + ez * ez;
+                if (e2 > max_E2) {
+                    const Real s = sqrt(max_E2 / e2);
+                    ex *= s; ey *= s; ez *= s;
+                    c = true;
+                }
+            }
+            E.x[n] = ex; E.y[n] = ey; E.z[n] = ez;
+        }
+    }
+    count_event(counters, C_MAX_E, c);
+}
+
+// nodeAvg (:1115-1265).  The reference's gather buffer starts as
+// {DBL_MAX, 0, 0, ...}: only a missing (-1,-1,-1) neighbour's x component is
+// recognised as missing (weight 0 and renormalisation); other missing
+// neighbours count as zeros with full weight.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_node_avg(Geom g, V3<Real> Eold, V3<Real> E, Real c1, Real c2, Real c3, Real c4) {
+    RH_CELL_PROLOGUE
+    if (upper_missing(g, i, j, k)) return;
+    const Real C[4] = {c1, c2, c3, c4};
+    // reference summation order of the 27 nodes (nodeEl111, 110, 011, ...)
+    const int8_t ord[27][3] = {{0, 0, 0}, {0, 0, -1}, {-1, 0, 0}, {0, 1, 0}, {1, 0, 0}, {0, -1, 0}, {0, 0, 1},
+                               {-1, 0, -1}, {0, -1, -1}, {0, 1, -1}, {1, 0, -1}, {-1, -1, 0}, {-1, 1, 0}, {1, 1, 0},
+                               {1, -1, 0}, {-1, 0, 1}, {0, 1, 1}, {1, 0, 1}, {0, -1, 1}, {-1, -1, 1}, {-1, 1, 1},
+                               {1, -1, 1}, {1, 1, 1}, {-1, -1, -1}, {-1, 1, -1}, {1, -1, -1}, {1, 1, -1}};
+    for (int l = 0; l < 3; ++l) {
+        const Real* a = l == 0 ? Eold.x : (l == 1 ? Eold.y : Eold.z);
+        Real w[27], v[27];
+        bool zero_found = false;
+        for (int q = 0; q < 27; ++q) {
+            const int m = nbr(g, i, j, k, ord[q][0], ord[q][1], ord[q][2]);
+            const int dist = (ord[q][0] != 0) + (ord[q][1] != 0) + (ord[q][2] != 0);
+            w[q] = C[dist];
+            if (m >= 0) v[q] = a[m];
+            else v[q] = Real(0);
+            if (m < 0 && l == 0 && q == 23) { w[q] = Real(0); zero_found = true; }
+        }
+        if (zero_found) {
+            Real cs = Real(0);
+            for (int q = 0; q < 27; ++q) cs += w[q];
+            if (cs > Real(0))
+                for (int q = 0; q < 27; ++q) w[q] /= cs;
+        }
+        Real acc = Real(0);
+        for (int q = 0; q < 27; ++q) acc += w[q] * v[q];
+        (l == 0 ? E.x : (l == 1 ? E.y : E.z))[n] = acc;
+    }
+}
+
+// faceCurl, Faraday (:1587-1671): faceB -= dt curl(nodeE)
+template <typename Real>
+__global__ __launch_bounds__(256) void k_face_curl(Geom g, V3<Real> E, V3<Real> F, Real dt) {
+    RH_CELL_PROLOGUE
+    if (upper_missing(g, i, j, k)) return;
+    const bool xn = has_neg(i, g.per[0]), yn = has_neg(j, g.per[1]), zn = has_neg(k, g.per[2]);
+    const bool doX = yn && zn, doY = xn && zn, doZ = xn && yn;
+    const Real dx = (Real)g.dx;
+    // node1 (-1,-1,0), node3 (-1,0,-1), node4 (-1,0,0), node5 (0,-1,0),
+    // node6 (0,-1,-1), node7 (0,0,-1), node8 (0,0,0)
+    if (doX) {
+        const int n5 = nbr(g, i, j, k, 0, -1, 0), n6 = nbr(g, i, j, k, 0, -1, -1), n7 = nbr(g, i, j, k, 0, 0, -1);
+        const Real curl = Real(0.5) * (+E.z[n5] + E.z[n6] - E.y[n6] - E.y[n7] - E.z[n7] - E.z[n] + E.y[n] + E.y[n5]) / dx;
+        F.x[n] += dt * curl;
+    }
+    if (doY) {
+        const int n3 = nbr(g, i, j, k, -1, 0, -1), n4 = nbr(g, i, j, k, -1, 0, 0), n7 = nbr(g, i, j, k, 0, 0, -1);
+        const Real curl = Real(0.5) * (-E.x[n4] - E.x[n] + E.z[n] + E.z[n7] + E.x[n7] + E.x[n3] - E.z[n3] - E.z[n4]) / dx;
+        F.y[n] += dt * curl;
+    }
+    if (doZ) {
+        const int n1 = nbr(g, i, j, k, -1, -1, 0), n4 = nbr(g, i, j, k, -1, 0, 0), n5 = nbr(g, i, j, k, 0, -1, 0);
+        const Real curl = Real(0.5) * (-E.x[n1] - E.x[n5] - E.y[n5] - E.y[n] + E.x[n] + E.x[n4] + E.y[n4] + E.y[n1]) / dx;
+        F.z[n] += dt * curl;
+    }
+}
+
+// calcCellEp (:1674-1737): Ep = -coeff grad(rho)/rho (isothermal) or -coeff grad(rho)
+template <typename Real>
+__global__ __launch_bounds__(256) void k_cell_Ep(Geom g, const Real* __restrict__ nrho, const Real* __restrict__ crho,
+                                                const uint8_t* __restrict__ inner, V3<Real> Ep, Real coeff, int isothermal) {
+    RH_CELL_PROLOGUE
+    if (!all_nbrs(g, i, j, k)) return;
+    Ep.x[n] = Ep.y[n] = Ep.z[n] = Real(0);
+    if (inner[n]) return;
+    // node values at offsets (a,b,c) in {-1,0}^3
+    auto N = [&](int a, int b, int c) { return nrho[nbr(g, i, j, k, a, b, c)]; };
+    const Real xP = Real(0.25) * (N(0, 0, 0) + N(0, 0, -1) + N(0, -1, -1) + N(0, -1, 0));
+    const Real xN = Real(0.25) * (N(-1, 0, 0) + N(-1, 0, -1) + N(-1, -1, -1) + N(-1, -1, 0));
+    const Real yP = Real(0.25) * (N(0, 0, 0) + N(0, 0, -1) + N(-1, 0, -1) + N(-1, 0, 0));
+    const Real yN = Real(0.25) * (N(0, -1, 0) + N(0, -1, -1) + N(-1, -1, -1) + N(-1, -1, 0));
+    const Real zP = Real(0.25) * (N(0, 0, 0) + N(0, -1, 0) + N(-1, -1, 0) + N(-1, 0, 0));
+    const Real zN = Real(0.25) * (N(0, 0, -1) + N(0, -1, -1) + N(-1, -1, -1) + N(-1, 0, -1));
+    const Real dx = (Real)g.dx;
+    const Real gx = (xP - xN) / dx, gy = (yP - yN) / dx, gz = (zP - zN) / dx;
+    if (!isothermal) {
+        Ep.x[n] = -coeff * gx; Ep.y[n] = -coeff * gy; Ep.z[n] = -coeff * gz;
+    } else if (fabs(crho[n]) > Real(0)) {
+        Ep.x[n] = -coeff * gx / crho[n]; Ep.y[n] = -coeff * gy / crho[n]; Ep.z[n] = -coeff * gz / crho[n];
+    }
+}
+
+}  // namespace rh

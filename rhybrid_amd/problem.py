@@ -1,0 +1,311 @@
+"""Problem set-up for the hot path: grid, parameters, species, initial fields
+and seeded initial particles, in the reference's layouts.
+
+Everything here is host-side numpy.  The same ``Problem`` object is uploaded
+to the GPU context (``rhybrid_amd.gpu``) and, in the tests, to the FP64
+oracle, so both start from identical bytes.
+
+Derived quantities follow the reference's initialisation code:
+  macroparticle weight  w = n*dV/ppc          src/particle_injector.cpp:305,682
+  thermal speed         vth = sqrt(kB*T/m)    src/particle_injector.cpp:310,687
+  min ion charge dens.  rel * upstream rho_q  src/user.cpp:1779-1789
+  e-pressure coeff      kB*Te/e (gamma=1), 2*kB*Te/(ne*e^2) (gamma=2)
+                                              src/user.cpp:1579-1589
+  ion speed cap         min(max_ion_velocity, 0.9*dx/dt)  src/user.cpp:1774-1777
+  inner-boundary flags  src/user.cpp:1882-1912
+  initial B (constantB) faces and cells      src/user.cpp:2107-2141,
+                                              src/magnetic_field.h:32-36
+  uniform fill          src/particle_injector.cpp:187-240 (epsCell = 1e-4)
+"""
+from __future__ import annotations
+
+import dataclasses
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+# CODATA values (Corsair's constants table is not part of the reference tree).
+KB = 1.3806505e-23
+QE = 1.60217653e-19
+MP = 1.672621716e-27
+MU0 = 4.0e-7 * np.pi
+EPS_CELL = 1.0e-4
+
+FIELD_NAMES = ["faceB", "cellB", "cellJ", "cellUe", "cellJi", "cellEp", "cellRhoQi",
+               "nodeRhoQi", "nodeE", "nodeB", "nodeJ", "nodeUe", "nodeJi", "nodeEta"]
+FIELD_DIM = [3, 3, 3, 3, 3, 3, 1, 1, 3, 3, 3, 3, 3, 1]
+FLAG_NAMES = ["innerFlagField", "innerFlagNode", "innerFlagParticle", "innerFlagCellEp"]
+
+
+@dataclass
+class Grid:
+    nx: int
+    ny: int
+    nz: int
+    periodic: Tuple[int, int, int]
+    dx: float
+    x0: float
+    y0: float
+    z0: float
+
+    @property
+    def nc(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    @property
+    def dV(self) -> float:
+        return self.dx ** 3
+
+    def ijk(self):
+        n = np.arange(self.nc, dtype=np.int64)
+        return n % self.nx, (n // self.nx) % self.ny, n // (self.nx * self.ny)
+
+    def all_neighbours(self) -> np.ndarray:
+        """True where all 26 neighbours exist (not an exterior cell)."""
+        i, j, k = self.ijk()
+        ok = np.ones(self.nc, dtype=bool)
+        for d, (c, n) in enumerate(zip((i, j, k), (self.nx, self.ny, self.nz))):
+            if not self.periodic[d]:
+                ok &= (c >= 1) & (c <= n - 2)
+        return ok
+
+
+@dataclass
+class Params:
+    dt: float
+    use_hall: int = 1
+    use_e_pressure: int = 0
+    isothermal: int = 1
+    e_pressure_coeff: float = 0.0
+    max_ion_speed2: float = 1e20
+    max_electron_speed2: float = 1e20
+    min_rho: float = 0.0
+    max_E2: float = 0.0
+    max_whistler: float = 0.0
+    n_linear_smoothings: int = 0
+    use_gaussian: int = 0
+    gauss_coeffs: Tuple[float, float, float, float] = (0.0, 0.0, 0.0, 0.0)
+    imf: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    imf_cell: Tuple[int, int, int, int, int, int] = (0, 0, 0, 0, 0, 0)
+    imf_face: Tuple[int, int, int, int, int, int] = (0, 0, 0, 0, 0, 0)
+    obstacle_ue: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    initial_flow_through: int = 0
+    use_gravity: int = 0
+    GMdt: float = 0.0
+
+
+@dataclass
+class Species:
+    name: str
+    q: float
+    m: float
+    w: float
+    R2_obstacle: float = -1.0
+    accumulate: int = 1
+    accelerate: int = 1
+
+
+@dataclass
+class Particles:
+    """Reference layout: global cell index and AoS state[7] =
+    {x,y,z (cell-local, m), vx,vy,vz (m/s), w} (src/particle_definition.h:29)."""
+    cell: np.ndarray   # int64 [n]
+    state: np.ndarray  # float64 [n, 7]
+
+    @property
+    def n(self) -> int:
+        return int(self.cell.shape[0])
+
+
+@dataclass
+class Problem:
+    name: str
+    grid: Grid
+    params: Params
+    species: List[Species]
+    fields: Dict[str, np.ndarray]
+    flags: Dict[str, np.ndarray]
+    particles: List[Particles]
+    steps: int = 1
+    meta: Dict[str, object] = field(default_factory=dict)
+
+    def copy(self) -> "Problem":
+        return dataclasses.replace(
+            self,
+            fields={k: v.copy() for k, v in self.fields.items()},
+            flags={k: v.copy() for k, v in self.flags.items()},
+            particles=[Particles(p.cell.copy(), p.state.copy()) for p in self.particles],
+            meta=dict(self.meta))
+
+    @property
+    def n_particles(self) -> int:
+        return sum(p.n for p in self.particles)
+
+
+def gaussian_coeffs(sigma: float) -> Tuple[float, float, float, float]:
+    """Nodal E smoothing weights normalised over 27 nodes (src/user.cpp:1794-1807)."""
+    g = lambda d: np.exp(-0.5 * d * d / (sigma * sigma)) / (sigma * np.sqrt(2 * np.pi))
+    c = [g(0.0), g(1.0), g(np.sqrt(2.0)), g(np.sqrt(3.0))]
+    s = c[0] + 6 * c[1] + 12 * c[2] + 8 * c[3]
+    return tuple(float(x / s) for x in c)
+
+
+def make_fields(grid: Grid, B0=(0.0, 0.0, 0.0), eta: float = 0.0) -> Dict[str, np.ndarray]:
+    f = {n: np.zeros(grid.nc * d, dtype=np.float64) for n, d in zip(FIELD_NAMES, FIELD_DIM)}
+    # constantB profile on faces and cells (USE_B_INITIAL, src/user.cpp:2107-2141)
+    for l in range(3):
+        f["faceB"][l::3] = B0[l]
+        f["cellB"][l::3] = B0[l]
+    f["nodeEta"][:] = eta
+    return f
+
+
+def make_flags(grid: Grid, R_field: float = -1.0, R_particle: float = -1.0,
+               R_zero_pe: float = -1.0) -> Dict[str, np.ndarray]:
+    """Inner-boundary flags (src/user.cpp:1882-1912)."""
+    i, j, k = grid.ijk()
+    dx = grid.dx
+    xc = grid.x0 + (i + 0.5) * dx
+    yc = grid.y0 + (j + 0.5) * dx
+    zc = grid.z0 + (k + 0.5) * dx
+    r2 = xc * xc + yc * yc + zc * zc
+    rp2 = (np.sqrt(r2) - 0.5 * np.sqrt(3.0) * dx) ** 2
+    xn, yn, zn = xc + 0.5 * dx, yc + 0.5 * dx, zc + 0.5 * dx
+    rn2 = xn * xn + yn * yn + zn * zn
+    R2f = R_field * R_field if R_field > 0 else -1.0
+    R2p = R_particle * R_particle if R_particle > 0 else -1.0
+    R2e = R_zero_pe * R_zero_pe if R_zero_pe > 0 else 0.0
+    return {
+        "innerFlagField": (r2 < R2f).astype(np.uint8),
+        "innerFlagNode": (rn2 < R2f).astype(np.uint8),
+        "innerFlagParticle": (rp2 < R2p).astype(np.uint8),
+        "innerFlagCellEp": (rp2 < R2e).astype(np.uint8),
+    }
+
+
+def uniform_particles(grid: Grid, sp: Species, ppc: int, velocity, T: float,
+                      rng: np.random.Generator, cells: Optional[np.ndarray] = None,
+                      R_exclude: float = -1.0) -> Particles:
+    """Uniform Maxwellian fill, InjectorUniform::injectParticles
+    (src/particle_injector.cpp:187-240): positions uniform in the cell shrunk by
+    epsCell, velocities U + vth*N(0,1) per component."""
+    if cells is None:
+        cells = np.nonzero(grid.all_neighbours())[0]
+    cells = np.asarray(cells, dtype=np.int64)
+    n = cells.size * ppc
+    cell = np.repeat(cells, ppc)
+    dx = grid.dx
+    st = np.empty((n, 7), dtype=np.float64)
+    st[:, 0:3] = 0.5 * dx + (1.0 - EPS_CELL) * dx * (rng.random((n, 3)) - 0.5)
+    vth = np.sqrt(KB * T / sp.m) if T > 0 else 0.0
+    st[:, 3:6] = np.asarray(velocity, dtype=np.float64)[None, :] + vth * rng.standard_normal((n, 3))
+    st[:, 6] = sp.w
+    if R_exclude > 0:
+        i = cell % grid.nx
+        j = (cell // grid.nx) % grid.ny
+        k = cell // (grid.nx * grid.ny)
+        gx = grid.x0 + i * dx + st[:, 0]
+        gy = grid.y0 + j * dx + st[:, 1]
+        gz = grid.z0 + k * dx + st[:, 2]
+        keep = gx * gx + gy * gy + gz * gz >= R_exclude * R_exclude
+        cell, st = cell[keep], st[keep]
+    return Particles(cell, st)
+
+
+def _constraints(p: Params, dx: float, dt: float, max_vi: float, max_ve: float,
+                 rho_up: float, min_rho_rel: float, max_vw: float) -> None:
+    vi = max_vi if max_vi >= 0 else 1e10
+    if vi > dx / dt:
+        vi = 0.9 * dx / dt
+    p.max_ion_speed2 = vi * vi
+    ve = max_ve if max_ve >= 0 else 1e10
+    p.max_electron_speed2 = ve * ve
+    p.min_rho = max(min_rho_rel, 0.0) * rho_up
+    p.max_whistler = max_vw
+
+
+def config_periodic(n: int = 32, ppc: int = 64, steps: int = 100, seed: int = 6461674,
+                    dx: float = 100e3, dt: float = 0.01) -> Problem:
+    """Plain method, small: periodic box of n^3 cells, dx=100 km, dt=0.01 s, H+
+    at n=3 cm^-3 with 64 ppc, drifting Maxwellian v=(-400,0,0) km/s, T=1e5 K,
+    B=(0,3,0) nT, isothermal electrons Te=1e5 K, eta=0."""
+    grid = Grid(n, n, n, (1, 1, 1), dx, -0.5 * n * dx, -0.5 * n * dx, -0.5 * n * dx)
+    dens, T, Te = 3e6, 1e5, 1e5
+    U = (-400e3, 0.0, 0.0)
+    B0 = (0.0, 3e-9, 0.0)
+    sp = Species("H+", QE, MP, dens * grid.dV / ppc)
+    p = Params(dt=dt, use_hall=1, use_e_pressure=1, isothermal=1,
+               e_pressure_coeff=KB * Te / QE, imf=B0)
+    # constraints as in examples/empty_box_periodic.cfg [Constraints]
+    _constraints(p, dx, dt, 1500e3, 1500e3, dens * QE, 0.05, 1500e3)
+    rng = np.random.default_rng(seed)
+    parts = uniform_particles(grid, sp, ppc, U, T, rng)
+    return Problem("periodic%d_ppc%d" % (n, ppc), grid, p, [sp], make_fields(grid, B0),
+                   make_flags(grid), [parts], steps=steps,
+                   meta={"ppc": ppc, "density": dens, "U": U, "T": T})
+
+
+def config_obstacle(nx: int = 128, ny: int = 96, nz: int = 96, ppc: int = 32,
+                    steps: int = 500, seed: int = 6461674, dx: float = 300e3,
+                    dt: float = 0.04, R_cells: float = 10.0, he_fraction: float = 0.04,
+                    n_sw: float = 3e6) -> Problem:
+    """Inflow/outflow box with an absorbing spherical obstacle of radius
+    R_cells cells at the centre; H+ and He++ (he_fraction by number) solar
+    wind, ppc each, drifting Maxwellians at 450 km/s, T=1e5 K, IMF
+    B=(-1.5,2.5,0) nT, IMF on the +x wall (as examples/mars.cfg [IMF]).
+    Choices the workload description leaves open: n_sw=3 cm^-3 total,
+    dt=0.04 s, isothermal electrons at Te=1e5 K, eta=0, uniform preload of
+    the interior (the reference's InjectorUniform) and no further inflow."""
+    grid = Grid(nx, ny, nz, (0, 0, 0), dx, -0.5 * nx * dx, -0.5 * ny * dx, -0.5 * nz * dx)
+    T, Te = 1e5, 1e5
+    U = (-450e3, 0.0, 0.0)
+    B0 = (-1.5e-9, 2.5e-9, 0.0)
+    R = R_cells * dx
+    nH, nHe = n_sw * (1 - he_fraction), n_sw * he_fraction
+    spH = Species("H+", QE, MP, nH * grid.dV / ppc, R2_obstacle=R * R)
+    spHe = Species("He++", 2 * QE, 3.97 * MP, nHe * grid.dV / ppc, R2_obstacle=R * R)
+    rho_up = (nH + 2 * nHe) * QE
+    p = Params(dt=dt, use_hall=1, use_e_pressure=1, isothermal=1,
+               e_pressure_coeff=KB * Te / QE, imf=B0,
+               imf_cell=(1, 0, 0, 0, 0, 0), imf_face=(1, 0, 0, 0, 0, 0))
+    _constraints(p, dx, dt, 1500e3, 1500e3, rho_up, 0.05, 1500e3)
+    rng = np.random.default_rng(seed)
+    flags = make_flags(grid, R_field=R, R_particle=R)
+    pH = uniform_particles(grid, spH, ppc, U, T, rng, R_exclude=R)
+    pHe = uniform_particles(grid, spHe, ppc, U, T, rng, R_exclude=R)
+    return Problem("obstacle%dx%dx%d_ppc%d" % (nx, ny, nz, ppc), grid, p, [spH, spHe],
+                   make_fields(grid, B0), flags, [pH, pHe], steps=steps,
+                   meta={"ppc": ppc, "density": n_sw, "U": U, "T": T, "R_cells": R_cells})
+
+
+def perturb(problem: Problem, seed: int = 1, scale: float = 0.2) -> Problem:
+    """Add a smooth, divergence-free perturbation to faceB (as a curl of a
+    random node-centred vector potential, so div B stays zero) and a random
+    resistivity, so that every term of the field update is exercised."""
+    g = problem.grid
+    rng = np.random.default_rng(seed)
+    Bmag = max(np.abs(problem.fields["faceB"]).max(), 1e-9)
+    # vector potential on cell edges, approximated by node values of a smooth field
+    i, j, k = g.ijk()
+    A = np.zeros((3, g.nc))
+    for l in range(3):
+        kx, ky, kz = rng.integers(1, 3, size=3)
+        ph = rng.random(3) * 2 * np.pi
+        A[l] = np.sin(2 * np.pi * kx * i / g.nx + ph[0]) * np.cos(2 * np.pi * ky * j / g.ny + ph[1]) \
+            * np.sin(2 * np.pi * kz * k / g.nz + ph[2])
+    A *= scale * Bmag * g.dx
+
+    def sh(a, d, s):
+        shape = (g.nz, g.ny, g.nx)
+        a3 = a.reshape(shape)
+        return np.roll(a3, -s, axis=2 - d).reshape(-1)
+    # B = curl A on faces: Bx(face +x of cell) = dAz/dy - dAy/dz with A on x-edges/nodes
+    dB = np.zeros((3, g.nc))
+    dB[0] = (A[2] - sh(A[2], 1, -1)) / g.dx - (A[1] - sh(A[1], 2, -1)) / g.dx
+    dB[1] = (A[0] - sh(A[0], 2, -1)) / g.dx - (A[2] - sh(A[2], 0, -1)) / g.dx
+    dB[2] = (A[1] - sh(A[1], 0, -1)) / g.dx - (A[0] - sh(A[0], 1, -1)) / g.dx
+    out = problem.copy()
+    for l in range(3):
+        out.fields["faceB"][l::3] += dB[l]
+    return out
