@@ -1,0 +1,195 @@
+"""Benchmark of the hot path on one MI355X.
+
+Workload: 128x96x96-cell inflow/outflow box (dx=300 km) with an absorbing
+sphere of radius 10 cells, H+ and He++ (4% by number) at 32 particles per
+cell each (~71 M macroparticles), drifting Maxwellians at 450 km/s, IMF
+(-1.5, 2.5, 0) nT, FP32 cell-local particles.  One step = push + cell sort +
+CIC deposit + FP64 ghost fold + particle boundary conditions + field update
+(propagate() of the reference, src/user.cpp:118-134, without injection).
+
+value = particle-steps per second over the whole timed region (all of the
+step, not only push+deposit); roofline = the kernel with the largest total
+time, algorithmic bytes per launch over its mean launch time from HIP events
+on the library's stream; cpu_baseline = the FP64 oracle on the host cores on
+a smaller box of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+
+
+def workload(scale: int = 1, seed: int = 6461674):
+    from rhybrid_amd import problem as P
+    return P.config_obstacle(nx=128 // scale, ny=96 // scale, nz=96 // scale, ppc=32,
+                             R_cells=10.0 / scale, seed=seed)
+
+
+def kernel_bytes(name: str, ctx, np_total: float, nspecies: int) -> float:
+    """Algorithmic HBM bytes of one launch of a kernel (FP32 store: 6 floats
+    per particle), for the mean species population."""
+    lay = ctx.layout()
+    nc, ntiles = lay["ncells"], lay["ntiles"]
+    next_ = (lay["tx"] + 2) * (lay["ty"] + 2) * (lay["tz"] + 2)
+    extk = next_ * 8
+    nkeys = ntiles * lay["tx"] * lay["ty"] * lay["tz"] * 8
+    npp = np_total / nspecies  # particles per species launch (mean)
+    if name == "push":       # pass 0: read 6 f32; 9 f32 fields per cell; write slot counts
+        return 24.0 * npp + 36.0 * nc + 4.0 * extk * ntiles + 4.0 * nkeys
+    if name == "move":       # pass 1: read 6 f32, write 6 f32 at the sorted slot; fields; slot bases
+        return 48.0 * npp + 36.0 * nc + 4.0 * extk * ntiles + 4.0 * nkeys
+    if name == "deposit":    # read 6 f32 per particle, subcell starts; read+write FP64 tile ghosts
+        return 24.0 * npp + 4.0 * nkeys + 64.0 * next_ * ntiles
+    if name == "fold":       # read FP64 tile ghosts once, write rho + J (f32)
+        return 32.0 * next_ * ntiles + 16.0 * nc
+    if name in ("sort_count", "sort_base"):
+        return 4.0 * extk * ntiles * (2 if name == "sort_base" else 1) + 4.0 * nkeys
+    return 0.0
+
+
+def gpu_bench(args):
+    from rhybrid_amd.gpu import RH_FP32, Context
+    t0 = time.time()
+    pr = workload()
+    t_gen = time.time() - t0
+    n0 = pr.n_particles
+    t0 = time.time()
+    ctx = Context(pr, precision=RH_FP32)
+    t_up = time.time() - t0
+    print("# workload %s: %d cells, %d particles (gen %.1fs, upload %.1fs)" %
+          (pr.name, pr.grid.nc, n0, t_gen, t_up), file=sys.stderr, flush=True)
+    del pr
+    ctx.step(args.warmup)
+    ctx.synchronize()
+    nspec = len(ctx.species)
+    np_before = sum(ctx.num_particles(s) for s in range(nspec))
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(args.steps)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.set_timing(False)
+    np_after = sum(ctx.num_particles(s) for s in range(nspec))
+    np_mean = 0.5 * (np_before + np_after)
+    names = ["push", "sort_count", "scan", "sort_base", "move", "deposit", "fold", "boundary",
+             "face2cell", "neumann", "imf", "cell2node", "node_J", "node2cell", "cell_Ue", "node_Ue",
+             "upwind_B", "node_E", "node_avg", "face_curl", "cell_Ep", "fill", "diag"]
+    ktimes = {}
+    for nm in names:
+        ms, n = ctx.kernel_time(nm)
+        if n:
+            ktimes[nm] = (ms, n)
+    total_k = sum(v[0] for v in ktimes.values())
+    pd_names = ("push", "sort_count", "scan", "sort_base", "move", "deposit", "fold")
+    t_pd = sum(ktimes[k][0] for k in pd_names if k in ktimes) / 1e3
+    dom = max(ktimes, key=lambda k: ktimes[k][0])
+    per_kernel = {}
+    for nm, (ms, n) in ktimes.items():
+        b = kernel_bytes(nm, ctx, np_mean, nspec)
+        avg_s = ms / 1e3 / n
+        per_kernel[nm] = {"ms_total": round(ms, 3), "launches": n, "us_avg": round(avg_s * 1e6, 2),
+                          "GBps": round(b / avg_s / 1e9, 1) if b > 0 else None}
+    dom_bytes = kernel_bytes(dom, ctx, np_mean, nspec)
+    dom_avg = ktimes[dom][0] / 1e3 / ktimes[dom][1]
+    achieved = dom_bytes / dom_avg / 1e9
+    lay = ctx.layout()
+    counters = ctx.counters()
+    ctx.close()
+    # the push+deposit part of the step against its algorithmic bytes
+    # (56 B per particle-step + 52 B per cell-step per species, FP32)
+    pd_bytes = 56.0 * np_mean + 52.0 * lay["ncells"] * nspec
+    res = {
+        "value": np_mean * args.steps / wall,
+        "ms_per_step": wall / args.steps * 1e3,
+        "np": np_mean,
+        "wall_s": wall,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "push_deposit": {"particle_steps_per_s": np_mean * args.steps / t_pd if t_pd > 0 else None,
+                         "ms_per_step": t_pd / args.steps * 1e3,
+                         "effective_GBps_vs_56B_per_particle": round(pd_bytes * args.steps / t_pd / 1e9, 1)},
+        "kernels": per_kernel,
+        "kernel_ms_sum_per_step": total_k / args.steps,
+        "tiles": [lay["tx"], lay["ty"], lay["tz"]],
+        "dropped_particles": counters.get("dropped", 0),
+    }
+    return res
+
+
+def cpu_baseline(threads: int):
+    """FP64 oracle (the reference algorithm in the reference's data layout),
+    on a 64x48x48-cell box of the same workload (1/8 of the particles), timed
+    over 2 steps after 1 warm-up step."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    O.build()
+    pr = workload(scale=2)
+    o = O.Oracle(pr, threads=threads)
+    o.step(1)
+    n = sum(o.num_particles(s) for s in range(o.nspecies))
+    t0 = time.perf_counter()
+    steps = 2
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": "FP64 oracle, 64x48x48-cell box of the same workload (%d particles), %d steps, "
+                      "full step incl. field update" % (n, steps)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one domain per GPU; "
+              "the halo and migration exchange between GPUs is not built).", file=sys.stderr)
+        sys.exit(2)
+    res = gpu_bench(args)
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(threads)
+    line = {
+        "metric": "particle-steps/s (push+deposit) & step ms at matched n/J error vs FP64 oracle",
+        "value": res["value"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "128x96x96-cell inflow/outflow box, dx=300 km, absorbing sphere R=10 cells, "
+                               "H+ and He++ (4%) at 32 ppc each, 450 km/s, T=1e5 K, IMF (-1.5,2.5,0) nT; "
+                               "full step (push, cell sort, CIC deposit, FP64 ghost fold, particle BCs, "
+                               "field update), no injection",
+                   "particles": res["np"], "cells": 128 * 96 * 96, "tile": res["tiles"],
+                   "parallelism": "single GPU"},
+        "roofline": res["roofline"],
+        "cpu_baseline": cpu,
+        "push_deposit": res["push_deposit"],
+        "kernels": res["kernels"],
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -105,14 +105,24 @@ public:
 template <typename Real>
 class Sim final : public Ctx {
 public:
+    // Per species: the sorted store (a, sub) and the destination of the next
+    // sort (d, sub_new), swapped after every sort; cnt_tile holds the slot
+    // counts of pass 0 and then the slot bases of pass 1.
     struct Species {
         rh_species_desc d;
         size_t cap = 0;
-        DevBuf<Real> a[6], b[6];
-        DevBuf<uint32_t> cell, packed, sub, sub_new, cnt_tile;
-        bool pushed = false;  // B/packed hold a push not yet sorted
-        PArrays<Real> A() { return {a[0].p, a[1].p, a[2].p, a[3].p, a[4].p, a[5].p, cell.p}; }
-        PArrays<Real> B() { return {b[0].p, b[1].p, b[2].p, b[3].p, b[4].p, b[5].p, nullptr}; }
+        DevBuf<Real> a[6], dst[6];
+        DevBuf<uint32_t> sub, sub_new, cnt_tile;
+        DevBuf<Real> partial;  // deposit: per-tile ghost partials (ntiles x next x 4)
+        bool deposited = false;
+        bool pushed = false;  // pass 0 done, pass 1 pending
+        PushParams pp{};      // parameters of the pending push
+        PArrays<Real> A() { return {a[0].p, a[1].p, a[2].p, a[3].p, a[4].p, a[5].p}; }
+        PArrays<Real> Dst() { return {dst[0].p, dst[1].p, dst[2].p, dst[3].p, dst[4].p, dst[5].p}; }
+        void swap_stores() {
+            for (int q = 0; q < 6; ++q) std::swap(a[q].p, dst[q].p);
+            std::swap(sub.p, sub_new.p);
+        }
     };
 
     Geom g{};
@@ -123,7 +133,6 @@ public:
     DevBuf<uint8_t> flg[RH_N_FLAGS];
     DevBuf<int> ext_cells, bc_cells;
     int n_ext = 0, n_bc = 0;
-    DevBuf<double> tile_acc;
     DevBuf<unsigned long long> dcount;
     DevBuf<double> diag_partial;
     std::vector<std::unique_ptr<Species>> sp;
@@ -168,10 +177,8 @@ public:
         ext_cells.alloc(std::max<size_t>(ext.size(), 1));
         if (n_ext) RH_HIP(hipMemcpy(ext_cells.p, ext.data(), ext.size() * sizeof(int), hipMemcpyHostToDevice));
         rebuild_bc_cells();
-        tile_acc.alloc((size_t)g.ntiles * g.next * 4);
-        fill(tile_acc.p, 0.0, tile_acc.n);
-        dcount.alloc(C_N);
-        RH_HIP(hipMemsetAsync(dcount.p, 0, sizeof(unsigned long long) * C_N, stream));
+        dcount.alloc((size_t)C_N * COUNTER_SLOTS);
+        RH_HIP(hipMemsetAsync(dcount.p, 0, sizeof(unsigned long long) * C_N * COUNTER_SLOTS, stream));
         diag_partial.alloc(DIAG_BLOCKS * 5);
         zero_u32.alloc(1);
         RH_HIP(hipMemsetAsync(zero_u32.p, 0, sizeof(uint32_t), stream));
@@ -223,9 +230,11 @@ public:
             }
         }
         for (int d = 0; d < 3; ++d)
-            if (t[d] < 1 || dims[d] % t[d]) throw ArgError("tile shape must divide the grid");
+            if (t[d] < 1 || (t[d] & (t[d] - 1)) || dims[d] % t[d])
+                throw ArgError("tile sides must be powers of two that divide the grid");
         if (t[0] * t[1] * t[2] > max_cells) throw ArgError("tile too large for this precision");
         g.tx = t[0]; g.ty = t[1]; g.tz = t[2];
+        g.ltx = __builtin_ctz(g.tx); g.lty = __builtin_ctz(g.ty); g.ltz = __builtin_ctz(g.tz);
         g.tc = g.tx * g.ty * g.tz;
         g.ntx = g.nx / g.tx; g.nty = g.ny / g.ty; g.ntz = g.nz / g.tz;
         g.ntiles = g.ntx * g.nty * g.ntz;
@@ -278,15 +287,15 @@ public:
     }
 
     template <typename K, typename... Args>
-    void launch(const char* name, K kernel, unsigned grid, unsigned block, size_t shmem, Args... args) {
-        if (grid == 0) return;
+    void launch(const char* name, K kernel, dim3 grid, dim3 block, size_t shmem, Args... args) {
+        if ((size_t)grid.x * grid.y * grid.z == 0) return;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timing) {
             e0 = get_event();
             e1 = get_event();
             RH_HIP(hipEventRecord(e0, stream));
         }
-        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, stream, args...);
+        hipLaunchKernelGGL(kernel, grid, block, shmem, stream, args...);
         RH_HIP(hipGetLastError());
         if (timing) {
             RH_HIP(hipEventRecord(e1, stream));
@@ -305,6 +314,9 @@ public:
     }
 
     static unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+    dim3 tile_grid() const { return dim3(g.ntx, g.nty, g.ntz); }
+    dim3 cell_grid() const { return dim3(cdiv(g.nx, 32), cdiv(g.ny, 8), g.nz); }
+    static dim3 cell_block() { return dim3(32, 8, 1); }
 
     V3<Real> v3(int id) {
         Real* p = fld[id].p;
@@ -325,6 +337,7 @@ public:
         s->sub_new.alloc((size_t)g.nkeys + 1);
         RH_HIP(hipMemsetAsync(s->sub.p, 0, sizeof(uint32_t) * ((size_t)g.nkeys + 1), stream));
         s->cnt_tile.alloc((size_t)g.ntiles * g.extk);
+        s->partial.alloc((size_t)g.ntiles * g.next * 4);
         ensure_capacity(*s, (size_t)std::max<int64_t>(d->capacity, 0));
         sp.push_back(std::move(s));
         RH_HIP(hipStreamSynchronize(stream));
@@ -335,9 +348,7 @@ public:
         if (cap <= s.cap && s.cap > 0) return;
         if (cap >= 0xFFFFFFFFull) throw CapError("particle capacity exceeds 32-bit indices");
         cap = std::max<size_t>(cap, 64);
-        for (int q = 0; q < 6; ++q) { s.a[q].alloc(cap); s.b[q].alloc(cap); }
-        s.cell.alloc(cap);
-        s.packed.alloc(cap);
+        for (int q = 0; q < 6; ++q) { s.a[q].alloc(cap); s.dst[q].alloc(cap); }
         s.cap = cap;
     }
 
@@ -401,19 +412,16 @@ public:
         std::vector<uint32_t> fillp(sub.begin(), sub.end() - 1);
         std::vector<Real> h[6];
         for (int q = 0; q < 6; ++q) h[q].resize((size_t)n);
-        std::vector<uint32_t> hc((size_t)n);
         for (int64_t p = 0; p < n; ++p) {
             const uint32_t d = fillp[key[p]]++;
             h[0][d] = u[0][p]; h[1][d] = u[1][p]; h[2][d] = u[2][p];
             h[3][d] = (Real)st[7 * p + 3]; h[4][d] = (Real)st[7 * p + 4]; h[5][d] = (Real)st[7 * p + 5];
-            hc[d] = (uint32_t)cell[p];
         }
         size_t cap = (size_t)std::max<int64_t>(S_.d.capacity, 0);
         if (cap < (size_t)n) cap = (size_t)n + (size_t)n / 2;
         ensure_capacity(S_, cap);
         for (int q = 0; q < 6; ++q)
             RH_HIP(hipMemcpyAsync(S_.a[q].p, h[q].data(), sizeof(Real) * n, hipMemcpyHostToDevice, stream));
-        RH_HIP(hipMemcpyAsync(S_.cell.p, hc.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, stream));
         RH_HIP(hipMemcpyAsync(S_.sub.p, sub.data(), sizeof(uint32_t) * sub.size(), hipMemcpyHostToDevice, stream));
         RH_HIP(hipStreamSynchronize(stream));
         S_.pushed = false;
@@ -429,19 +437,29 @@ public:
 
     void download_particles(int s, int64_t capn, int64_t* cell, double* st) override {
         Species& S_ = S(s);
-        uint32_t np = 0;
-        RH_HIP(hipMemcpyAsync(&np, S_.sub.p + g.nkeys, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        if (S_.pushed) throw StateError("download between propagate and wait_particle_sends");
+        std::vector<uint32_t> sub((size_t)g.nkeys + 1);
+        RH_HIP(hipMemcpyAsync(sub.data(), S_.sub.p, sizeof(uint32_t) * sub.size(), hipMemcpyDeviceToHost, stream));
         RH_HIP(hipStreamSynchronize(stream));
+        const uint32_t np = sub[g.nkeys];
         std::vector<Real> h[6];
-        std::vector<uint32_t> hc(np);
         for (int q = 0; q < 6; ++q) {
             h[q].resize(np);
             if (np) RH_HIP(hipMemcpy(h[q].data(), S_.a[q].p, sizeof(Real) * np, hipMemcpyDeviceToHost));
         }
-        if (np) RH_HIP(hipMemcpy(hc.data(), S_.cell.p, sizeof(uint32_t) * np, hipMemcpyDeviceToHost));
+        // the cell of a particle follows from the subcell range holding it
+        std::vector<uint32_t> hc(np);
+        for (uint32_t key = 0; key < g.nkeys; ++key) {
+            if (sub[key] == sub[key + 1]) continue;
+            const int T = (int)(key / (uint32_t)(g.tc * 8)), lc = (int)((key % (uint32_t)(g.tc * 8)) >> 3);
+            int ox, oy, oz;
+            tile_origin(g, T, ox, oy, oz);
+            const uint32_t c = (uint32_t)cell_index(g, ox + lc % g.tx, oy + (lc / g.tx) % g.ty, oz + lc / (g.tx * g.ty));
+            for (uint32_t p = sub[key]; p < sub[key + 1]; ++p) hc[p] = c;
+        }
         int64_t out = 0;
         for (uint32_t p = 0; p < np; ++p) {
-            if (hc[p] == DEAD_CELL) continue;
+            if (h[0][p] < Real(0)) continue;
             if (out >= capn) throw CapError("download buffer too small");
             if (cell) cell[out] = hc[p];
             if (st) {
@@ -471,8 +489,9 @@ public:
         pp.initial_flow_through = P.initial_flow_through;
         const V3<Real> fb = v3(RH_FACE_B), ue = v3(RH_CELL_UE), ep = v3(RH_CELL_EP);
         PushFields<Real> pf{fb.x, fb.y, fb.z, ue.x, ue.y, ue.z, ep.x, ep.y, ep.z};
-        launch("push", k_push_rank<Real>, g.ntiles, 64, sizeof(uint32_t) * g.extk, g, pp, pf, S_.A(), S_.B(),
-               S_.packed.p, (const uint32_t*)S_.sub.p, S_.cnt_tile.p, dcount.p, kbits);
+        launch("push", k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, pp, pf,
+               S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.cnt_tile.p, dcount.p, kbits, 0, (uint32_t)S_.cap);
+        S_.pp = pp;
         S_.pushed = true;
     }
 
@@ -492,31 +511,43 @@ public:
     void wait_particle_sends(int s) override {
         Species& S_ = S(s);
         if (!S_.pushed) throw StateError("wait_particle_sends without a preceding propagate");
-        const unsigned kb = cdiv(g.nkeys, 256);
-        launch("sort_count", k_slot_offsets<false>, kb, 256, 0, g, S_.cnt_tile.p, S_.sub_new.p);
+        launch("sort_count", k_slot_offsets<false>, tile_grid(), g.tc * 8, 0, g, S_.cnt_tile.p, S_.sub_new.p);
         scan(S_.sub_new.p, g.nkeys, S_.sub_new.p, 0);
-        launch("sort_base", k_slot_offsets<true>, kb, 256, 0, g, S_.cnt_tile.p, S_.sub_new.p);
-        launch("scatter", k_scatter<Real>, g.ntiles, 64, 0, g, S_.B(), (const uint32_t*)S_.packed.p,
-               (const uint32_t*)S_.cnt_tile.p, (const uint32_t*)S_.sub.p, S_.A(), (uint32_t)S_.cap, dcount.p);
-        std::swap(S_.sub.p, S_.sub_new.p);
+        launch("sort_base", k_slot_offsets<true>, tile_grid(), g.tc * 8, 0, g, S_.cnt_tile.p, S_.sub_new.p);
+        const V3<Real> fb = v3(RH_FACE_B), ue = v3(RH_CELL_UE), ep = v3(RH_CELL_EP);
+        PushFields<Real> pf{fb.x, fb.y, fb.z, ue.x, ue.y, ue.z, ep.x, ep.y, ep.z};
+        launch("move", k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, S_.pp, pf,
+               S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.cnt_tile.p, dcount.p, kbits, 1, (uint32_t)S_.cap);
+        S_.swap_stores();
         S_.pushed = false;
     }
 
-    void clear_accumulation() override { fill(tile_acc.p, 0.0, tile_acc.n); }
+    void clear_accumulation() override {
+        for (auto& s_ : sp) s_->deposited = false;
+    }
 
     void accumulate(int s) override {
         Species& S_ = S(s);
         if (!S_.d.accumulate) return;
         if (S_.pushed) throw StateError("accumulate between propagate and wait_particle_sends");
         const int NS = g.tc * 8;
-        launch("deposit", k_deposit<Real>, g.ntiles, NS, sizeof(Real) * NS * 32, g, S_.A(),
-               (const uint32_t*)S_.sub.p, S_.d.w * S_.d.q, tile_acc.p);
+        launch("deposit", k_deposit<Real>, tile_grid(), NS, deposit_lds_bytes<Real>(g), g, S_.A(),
+               (const uint32_t*)S_.sub.p, S_.partial.p);
+        S_.deposited = true;
     }
 
     void add_remote_updates() override {
         const V3<Real> ji = v3(RH_CELL_JI);
-        launch("fold", k_fold<Real>, cdiv(g.nc, 256), 256, 0, g, (const double*)tile_acc.p,
-               1.0 / (g.dx * g.dx * g.dx), P.min_rho, fld[RH_CELL_RHOQ].p, ji.x, ji.y, ji.z, dcount.p);
+        FoldSources<Real> src{};
+        for (auto& s_ : sp) {
+            if (!s_->deposited) continue;
+            if (src.n == MAX_FOLD_SPECIES) throw ArgError("too many accumulated species");
+            src.partial[src.n] = s_->partial.p;
+            src.wq[src.n] = s_->d.w * s_->d.q;
+            src.n++;
+        }
+        launch("fold", k_fold<Real>, cell_grid(), cell_block(), 0, g, src, 1.0 / (g.dx * g.dx * g.dx), P.min_rho,
+               fld[RH_CELL_RHOQ].p, ji.x, ji.y, ji.z, dcount.p);
     }
 
     void apply_bcs(int s) override {
@@ -532,17 +563,20 @@ public:
     }
 
     void cell2node(int from, int to, int dim) {
-        launch("cell2node", k_cell2node<Real>, cdiv(g.nc, 256), 256, 0, g, (const Real*)fld[from].p, fld[to].p, dim);
+        launch("cell2node", k_cell2node<Real>, cell_grid(), cell_block(), 0, g, (const Real*)fld[from].p, fld[to].p, dim);
     }
 
     void propagate_B() override {
-        const unsigned cb = cdiv(g.nc, 256), eb = cdiv(n_ext, 256);
+        for (auto& sp_ : sp)
+            if (sp_->pushed) throw StateError("propagate_B between propagate and wait_particle_sends");
+        const dim3 cb = cell_grid(), bb = cell_block();
+        const unsigned eb = cdiv(n_ext, 256);
         const int imf_cell = P.imf_cell[0] | P.imf_cell[1] << 1 | P.imf_cell[2] << 2 | P.imf_cell[3] << 3 |
                              P.imf_cell[4] << 4 | P.imf_cell[5] << 5;
         const int imf_face = P.imf_face[0] | P.imf_face[1] << 1 | P.imf_face[2] << 2 | P.imf_face[3] << 3 |
                              P.imf_face[4] << 4 | P.imf_face[5] << 5;
         const Real bx = (Real)P.imf[0], by = (Real)P.imf[1], bz = (Real)P.imf[2];
-        launch("face2cell", k_face2cell<Real>, cb, 256, 0, g, v3(RH_FACE_B), v3(RH_CELL_B));
+        launch("face2cell", k_face2cell<Real>, cb, bb, 0, g, v3(RH_FACE_B), v3(RH_CELL_B));
         neumann_cell(RH_CELL_B, 3);
         neumann_cell(RH_CELL_JI, 3);
         neumann_cell(RH_CELL_RHOQ, 1);
@@ -552,21 +586,21 @@ public:
         cell2node(RH_CELL_JI, RH_NODE_JI, 3);
         launch("neumann", k_neumann_face<Real>, eb, 256, 0, g, (const int*)ext_cells.p, n_ext, v3(RH_FACE_B));
         if (imf_face) launch("imf", k_imf<Real>, eb, 256, 0, g, (const int*)ext_cells.p, n_ext, v3(RH_FACE_B), 1, imf_face, bx, by, bz);
-        launch("node_J", k_node_J<Real>, cb, 256, 0, g, v3(RH_FACE_B), v3(RH_NODE_B), (const Real*)fld[RH_NODE_RHOQ].p,
+        launch("node_J", k_node_J<Real>, cb, bb, 0, g, v3(RH_FACE_B), v3(RH_NODE_B), (const Real*)fld[RH_NODE_RHOQ].p,
                v3(RH_NODE_J), (Real)P.max_whistler, dcount.p);
-        launch("node2cell", k_node2cell<Real>, cb, 256, 0, g, v3(RH_NODE_J), v3(RH_CELL_J));
-        launch("cell_Ue", k_ue<Real, false>, cb, 256, 0, g, v3(RH_CELL_J), v3(RH_CELL_JI), fld[RH_CELL_RHOQ].p,
+        launch("node2cell", k_node2cell<Real>, cb, bb, 0, g, v3(RH_NODE_J), v3(RH_CELL_J));
+        launch("cell_Ue", k_ue<Real, false>, cb, bb, 0, g, v3(RH_CELL_J), v3(RH_CELL_JI), fld[RH_CELL_RHOQ].p,
                (const uint8_t*)flg[RH_INNER_FIELD].p, v3(RH_CELL_UE), P.use_hall, (Real)P.obstacle_ue[0],
                (Real)P.obstacle_ue[1], (Real)P.obstacle_ue[2], (Real)P.max_electron_speed2, (Real)P.min_rho, dcount.p);
         neumann_cell(RH_CELL_UE, 3);
-        launch("node_Ue", k_ue<Real, true>, cb, 256, 0, g, v3(RH_NODE_J), v3(RH_NODE_JI), fld[RH_NODE_RHOQ].p,
+        launch("node_Ue", k_ue<Real, true>, cb, bb, 0, g, v3(RH_NODE_J), v3(RH_NODE_JI), fld[RH_NODE_RHOQ].p,
                (const uint8_t*)flg[RH_INNER_NODE].p, v3(RH_NODE_UE), P.use_hall, (Real)P.obstacle_ue[0],
                (Real)P.obstacle_ue[1], (Real)P.obstacle_ue[2], (Real)P.max_electron_speed2, (Real)P.min_rho, dcount.p);
-        launch("upwind_B", k_upwind_B<Real>, cb, 256, 0, g, v3(RH_CELL_B), v3(RH_NODE_UE), v3(RH_NODE_B));
-        launch("node_E", k_node_E<Real>, cb, 256, 0, g, v3(RH_NODE_UE), v3(RH_NODE_B), (const Real*)fld[RH_NODE_ETA].p,
+        launch("upwind_B", k_upwind_B<Real>, cb, bb, 0, g, v3(RH_CELL_B), v3(RH_NODE_UE), v3(RH_NODE_B));
+        launch("node_E", k_node_E<Real>, cb, bb, 0, g, v3(RH_NODE_UE), v3(RH_NODE_B), (const Real*)fld[RH_NODE_ETA].p,
                v3(RH_NODE_J), v3(RH_NODE_E), (Real)P.max_E2, dcount.p);
         for (int it = 0; it < P.n_linear_smoothings; ++it) {
-            launch("node2cell", k_node2cell<Real>, cb, 256, 0, g, v3(RH_NODE_E), v3(RH_CELL_J));
+            launch("node2cell", k_node2cell<Real>, cb, bb, 0, g, v3(RH_NODE_E), v3(RH_CELL_J));
             neumann_cell(RH_CELL_J, 3);
             cell2node(RH_CELL_J, RH_NODE_E, 3);
             fill(fld[RH_CELL_J].p, Real(0), fld[RH_CELL_J].n);
@@ -574,12 +608,12 @@ public:
         if (P.use_gaussian) {
             RH_HIP(hipMemcpyAsync(tmpv.p, fld[RH_NODE_E].p, sizeof(Real) * 3 * g.nc, hipMemcpyDeviceToDevice, stream));
             V3<Real> old{tmpv.p, tmpv.p + g.nc, tmpv.p + 2 * (size_t)g.nc};
-            launch("node_avg", k_node_avg<Real>, cb, 256, 0, g, old, v3(RH_NODE_E), (Real)P.gauss_coeffs[0],
+            launch("node_avg", k_node_avg<Real>, cb, bb, 0, g, old, v3(RH_NODE_E), (Real)P.gauss_coeffs[0],
                    (Real)P.gauss_coeffs[1], (Real)P.gauss_coeffs[2], (Real)P.gauss_coeffs[3]);
         }
-        launch("face_curl", k_face_curl<Real>, cb, 256, 0, g, v3(RH_NODE_E), v3(RH_FACE_B), (Real)P.dt);
+        launch("face_curl", k_face_curl<Real>, cb, bb, 0, g, v3(RH_NODE_E), v3(RH_FACE_B), (Real)P.dt);
         if (P.use_e_pressure)
-            launch("cell_Ep", k_cell_Ep<Real>, cb, 256, 0, g, (const Real*)fld[RH_NODE_RHOQ].p,
+            launch("cell_Ep", k_cell_Ep<Real>, cb, bb, 0, g, (const Real*)fld[RH_NODE_RHOQ].p,
                    (const Real*)fld[RH_CELL_RHOQ].p, (const uint8_t*)flg[RH_INNER_CELLEP].p, v3(RH_CELL_EP),
                    (Real)P.e_pressure_coeff, P.isothermal);
     }
@@ -615,9 +649,12 @@ public:
     }
 
     void counters(double* out) override {
-        unsigned long long h[C_N];
-        RH_HIP(hipMemcpyAsync(h, dcount.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+        std::vector<unsigned long long> hs((size_t)C_N * COUNTER_SLOTS);
+        RH_HIP(hipMemcpyAsync(hs.data(), dcount.p, sizeof(unsigned long long) * hs.size(), hipMemcpyDeviceToHost, stream));
         RH_HIP(hipStreamSynchronize(stream));
+        unsigned long long h[C_N] = {};
+        for (int sl = 0; sl < COUNTER_SLOTS; ++sl)
+            for (int q = 0; q < C_N; ++q) h[q] += hs[(size_t)sl * C_N + q];
         for (int q = 0; q < RH_N_COUNTERS; ++q) out[q] = (double)h[q];
         if (h[C_RANK_OVERFLOW]) throw CapError("particle sort overflow (a subcell slot exceeded 2^20 particles in one tile)");
     }

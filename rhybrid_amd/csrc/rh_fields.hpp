@@ -22,11 +22,17 @@ struct V3 {
     __device__ __forceinline__ Real& c(int l, int n) const { return l == 0 ? x[n] : (l == 1 ? y[n] : z[n]); }
 };
 
-#define RH_CELL_PROLOGUE                          \
-    const int n = blockIdx.x * 256 + threadIdx.x; \
-    if (n >= g.nc) return;                        \
-    int i, j, k;                                  \
-    cell_ijk(g, n, i, j, k);
+// 3D launch: blocks of 32x8 cells in x,y, one z layer per grid row
+#define RH_CELL_IJK                                      \
+    const int i = blockIdx.x * 32 + threadIdx.x;         \
+    const int j = blockIdx.y * 8 + threadIdx.y;          \
+    const int k = blockIdx.z;                            \
+    const bool in_grid = i < g.nx && j < g.ny;           \
+    const int n = in_grid ? cell_index(g, i, j, k) : 0;
+
+#define RH_CELL_PROLOGUE \
+    RH_CELL_IJK          \
+    if (!in_grid) return;
 
 // face2Cell (:1043-1056)
 template <typename Real>
@@ -173,11 +179,9 @@ __global__ __launch_bounds__(256) void k_imf(Geom g, const int* __restrict__ ext
 template <typename Real>
 __global__ __launch_bounds__(256) void k_node_J(Geom g, V3<Real> F, V3<Real> nodeB, const Real* __restrict__ nodeRho,
                                                V3<Real> nodeJ, Real max_vw, unsigned long long* counters) {
-    const int n = blockIdx.x * 256 + threadIdx.x;
+    RH_CELL_IJK
     bool limited = false;
-    if (n < g.nc) {
-        int i, j, k;
-        cell_ijk(g, n, i, j, k);
+    if (in_grid) {
         if (!upper_missing(g, i, j, k)) {
             const int c100 = nbr(g, i, j, k, 1, 0, 0), c010 = nbr(g, i, j, k, 0, 1, 0), c001 = nbr(g, i, j, k, 0, 0, 1);
             const int c110 = nbr(g, i, j, k, 1, 1, 0), c101 = nbr(g, i, j, k, 1, 0, 1), c011 = nbr(g, i, j, k, 0, 1, 1);
@@ -211,11 +215,9 @@ __global__ __launch_bounds__(256) void k_ue(Geom g, V3<Real> J, V3<Real> Ji, Rea
                                            const uint8_t* __restrict__ inner, V3<Real> Ue, int use_hall,
                                            Real ox_, Real oy_, Real oz_, Real max_ue2, Real min_rho,
                                            unsigned long long* counters) {
-    const int n = blockIdx.x * 256 + threadIdx.x;
+    RH_CELL_IJK
     bool c_ue = false, c_rho = false;
-    if (n < g.nc) {
-        int i, j, k;
-        cell_ijk(g, n, i, j, k);
+    if (in_grid) {
         const bool run = NODE ? !upper_missing(g, i, j, k) : all_nbrs(g, i, j, k);
         if (run) {
             if (inner[n]) {
@@ -286,11 +288,9 @@ __global__ __launch_bounds__(256) void k_upwind_B(Geom g, V3<Real> cellB, V3<Rea
 template <typename Real>
 __global__ __launch_bounds__(256) void k_node_E(Geom g, V3<Real> Ue, V3<Real> B, const Real* __restrict__ eta,
                                                V3<Real> J, V3<Real> E, Real max_E2, unsigned long long* counters) {
-    const int n = blockIdx.x * 256 + threadIdx.x;
+    RH_CELL_IJK
     bool c = false;
-    if (n < g.nc) {
-        int i, j, k;
-        cell_ijk(g, n, i, j, k);
+    if (in_grid) {
         if (!upper_missing(g, i, j, k)) {
             Real ex = -(Ue.y[n] * B.z[n] - Ue.z[n] * B.y[n]);
             Real ey = -(Ue.z[n] * B.x[n] - Ue.x[n] * B.z[n]);

@@ -40,7 +40,8 @@ struct Geom {
     int per[3];
     double dx;
     double x0, y0, z0;
-    int tx, ty, tz, tc;        // tile shape, cells per tile
+    int tx, ty, tz, tc;        // tile shape (powers of two), cells per tile
+    int ltx, lty, ltz;         // log2 of the tile shape
     int ntx, nty, ntz, ntiles; // tiles per axis
     int ex, ey, ez, next;      // extended region shape and cell count
     int extk;                  // next * 8 subcell slots in the extended region
@@ -89,7 +90,14 @@ RH_HD bool upper_missing(const Geom& g, int i, int j, int k) {
 }
 
 RH_HD int tile_of(const Geom& g, int i, int j, int k) {
-    return i / g.tx + g.ntx * (j / g.ty + g.nty * (k / g.tz));
+    return (i >> g.ltx) + g.ntx * ((j >> g.lty) + g.nty * (k >> g.ltz));
+}
+
+// local cell index inside a tile -> local coordinates
+RH_HD void local_ijk(const Geom& g, int lc, int& li, int& lj, int& lk) {
+    li = lc & (g.tx - 1);
+    lj = (lc >> g.ltx) & (g.ty - 1);
+    lk = lc >> (g.ltx + g.lty);
 }
 
 RH_HD void tile_origin(const Geom& g, int T, int& ox, int& oy, int& oz) {
@@ -100,18 +108,19 @@ RH_HD void tile_origin(const Geom& g, int T, int& ox, int& oy, int& oz) {
 
 RH_HD uint32_t cell_key(const Geom& g, int i, int j, int k) {
     const int T = tile_of(g, i, j, k);
-    const int lc = (i % g.tx) + g.tx * ((j % g.ty) + g.ty * (k % g.tz));
+    const int lc = (i & (g.tx - 1)) + ((j & (g.ty - 1)) << g.ltx) + ((k & (g.tz - 1)) << (g.ltx + g.lty));
     return (uint32_t)(T * g.tc + lc);
 }
 
 // Candidate (tile index along the axis, extended coordinate 0..t+1) pairs
 // whose extended region holds coordinate i of an axis of n cells cut into nt
-// tiles of t cells.  Visited in a fixed order: tile offset -1,0,+1, then
+// tiles of t = 2^lt cells.  Visited in a fixed order: tile offset -1,0,+1, then
 // representative -n,0,+n (the same tile can hold a periodic cell twice when
 // there are fewer than three tiles along the axis).  Returns the count (<=6).
-RH_HD int axis_candidates(int i, int n, int t, int nt, int per, int* tl, int* el) {
+RH_HD int axis_candidates(int i, int n, int lt, int nt, int per, int* tl, int* el) {
+    const int t = 1 << lt;
     int m = 0;
-    const int ti = i / t;
+    const int ti = i >> lt;
     int seen[3];
     int ns = 0;
     for (int d = -1; d <= 1; ++d) {
@@ -124,7 +133,7 @@ RH_HD int axis_candidates(int i, int n, int t, int nt, int per, int* tl, int* el
         for (int q = 0; q < ns; ++q) dup = dup || (seen[q] == tt);
         if (dup) continue;
         seen[ns++] = tt;
-        const int l0 = i - tt * t;
+        const int l0 = i - (tt << lt);
         const int nrep = per ? 3 : 1;
         for (int r = 0; r < nrep; ++r) {
             const int l = per ? l0 + (r - 1) * n : l0;

@@ -4,11 +4,11 @@
 // cross-tile ghost fold, particle boundary conditions and FP64 diagnostics.
 //
 // Storage (per species, structure of arrays, Real = float or double):
-//   x,y,z    cell-local position in units of dx, in [0,1)
+//   x,y,z    cell-local position in units of dx, in [0,1); x = -1 marks a
+//            particle removed by the boundary conditions
 //   vx,vy,vz velocity in m/s
-//   cell     global cell index n, DEAD_CELL once removed
 // sorted by subcell key (rh_geom.hpp); sub[k] .. sub[k+1] is the range of
-// subcell key k.
+// subcell key k, so a particle's cell is implied by its index.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,7 +20,6 @@ namespace rh {
 template <typename Real>
 struct PArrays {
     Real *x, *y, *z, *vx, *vy, *vz;
-    uint32_t* cell;
 };
 
 template <typename Real>
@@ -41,125 +40,187 @@ struct PushParams {
 enum DevCounter { C_MAX_VI = 0, C_MIN_CELL_RHO, C_MAX_CELL_UE, C_MIN_NODE_RHO, C_MAX_NODE_UE,
                   C_MAX_E, C_MAX_VW, C_DROPPED, C_RANK_OVERFLOW, C_N };
 
+// Counters are spread over COUNTER_SLOTS copies (summed when read) so that
+// waves of different blocks do not all hit one address.
+constexpr int COUNTER_SLOTS = 64;
+
 __device__ __forceinline__ void count_event(unsigned long long* counters, int id, bool pred) {
     // one atomic per wave: integer, so the total does not depend on order
     const unsigned long long b = __ballot(pred);
-    if (b && (threadIdx.x & 63) == (unsigned)__builtin_ctzll(b))
-        atomicAdd(counters + id, (unsigned long long)__popcll(b));
+    const unsigned lane = (threadIdx.x + blockDim.x * threadIdx.y) & 63;
+    if (b && lane == (unsigned)__builtin_ctzll(b)) {
+        const unsigned slot = (blockIdx.x + 7u * blockIdx.y + 13u * blockIdx.z) & (COUNTER_SLOTS - 1);
+        atomicAdd(counters + slot * C_N + id, (unsigned long long)__popcll(b));
+    }
 }
 
 template <typename Real> __device__ __forceinline__ Real below_one();
 template <> __device__ __forceinline__ float below_one<float>() { return 0x1.fffffep-1f; }
 template <> __device__ __forceinline__ double below_one<double>() { return 0x1.fffffffffffffp-1; }
 
-// One workgroup (one wave) per tile.  Pushes the tile's particles
-// (BorisBuneman::propagate, src/particle_propagator_boris_buneman.h:64-194,
-// with getFields/face2r/cell2r, src/hybrid_propagator.cpp:1740-1891), moves
-// them in FP64 (the cell crossing), and ranks each particle among the
-// particles of the tile that go to the same subcell slot of the tile's
-// extended region, in memory order.  Writes the pushed state to B at the same
-// index, packed (slot << 20 | rank), and the tile's slot counts.
+// The push and the cell migration, one workgroup of four waves per old tile,
+// run twice with the same compiled body so that both passes compute bit-for-
+// bit the same pushed state, destination slot and rank:
+//   pass 0: BorisBuneman::propagate (src/particle_propagator_boris_buneman.h:
+//           64-194, fields by face2r/cell2r, src/hybrid_propagator.cpp:
+//           1740-1891), the move in FP64, and the destination slot in the
+//           tile's extended region; counts particles per slot;
+//   pass 1: the same, then writes each particle to (slot base + rank), the
+//           slot bases being the scan of all tiles' counts.
+// The tile's particles are taken 256 at a time; the rank of a particle is its
+// position among the tile's particles with the same slot in memory order (the
+// migration is a stable counting sort).  Within a chunk, each wave ranks its
+// lanes with ballots and publishes its per-slot group sizes; a lane's rank
+// adds the slot's running count and the group sizes of the earlier waves.
+// A particle's cell comes from the tile's subcell ranges (binary search in
+// LDS); the tile's fields are staged in LDS; removed particles carry x < 0.
+constexpr int PM_THREADS = 256;
+constexpr int PM_WAVES = PM_THREADS / 64;
+
+RH_HD size_t pm_align8(size_t b) { return (b + 7) & ~size_t(7); }
+
 template <typename Real>
-__global__ __launch_bounds__(64) void k_push_rank(Geom g, PushParams pp, PushFields<Real> f,
-                                                 PArrays<Real> A, PArrays<Real> B,
-                                                 uint32_t* __restrict__ packed,
-                                                 const uint32_t* __restrict__ sub,
-                                                 uint32_t* __restrict__ cnt_tile,
-                                                 unsigned long long* __restrict__ counters, int kbits) {
-    extern __shared__ uint32_t cnt[];
-    const int T = blockIdx.x;
-    const int lane = threadIdx.x;
-    for (int e = lane; e < g.extk; e += 64) cnt[e] = 0;
+RH_HD size_t push_move_lds_bytes(const Geom& g) {
+    const size_t ns = (size_t)g.tc * 8;
+    const size_t nh = (size_t)(g.tx + 1) * (g.ty + 1) * (g.tz + 1);
+    return pm_align8(4 * (size_t)g.extk + 4 * (ns + 1) + (size_t)PM_WAVES * g.extk) +
+           sizeof(Real) * (3 * nh + 6 * (size_t)g.tc);
+}
+
+template <typename Real>
+__global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp, PushFields<Real> f,
+                                                         PArrays<Real> A, PArrays<Real> D,
+                                                         const uint32_t* __restrict__ sub,
+                                                         uint32_t* __restrict__ cnt_tile,
+                                                         unsigned long long* __restrict__ counters, int kbits,
+                                                         int pass, uint32_t cap) {
+    extern __shared__ unsigned char pm_smem[];
+    const int NS = g.tc * 8;
+    const int hx = g.tx + 1, hy = g.ty + 1, nh = (g.tx + 1) * (g.ty + 1) * (g.tz + 1);
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(pm_smem);     // extk running slot counts
+    uint32_t* ss = cnt + g.extk;                               // NS+1 subcell starts
+    uint8_t* grp = reinterpret_cast<uint8_t*>(ss + NS + 1);    // PM_WAVES x extk group sizes
+    Real* fl = reinterpret_cast<Real*>(pm_smem + pm_align8(4 * (size_t)g.extk + 4 * (size_t)(NS + 1) +
+                                                          (size_t)PM_WAVES * g.extk));
+    Real* sbx = fl;            // faceB of local cells -1..t-1 (halo on the low side)
+    Real* sby = fl + nh;
+    Real* sbz = fl + 2 * nh;
+    Real* sue = fl + 3 * nh;   // cellUe, cellEp of the tile's cells
+    Real* sep = sue + 3 * g.tc;
+    const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int ox = blockIdx.x << g.ltx, oy = blockIdx.y << g.lty, oz = blockIdx.z << g.ltz;
+    for (int e = tid; e < g.extk; e += PM_THREADS) cnt[e] = 0;
+    for (int e = tid; e < PM_WAVES * g.extk; e += PM_THREADS) grp[e] = 0;
+    for (int q = tid; q <= NS; q += PM_THREADS) ss[q] = sub[(size_t)T * NS + q];
+    for (int q = tid; q < nh; q += PM_THREADS) {
+        const int a = q % hx - 1, b = (q / hx) % hy - 1, c = q / (hx * hy) - 1;
+        const int n = nbr(g, ox, oy, oz, a, b, c);
+        sbx[q] = n >= 0 ? f.fbx[n] : Real(0);
+        sby[q] = n >= 0 ? f.fby[n] : Real(0);
+        sbz[q] = n >= 0 ? f.fbz[n] : Real(0);
+    }
+    for (int q = tid; q < g.tc; q += PM_THREADS) {
+        int li, lj, lk;
+        local_ijk(g, q, li, lj, lk);
+        const int n = cell_index(g, ox + li, oy + lj, oz + lk);
+        sue[q] = f.uex[n]; sue[g.tc + q] = f.uey[n]; sue[2 * g.tc + q] = f.uez[n];
+        if (pp.use_e_pressure) { sep[q] = f.epx[n]; sep[g.tc + q] = f.epy[n]; sep[2 * g.tc + q] = f.epz[n]; }
+        else { sep[q] = sep[g.tc + q] = sep[2 * g.tc + q] = Real(0); }
+    }
     __syncthreads();
-    int ox, oy, oz;
-    tile_origin(g, T, ox, oy, oz);
-    const uint32_t p0 = sub[(size_t)T * g.tc * 8];
-    const uint32_t p1 = sub[(size_t)(T + 1) * g.tc * 8];
+    const uint32_t p0 = ss[0], p1 = ss[NS];
     const uint32_t deadkey = (uint32_t)MAX_EXTK;
-    for (uint32_t base = p0; base < p1; base += 64) {
-        const uint32_t p = base + lane;
+    // prefetch of the first chunk
+    Real nX = 0, nY = 0, nZ = 0, nvx = 0, nvy = 0, nvz = 0;
+    {
+        const uint32_t p = p0 + tid;
+        if (p < p1) { nX = A.x[p]; nY = A.y[p]; nZ = A.z[p]; nvx = A.vx[p]; nvy = A.vy[p]; nvz = A.vz[p]; }
+    }
+    for (uint32_t base = p0; base < p1; base += PM_THREADS) {
+        const uint32_t p = base + tid;
         const bool act = p < p1;
+        Real X = nX, Y = nY, Z = nZ, vx = nvx, vy = nvy, vz = nvz;
+        {
+            const uint32_t pn = p + PM_THREADS;
+            if (pn < p1) { nX = A.x[pn]; nY = A.y[pn]; nZ = A.z[pn]; nvx = A.vx[pn]; nvy = A.vy[pn]; nvz = A.vz[pn]; }
+        }
         uint32_t key = deadkey;
         bool clamped = false, dropped = false;
-        if (act) {
-            const uint32_t n = A.cell[p];
-            Real X = A.x[p], Y = A.y[p], Z = A.z[p];
-            Real vx = A.vx[p], vy = A.vy[p], vz = A.vz[p];
-            if (n != DEAD_CELL) {
-                int i, j, k;
-                cell_ijk(g, (int)n, i, j, k);
-                bool accel = pp.accelerate && all_nbrs(g, i, j, k) && !pp.initial_flow_through;
-                if (accel) {
-                    const int nmx = nbr(g, i, j, k, -1, 0, 0);
-                    const int nmy = nbr(g, i, j, k, 0, -1, 0);
-                    const int nmz = nbr(g, i, j, k, 0, 0, -1);
-                    // face2r: each component linear along its own axis only
-                    const Real Bx = X * f.fbx[n] + (Real(1) - X) * f.fbx[nmx];
-                    const Real By = Y * f.fby[n] + (Real(1) - Y) * f.fby[nmy];
-                    const Real Bz = Z * f.fbz[n] + (Real(1) - Z) * f.fbz[nmz];
-                    // cell2r: nearest grid point
-                    const Real Ux = f.uex[n], Uy = f.uey[n], Uz = f.uez[n];
-                    Real Ex = By * Uz - Bz * Uy;
-                    Real Ey = Bz * Ux - Bx * Uz;
-                    Real Ez = Bx * Uy - By * Ux;
-                    if (pp.use_e_pressure) { Ex += f.epx[n]; Ey += f.epy[n]; Ez += f.epz[n]; }
-                    const Real qm = (Real)pp.qmdt2;
-                    const Real dvx = qm * Ex, dvy = qm * Ey, dvz = qm * Ez;
-                    const Real tx = qm * Bx, ty = qm * By, tz = qm * Bz;
-                    const Real t2 = tx * tx + ty * ty + tz * tz;
-                    const Real b2 = Real(2) / (Real(1) + t2);
-                    const Real sx = b2 * tx, sy = b2 * ty, sz = b2 * tz;
-                    const Real vmx = vx + dvx, vmy = vy + dvy, vmz = vz + dvz;
-                    const Real v0x = vmx + vmy * tz - vmz * ty;
-                    const Real v0y = vmy + vmz * tx - vmx * tz;
-                    const Real v0z = vmz + vmx * ty - vmy * tx;
-                    vx = vmx + v0y * sz - v0z * sy + dvx;
-                    vy = vmy + v0z * sx - v0x * sz + dvy;
-                    vz = vmz + v0x * sy - v0y * sx + dvz;
-                    if (pp.use_gravity) {
-                        const double rx = g.x0 + (i + (double)X) * g.dx;
-                        const double ry = g.y0 + (j + (double)Y) * g.dx;
-                        const double rz = g.z0 + (k + (double)Z) * g.dx;
-                        const double r = sqrt(rx * rx + ry * ry + rz * rz);
-                        const double r3 = r * r * r;
-                        if (r3 > 0) {
-                            const double s = -pp.GMdt / r3;
-                            vx += (Real)(s * rx); vy += (Real)(s * ry); vz += (Real)(s * rz);
-                        }
-                    }
-                    const Real v2 = vx * vx + vy * vy + vz * vz;
-                    if (v2 > (Real)pp.max_vi2) {
-                        const Real norm = sqrt((Real)pp.max_vi2 / v2);
-                        vx *= norm; vy *= norm; vz *= norm;
-                        clamped = true;
+        if (act && X >= Real(0)) {
+            int lo = 0, hi = NS;  // ss[lo] <= p < ss[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (ss[mid] <= p) lo = mid; else hi = mid;
+            }
+            const int lc = lo >> 3;
+            int li, lj, lk;
+            local_ijk(g, lc, li, lj, lk);
+            const int i = ox + li, j = oy + lj, k = oz + lk;
+            const bool accel = pp.accelerate && all_nbrs(g, i, j, k) && !pp.initial_flow_through;
+            if (accel) {
+                const int h = (li + 1) + hx * ((lj + 1) + hy * (lk + 1));
+                // face2r: each component linear along its own axis only
+                const Real Bx = X * sbx[h] + (Real(1) - X) * sbx[h - 1];
+                const Real By = Y * sby[h] + (Real(1) - Y) * sby[h - hx];
+                const Real Bz = Z * sbz[h] + (Real(1) - Z) * sbz[h - hx * hy];
+                // cell2r: nearest grid point; E = B x Ue (= -Ue x B) (+ Ep)
+                const Real Ux = sue[lc], Uy = sue[g.tc + lc], Uz = sue[2 * g.tc + lc];
+                Real Ex = By * Uz - Bz * Uy;
+                Real Ey = Bz * Ux - Bx * Uz;
+                Real Ez = Bx * Uy - By * Ux;
+                Ex += sep[lc]; Ey += sep[g.tc + lc]; Ez += sep[2 * g.tc + lc];
+                const Real qm = (Real)pp.qmdt2;
+                const Real dvx = qm * Ex, dvy = qm * Ey, dvz = qm * Ez;
+                const Real tx = qm * Bx, ty = qm * By, tz = qm * Bz;
+                const Real t2 = tx * tx + ty * ty + tz * tz;
+                const Real b2 = Real(2) / (Real(1) + t2);
+                const Real sx = b2 * tx, sy = b2 * ty, sz = b2 * tz;
+                const Real vmx = vx + dvx, vmy = vy + dvy, vmz = vz + dvz;
+                const Real v0x = vmx + vmy * tz - vmz * ty;
+                const Real v0y = vmy + vmz * tx - vmx * tz;
+                const Real v0z = vmz + vmx * ty - vmy * tx;
+                vx = vmx + v0y * sz - v0z * sy + dvx;
+                vy = vmy + v0z * sx - v0x * sz + dvy;
+                vz = vmz + v0x * sy - v0y * sx + dvz;
+                if (pp.use_gravity) {
+                    const double rx = g.x0 + (i + (double)X) * g.dx;
+                    const double ry = g.y0 + (j + (double)Y) * g.dx;
+                    const double rz = g.z0 + (k + (double)Z) * g.dx;
+                    const double r = sqrt(rx * rx + ry * ry + rz * rz);
+                    const double r3 = r * r * r;
+                    if (r3 > 0) {
+                        const double s = -pp.GMdt / r3;
+                        vx += (Real)(s * rx); vy += (Real)(s * ry); vz += (Real)(s * rz);
                     }
                 }
-                // move in FP64; the integer part is the cell crossing
-                double Xd = (double)X + (double)vx * pp.dt_dx;
-                double Yd = (double)Y + (double)vy * pp.dt_dx;
-                double Zd = (double)Z + (double)vz * pp.dt_dx;
-                const double fx = floor(Xd), fy = floor(Yd), fz = floor(Zd);
-                int sx = (int)fx, sy = (int)fy, sz = (int)fz;
-                bool ok = (fx >= -1.0 && fx <= 1.0 && fy >= -1.0 && fy <= 1.0 && fz >= -1.0 && fz <= 1.0);
-                int nn = -1;
-                if (ok) nn = nbr(g, i, j, k, sx, sy, sz);
-                if (nn >= 0) {
-                    X = (Real)(Xd - fx); Y = (Real)(Yd - fy); Z = (Real)(Zd - fz);
-                    if (X >= Real(1)) X = below_one<Real>();
-                    if (Y >= Real(1)) Y = below_one<Real>();
-                    if (Z >= Real(1)) Z = below_one<Real>();
-                    const int o = (X >= Real(0.5)) + 2 * (Y >= Real(0.5)) + 4 * (Z >= Real(0.5));
-                    const int ex = i - ox + sx + 1, ey = j - oy + sy + 1, ez = k - oz + sz + 1;
-                    key = (uint32_t)((ex + g.ex * (ey + g.ey * ez)) * 8 + o);
-                } else {
-                    dropped = true;
+                const Real v2 = vx * vx + vy * vy + vz * vz;
+                if (v2 > (Real)pp.max_vi2) {
+                    const Real norm = sqrt((Real)pp.max_vi2 / v2);
+                    vx *= norm; vy *= norm; vz *= norm;
+                    clamped = true;
                 }
             }
-            B.x[p] = X; B.y[p] = Y; B.z[p] = Z;
-            B.vx[p] = vx; B.vy[p] = vy; B.vz[p] = vz;
+            // move in FP64; the integer part is the cell crossing
+            const double Xd = (double)X + (double)vx * pp.dt_dx;
+            const double Yd = (double)Y + (double)vy * pp.dt_dx;
+            const double Zd = (double)Z + (double)vz * pp.dt_dx;
+            const double fx = floor(Xd), fy = floor(Yd), fz = floor(Zd);
+            const bool ok = (fx >= -1.0 && fx <= 1.0 && fy >= -1.0 && fy <= 1.0 && fz >= -1.0 && fz <= 1.0);
+            const int sx = (int)fx, sy = (int)fy, sz = (int)fz;
+            if (ok && nbr(g, i, j, k, sx, sy, sz) >= 0) {
+                X = (Real)(Xd - fx); Y = (Real)(Yd - fy); Z = (Real)(Zd - fz);
+                if (X >= Real(1)) X = below_one<Real>();
+                if (Y >= Real(1)) Y = below_one<Real>();
+                if (Z >= Real(1)) Z = below_one<Real>();
+                const int o = (X >= Real(0.5)) + 2 * (Y >= Real(0.5)) + 4 * (Z >= Real(0.5));
+                key = (uint32_t)(((li + sx + 1) + g.ex * ((lj + sy + 1) + g.ey * (lk + sz + 1))) * 8 + o);
+            } else {
+                dropped = true;
+            }
         }
-        // stable rank among the lanes holding the same slot
-        unsigned long long match = __ballot(act && key != deadkey);
+        // rank among this wave's lanes holding the same slot
+        unsigned long long match = __ballot(key != deadkey);
         for (int b = 0; b < kbits; ++b) {
             const bool bit = (key >> b) & 1u;
             const unsigned long long bal = __ballot(bit);
@@ -168,52 +229,56 @@ __global__ __launch_bounds__(64) void k_push_rank(Geom g, PushParams pp, PushFie
         const unsigned long long below = (lane == 0) ? 0ull : (match & ((~0ull) >> (64 - lane)));
         const uint32_t rin = (uint32_t)__popcll(below);
         const uint32_t gsz = (uint32_t)__popcll(match);
-        if (act) {
-            if (key != deadkey) {
-                const uint32_t c0 = cnt[key];
-                const uint32_t rank = c0 + rin;
-                if (rin == gsz - 1) cnt[key] = c0 + gsz;
-                packed[p] = (rank <= RANK_MASK) ? ((key << RANK_BITS) | rank) : DEAD_PACKED;
-                if (rank > RANK_MASK) atomicAdd(counters + C_RANK_OVERFLOW, 1ull);
-            } else {
-                packed[p] = DEAD_PACKED;
+        const bool leader = key != deadkey && rin == gsz - 1;
+        if (leader) grp[w * g.extk + key] = (uint8_t)gsz;
+        __syncthreads();
+        if (key != deadkey) {
+            uint32_t rank = cnt[key] + rin;
+            for (int v = 0; v < w; ++v) rank += grp[v * g.extk + key];
+            if (pass == 1) {
+                const uint32_t dst = cnt_tile[(size_t)T * g.extk + key] + rank;
+                if (dst < cap) {
+                    D.x[dst] = X; D.y[dst] = Y; D.z[dst] = Z;
+                    D.vx[dst] = vx; D.vy[dst] = vy; D.vz[dst] = vz;
+                } else {
+                    atomicAdd(counters + C_RANK_OVERFLOW, 1ull);  // slot 0
+                }
             }
         }
-        count_event(counters, C_MAX_VI, clamped);
-        count_event(counters, C_DROPPED, dropped);
+        __syncthreads();
+        if (leader) {
+            atomicAdd(&cnt[key], gsz);  // integer: the order of the four waves does not matter
+            grp[w * g.extk + key] = 0;
+        }
+        if (pass == 0) {
+            count_event(counters, C_MAX_VI, clamped);
+            count_event(counters, C_DROPPED, dropped);
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    for (int e = lane; e < g.extk; e += 64) cnt_tile[(size_t)T * g.extk + e] = cnt[e];
+    if (pass == 0)
+        for (int e = tid; e < g.extk; e += PM_THREADS) cnt_tile[(size_t)T * g.extk + e] = cnt[e];
 }
 
-// decode a subcell key into tile, cell coordinates and octant
-__device__ __forceinline__ void decode_key(const Geom& g, uint32_t key, int& T, int& i, int& j, int& k, int& o) {
-    T = (int)(key / (uint32_t)(g.tc * 8));
-    const int r = (int)(key % (uint32_t)(g.tc * 8));
-    o = r & 7;
-    const int lc = r >> 3;
-    int ox, oy, oz;
-    tile_origin(g, T, ox, oy, oz);
-    i = ox + lc % g.tx;
-    j = oy + (lc / g.tx) % g.ty;
-    k = oz + lc / (g.tx * g.ty);
-}
-
-// Number of particles arriving in subcell key k: the slot counts of every
-// tile whose extended region holds the cell, in a fixed order.  With
-// write_base, the slot counts are replaced by the first destination index of
-// each slot (the scan of totals must already be in sub).
+// Number of particles arriving in each subcell: the slot counts of every tile
+// whose extended region holds the cell, in a fixed order.  With WRITE_BASE,
+// the slot counts are replaced by the first destination index of each slot
+// (out must already hold the exclusive scan of the totals).  One block per
+// tile (3D grid of tiles), one thread per subcell of the tile.
 template <bool WRITE_BASE>
-__global__ __launch_bounds__(256) void k_slot_offsets(Geom g, uint32_t* __restrict__ cnt_tile,
+__global__ __launch_bounds__(512) void k_slot_offsets(Geom g, uint32_t* __restrict__ cnt_tile,
                                                      uint32_t* __restrict__ out) {
-    const uint32_t key = blockIdx.x * 256u + threadIdx.x;
-    if (key >= g.nkeys) return;
-    int T, i, j, k, o;
-    decode_key(g, key, T, i, j, k, o);
+    const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
+    const int tid = threadIdx.x;
+    const uint32_t key = (uint32_t)T * (uint32_t)(g.tc * 8) + tid;
+    const int o = tid & 7;
+    int li, lj, lk;
+    local_ijk(g, tid >> 3, li, lj, lk);
+    const int i = (blockIdx.x << g.ltx) + li, j = (blockIdx.y << g.lty) + lj, k = (blockIdx.z << g.ltz) + lk;
     int tx[6], ex[6], ty[6], ey[6], tz[6], ez[6];
-    const int mx = axis_candidates(i, g.nx, g.tx, g.ntx, g.per[0], tx, ex);
-    const int my = axis_candidates(j, g.ny, g.ty, g.nty, g.per[1], ty, ey);
-    const int mz = axis_candidates(k, g.nz, g.tz, g.ntz, g.per[2], tz, ez);
+    const int mx = axis_candidates(i, g.nx, g.ltx, g.ntx, g.per[0], tx, ex);
+    const int my = axis_candidates(j, g.ny, g.lty, g.nty, g.per[1], ty, ey);
+    const int mz = axis_candidates(k, g.nz, g.ltz, g.ntz, g.per[2], tz, ez);
     uint32_t run = WRITE_BASE ? out[key] : 0u;
     for (int c = 0; c < mz; ++c)
         for (int b = 0; b < my; ++b)
@@ -282,144 +347,154 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const uint32_t* __r
     if (base + SCAN_ITEMS >= n && base < n) out[n] = run;
 }
 
-// Moves every particle of an old tile to its slot: base of the slot (written
-// by k_slot_offsets<true>) plus its rank.  One wave per old tile.
-template <typename Real>
-__global__ __launch_bounds__(64) void k_scatter(Geom g, PArrays<Real> B, const uint32_t* __restrict__ packed,
-                                               const uint32_t* __restrict__ cnt_tile,
-                                               const uint32_t* __restrict__ sub_old, PArrays<Real> A,
-                                               uint32_t cap, unsigned long long* __restrict__ counters) {
-    const int T = blockIdx.x;
-    const int lane = threadIdx.x;
-    int ox, oy, oz;
-    tile_origin(g, T, ox, oy, oz);
-    const uint32_t p0 = sub_old[(size_t)T * g.tc * 8];
-    const uint32_t p1 = sub_old[(size_t)(T + 1) * g.tc * 8];
-    for (uint32_t p = p0 + lane; p < p1; p += 64) {
-        const uint32_t pk = packed[p];
-        if (pk == DEAD_PACKED) continue;
-        const uint32_t slot = pk >> RANK_BITS;
-        const uint32_t dst = cnt_tile[(size_t)T * g.extk + slot] + (pk & RANK_MASK);
-        if (dst >= cap) {  // cannot happen with consistent counts; never write out of bounds
-            atomicAdd(counters + C_RANK_OVERFLOW, 1ull);
-            continue;
-        }
-        const int e = (int)(slot >> 3);
-        const int ex = e % g.ex, ey = (e / g.ex) % g.ey, ez = e / (g.ex * g.ey);
-        const int i = wrap_axis(ox + ex - 1, g.nx, g.per[0]);
-        const int j = wrap_axis(oy + ey - 1, g.ny, g.per[1]);
-        const int k = wrap_axis(oz + ez - 1, g.nz, g.per[2]);
-        A.x[dst] = B.x[p]; A.y[dst] = B.y[p]; A.z[dst] = B.z[p];
-        A.vx[dst] = B.vx[p]; A.vy[dst] = B.vy[p]; A.vz[dst] = B.vz[p];
-        A.cell[dst] = (uint32_t)cell_index(g, i, j, k);
-    }
-}
-
 // Cloud-in-cell deposit (Accumulator::accumulateCell,
 // src/particle_accumulator.cpp:43-156).  One workgroup per tile, one thread
 // per subcell: a subcell's particles share one 2x2x2 footprint, so the thread
 // sums, in particle order, the 8 corner weights times (1, vx, vy, vz).  The
-// subcell sums are staged in LDS and every cell of the tile's extended region
-// adds the (up to 64) subcell terms that reach it in a fixed order.  The tile
-// result, times w*q, is added in FP64 to the tile's own ghost buffer; no
-// atomics, so the result is the same bit for bit on every run.
+// subcell sums are staged in LDS (two quantities at a time) and every cell of
+// the tile's extended region adds the (up to 64) subcell terms that reach it
+// in a fixed order.  The result is the tile's ghost partial of this species
+// (unscaled); k_fold sums partials over tiles and species in FP64.  No
+// atomics, so every run gives the same bits.
+template <typename Real>
+__device__ __forceinline__ void cic_accumulate(Real (&acc)[4][8], bool hx, bool hy, bool hz, Real X, Real Y, Real Z,
+                                               Real vx, Real vy, Real vz) {
+    // weights of the lower / upper target along each axis
+    const Real wx1 = hx ? X - Real(0.5) : X + Real(0.5);
+    const Real wy1 = hy ? Y - Real(0.5) : Y + Real(0.5);
+    const Real wz1 = hz ? Z - Real(0.5) : Z + Real(0.5);
+    const Real live = X >= Real(0) ? Real(1) : Real(0);  // removed particles add nothing
+    const Real wx[2] = {live - live * wx1, live * wx1}, wy[2] = {Real(1) - wy1, wy1}, wz[2] = {Real(1) - wz1, wz1};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const Real w = wx[c & 1] * wy[(c >> 1) & 1] * wz[c >> 2];
+        acc[0][c] += w;
+        acc[1][c] += w * vx;
+        acc[2][c] += w * vy;
+        acc[3][c] += w * vz;
+    }
+}
+
+// LDS of the deposit: half of the subcell sums (2 quantities x 8 corners) as
+// [quantity*8+corner][octant][cell], each octant block padded by one word so
+// that neighbouring cells fall in different banks; the extended-region
+// coordinates of the tile; the first half's results.
+template <typename Real>
+RH_HD size_t deposit_lds_bytes(const Geom& g) {
+    return sizeof(Real) * 16 * 8 * (size_t)(g.tc + 1) + 4 * (size_t)g.next + sizeof(Real) * 2 * (size_t)g.next;
+}
+
+template <typename Real>
+struct alignas(4 * sizeof(Real)) Real4 { Real v[4]; };
+
 template <typename Real>
 __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const uint32_t* __restrict__ sub,
-                                                double wq, double* __restrict__ tile_acc) {
+                                                Real* __restrict__ partial) {
     extern __shared__ unsigned char smem_raw[];
-    Real* S = reinterpret_cast<Real*>(smem_raw);
     const int NS = g.tc * 8;
-    const int T = blockIdx.x;
+    const int OB = g.tc + 1;          // octant block (padded)
+    const int QS = 8 * OB;            // one (quantity, corner) plane
+    Real* S = reinterpret_cast<Real*>(smem_raw);
+    uint32_t* extc = reinterpret_cast<uint32_t*>(smem_raw + sizeof(Real) * 16 * QS);
+    const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
     const int ls = threadIdx.x;
-    {
-        const uint32_t key = (uint32_t)(T * NS + ls);
-        const uint32_t p0 = sub[key], p1 = sub[key + 1];
-        const int o = ls & 7;
-        Real acc[4][8];
+    for (int e = ls; e < g.next; e += blockDim.x) {
+        const int exy = g.ex * g.ey;
+        const int cz = e / exy, r = e - cz * exy, cy = r / g.ex, cx = r - cy * g.ex;
+        extc[e] = (uint32_t)cx | ((uint32_t)cy << 8) | ((uint32_t)cz << 16);
+    }
+    const uint32_t key = (uint32_t)(T * NS + ls);
+    const uint32_t p0 = sub[key], p1 = sub[key + 1];
+    const int o = ls & 7, cell = ls >> 3;
+    const bool hx = o & 1, hy = (o >> 1) & 1, hz = (o >> 2) & 1;
+    Real acc[4][8];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
-            for (int c = 0; c < 8; ++c) acc[q][c] = Real(0);
-        const bool hx = o & 1, hy = (o >> 1) & 1, hz = (o >> 2) & 1;
-        for (uint32_t p = p0; p < p1; ++p) {
-            const Real X = A.x[p], Y = A.y[p], Z = A.z[p];
-            const Real vx = A.vx[p], vy = A.vy[p], vz = A.vz[p];
-            // weights of the lower / upper target along each axis
-            const Real wx1 = hx ? X - Real(0.5) : X + Real(0.5);
-            const Real wy1 = hy ? Y - Real(0.5) : Y + Real(0.5);
-            const Real wz1 = hz ? Z - Real(0.5) : Z + Real(0.5);
-            const Real wx0 = Real(1) - wx1, wy0 = Real(1) - wy1, wz0 = Real(1) - wz1;
-            const Real wx[2] = {wx0, wx1}, wy[2] = {wy0, wy1}, wz[2] = {wz0, wz1};
+        for (int c = 0; c < 8; ++c) acc[q][c] = Real(0);
+    for (uint32_t p = p0; p < p1; ++p) cic_accumulate(acc, hx, hy, hz, A.x[p], A.y[p], A.z[p], A.vx[p], A.vy[p], A.vz[p]);
+    Real4<Real>* out = reinterpret_cast<Real4<Real>*>(partial) + (size_t)T * g.next;
+    Real* R = reinterpret_cast<Real*>(extc + g.next);  // first-half results, next x 2
 #pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const Real w = wx[c & 1] * wy[(c >> 1) & 1] * wz[c >> 2];
-                acc[0][c] += w;
-                acc[1][c] += w * vx;
-                acc[2][c] += w * vy;
-                acc[3][c] += w * vz;
+    for (int half = 0; half < 2; ++half) {
+        if (half) __syncthreads();  // the first half's reads are done
+#pragma unroll
+        for (int qq = 0; qq < 2; ++qq)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) S[(qq * 8 + c) * QS + o * OB + cell] = acc[2 * half + qq][c];
+        __syncthreads();
+        for (int e = ls; e < g.next; e += blockDim.x) {
+            const uint32_t ec = extc[e];
+            const int lx = (int)(ec & 255) - 1, ly = (int)((ec >> 8) & 255) - 1, lz = (int)(ec >> 16) - 1;
+            // per axis: (cell, half, corner) with cell + half - 1 + corner = target
+            int cx[4], bx[4], nx_ = 0, cy[4], by[4], ny_ = 0, cz[4], bz[4], nz_ = 0;
+            for (int h = 0; h < 2; ++h)
+                for (int kk = 0; kk < 2; ++kk) {
+                    const int c1 = lx + 1 - h - kk, c2 = ly + 1 - h - kk, c3 = lz + 1 - h - kk;
+                    if (c1 >= 0 && c1 < g.tx) { cx[nx_] = c1 + h * OB; bx[nx_++] = kk; }
+                    if (c2 >= 0 && c2 < g.ty) { cy[ny_] = (c2 << g.ltx) + 2 * h * OB; by[ny_++] = 2 * kk; }
+                    if (c3 >= 0 && c3 < g.tz) { cz[nz_] = (c3 << (g.ltx + g.lty)) + 4 * h * OB; bz[nz_++] = 4 * kk; }
+                }
+            Real res[2];
+#pragma unroll
+            for (int qq = 0; qq < 2; ++qq) {
+                Real sum = Real(0);
+                const Real* Sq = S + (size_t)qq * 8 * QS;
+                for (int c = 0; c < nz_; ++c)
+                    for (int b = 0; b < ny_; ++b)
+                        for (int a = 0; a < nx_; ++a)
+                            sum += Sq[(bx[a] + by[b] + bz[c]) * QS + cx[a] + cy[b] + cz[c]];
+                res[qq] = sum;
+            }
+            if (half == 0) {
+                R[2 * e] = res[0];
+                R[2 * e + 1] = res[1];
+            } else {
+                Real4<Real> r;
+                r.v[0] = R[2 * e]; r.v[1] = R[2 * e + 1]; r.v[2] = res[0]; r.v[3] = res[1];
+                out[e] = r;
             }
         }
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int c = 0; c < 8; ++c) S[(q * 8 + c) * NS + ls] = acc[q][c];
-    }
-    __syncthreads();
-    const int nout = g.next * 4;
-    double* out = tile_acc + (size_t)T * nout;
-    for (int idx = ls; idx < nout; idx += blockDim.x) {
-        const int e = idx >> 2, q = idx & 3;
-        const int lx = e % g.ex - 1, ly = (e / g.ex) % g.ey - 1, lz = e / (g.ex * g.ey) - 1;
-        Real sum = Real(0);
-        // target local coordinate l = cell + half - 1 + corner, half/corner in {0,1}
-        for (int hz = 0; hz < 2; ++hz)
-            for (int kz = 0; kz < 2; ++kz) {
-                const int cz = lz + 1 - hz - kz;
-                if (cz < 0 || cz >= g.tz) continue;
-                for (int hy = 0; hy < 2; ++hy)
-                    for (int ky = 0; ky < 2; ++ky) {
-                        const int cy = ly + 1 - hy - ky;
-                        if (cy < 0 || cy >= g.ty) continue;
-                        for (int hx = 0; hx < 2; ++hx)
-                            for (int kx = 0; kx < 2; ++kx) {
-                                const int cx = lx + 1 - hx - kx;
-                                if (cx < 0 || cx >= g.tx) continue;
-                                const int lsub = (cx + g.tx * (cy + g.ty * cz)) * 8 + hx + 2 * hy + 4 * hz;
-                                const int corner = kx + 2 * ky + 4 * kz;
-                                sum += S[(q * 8 + corner) * NS + lsub];
-                            }
-                    }
-            }
-        out[idx] += wq * (double)sum;
     }
 }
 
 // Accumulator::addRemoteUpdates (src/particle_accumulator.cpp:260-335): each
-// cell sums, in FP64 and a fixed order, the ghost contributions of every tile
-// whose extended region holds it, then rho/=dV with the rho floor, J/=dV.
+// cell sums in FP64, species by species and in a fixed tile order, the ghost
+// partials of every tile whose extended region holds it, times the species'
+// w*q; then rho/=dV with the rho floor, J/=dV.
+constexpr int MAX_FOLD_SPECIES = 8;
 template <typename Real>
-__global__ __launch_bounds__(256) void k_fold(Geom g, const double* __restrict__ tile_acc, double inv_dV,
-                                             double min_rho, Real* __restrict__ rho, Real* __restrict__ jx,
-                                             Real* __restrict__ jy, Real* __restrict__ jz,
-                                             unsigned long long* __restrict__ counters) {
-    const int n = blockIdx.x * 256 + threadIdx.x;
+struct FoldSources {
+    const Real* partial[MAX_FOLD_SPECIES];
+    double wq[MAX_FOLD_SPECIES];
+    int n;
+};
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_fold(Geom g, FoldSources<Real> src, double inv_dV, double min_rho,
+                                             Real* __restrict__ rho, Real* __restrict__ jx, Real* __restrict__ jy,
+                                             Real* __restrict__ jz, unsigned long long* __restrict__ counters) {
+    const int i = blockIdx.x * 32 + threadIdx.x, j = blockIdx.y * 8 + threadIdx.y, k = blockIdx.z;
     bool floored = false;
-    if (n < g.nc) {
-        int i, j, k;
-        cell_ijk(g, n, i, j, k);
+    if (i < g.nx && j < g.ny) {
+        const int n = cell_index(g, i, j, k);
         int tx[6], ex[6], ty[6], ey[6], tz[6], ez[6];
-        const int mx = axis_candidates(i, g.nx, g.tx, g.ntx, g.per[0], tx, ex);
-        const int my = axis_candidates(j, g.ny, g.ty, g.nty, g.per[1], ty, ey);
-        const int mz = axis_candidates(k, g.nz, g.tz, g.ntz, g.per[2], tz, ez);
+        const int mx = axis_candidates(i, g.nx, g.ltx, g.ntx, g.per[0], tx, ex);
+        const int my = axis_candidates(j, g.ny, g.lty, g.nty, g.per[1], ty, ey);
+        const int mz = axis_candidates(k, g.nz, g.ltz, g.ntz, g.per[2], tz, ez);
         double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-        for (int c = 0; c < mz; ++c)
-            for (int b = 0; b < my; ++b)
-                for (int a = 0; a < mx; ++a) {
-                    const int TT = tx[a] + g.ntx * (ty[b] + g.nty * tz[c]);
-                    const int e = ex[a] + g.ex * (ey[b] + g.ey * ez[c]);
-                    const double* v = tile_acc + ((size_t)TT * g.next + e) * 4;
-                    s0 += v[0]; s1 += v[1]; s2 += v[2]; s3 += v[3];
-                }
+        for (int s = 0; s < src.n; ++s) {
+            double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+            for (int c = 0; c < mz; ++c)
+                for (int b = 0; b < my; ++b)
+                    for (int a = 0; a < mx; ++a) {
+                        const int TT = tx[a] + g.ntx * (ty[b] + g.nty * tz[c]);
+                        const int e = ex[a] + g.ex * (ey[b] + g.ey * ez[c]);
+                        const Real4<Real> v = reinterpret_cast<const Real4<Real>*>(src.partial[s])[(size_t)TT * g.next + e];
+                        t0 += (double)v.v[0]; t1 += (double)v.v[1]; t2 += (double)v.v[2]; t3 += (double)v.v[3];
+                    }
+            s0 += src.wq[s] * t0; s1 += src.wq[s] * t1; s2 += src.wq[s] * t2; s3 += src.wq[s] * t3;
+        }
         double r = s0 * inv_dV;
         if (r < min_rho) { r = min_rho; floored = true; }
         rho[n] = (Real)r;
@@ -432,8 +507,8 @@ __global__ __launch_bounds__(256) void k_fold(Geom g, const double* __restrict__
 
 // ParticleBoundaryCondHybrid::apply (src/particle_boundary_cond_hybrid.h:
 // 112-185): empty exterior cells, remove particles inside the obstacle in
-// innerFlagParticle cells.  Removed particles get cell = DEAD_CELL; the next
-// sort drops them.  One thread per listed cell.
+// innerFlagParticle cells.  Removed particles get x = -1; the next sort drops
+// them.  One thread per listed cell.
 template <typename Real>
 __global__ __launch_bounds__(256) void k_boundary(Geom g, const int* __restrict__ cells, int ncells,
                                                  const uint32_t* __restrict__ sub, PArrays<Real> A,
@@ -448,9 +523,10 @@ __global__ __launch_bounds__(256) void k_boundary(Geom g, const int* __restrict_
     const bool exterior = !all_nbrs(g, i, j, k);
     const double xb = g.x0 + i * g.dx, yb = g.y0 + j * g.dx, zb = g.z0 + k * g.dx;
     for (uint32_t p = p0; p < p1; ++p) {
-        if (exterior) { A.cell[p] = DEAD_CELL; continue; }
+        if (exterior) { A.x[p] = Real(-1); continue; }
+        if (A.x[p] < Real(0)) continue;
         const double x = xb + (double)A.x[p] * g.dx, y = yb + (double)A.y[p] * g.dx, z = zb + (double)A.z[p] * g.dx;
-        if (x * x + y * y + z * z < R2) A.cell[p] = DEAD_CELL;
+        if (x * x + y * y + z * z < R2) A.x[p] = Real(-1);
     }
 }
 
@@ -464,7 +540,7 @@ __global__ __launch_bounds__(256) void k_diag(PArrays<Real> A, const uint32_t* _
     const uint32_t np = *np_ptr;
     double s[5] = {0, 0, 0, 0, 0};
     for (uint32_t p = blockIdx.x * 256u + threadIdx.x; p < np; p += 256u * gridDim.x) {
-        if (A.cell[p] == DEAD_CELL) continue;
+        if (A.x[p] < Real(0)) continue;
         const double vx = A.vx[p], vy = A.vy[p], vz = A.vz[p];
         s[0] += 1.0; s[1] += vx; s[2] += vy; s[3] += vz; s[4] += vx * vx + vy * vy + vz * vz;
     }
