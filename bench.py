@@ -44,10 +44,10 @@ def kernel_bytes(name: str, ctx, np_total: float, nspecies: int) -> float:
     extk = next_ * 8
     nkeys = ntiles * lay["tx"] * lay["ty"] * lay["tz"] * 8
     npp = np_total / nspecies  # particles per species launch (mean)
-    if name == "push":       # pass 0: read 6 f32; 9 f32 fields per cell; write slot counts
-        return 24.0 * npp + 36.0 * nc + 4.0 * extk * ntiles + 4.0 * nkeys
-    if name == "move":       # pass 1: read 6 f32, write 6 f32 at the sorted slot; fields; slot bases
-        return 48.0 * npp + 36.0 * nc + 4.0 * extk * ntiles + 4.0 * nkeys
+    if name == "push":       # pass 0: read 6 f32 + local cell byte; 9 f32 fields per cell; write slot counts
+        return 25.0 * npp + 36.0 * nc + 4.0 * extk * ntiles + 4.0 * ntiles
+    if name == "move":       # pass 1: read and write 6 f32 + local cell byte; fields; slot bases
+        return 50.0 * npp + 36.0 * nc + 4.0 * extk * ntiles + 4.0 * ntiles
     if name == "deposit":    # read 6 f32 per particle, subcell starts; read+write FP64 tile ghosts
         return 24.0 * npp + 4.0 * nkeys + 64.0 * next_ * ntiles
     if name == "fold":       # read FP64 tile ghosts once, write rho + J (f32)

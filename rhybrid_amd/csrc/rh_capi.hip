@@ -112,15 +112,17 @@ public:
         rh_species_desc d;
         size_t cap = 0;
         DevBuf<Real> a[6], dst[6];
+        DevBuf<uint8_t> a_lc, dst_lc;
         DevBuf<uint32_t> sub, sub_new, cnt_tile;
         DevBuf<Real> partial;  // deposit: per-tile ghost partials (ntiles x next x 4)
         bool deposited = false;
         bool pushed = false;  // pass 0 done, pass 1 pending
         PushParams pp{};      // parameters of the pending push
-        PArrays<Real> A() { return {a[0].p, a[1].p, a[2].p, a[3].p, a[4].p, a[5].p}; }
-        PArrays<Real> Dst() { return {dst[0].p, dst[1].p, dst[2].p, dst[3].p, dst[4].p, dst[5].p}; }
+        PArrays<Real> A() { return {a[0].p, a[1].p, a[2].p, a[3].p, a[4].p, a[5].p, a_lc.p}; }
+        PArrays<Real> Dst() { return {dst[0].p, dst[1].p, dst[2].p, dst[3].p, dst[4].p, dst[5].p, dst_lc.p}; }
         void swap_stores() {
             for (int q = 0; q < 6; ++q) std::swap(a[q].p, dst[q].p);
+            std::swap(a_lc.p, dst_lc.p);
             std::swap(sub.p, sub_new.p);
         }
     };
@@ -138,6 +140,8 @@ public:
     std::vector<std::unique_ptr<Species>> sp;
     std::vector<DevBuf<uint32_t>> scan_lv;  // per level: block sums
     DevBuf<uint32_t> zero_u32;
+    DevBuf<uint16_t> dep_list;   // deposit gather lists (next x 64)
+    DevBuf<uint8_t> dep_count;
     int kbits = 12;
     bool timing = false;
     std::map<std::string, Timer> timers;
@@ -180,6 +184,15 @@ public:
         dcount.alloc((size_t)C_N * COUNTER_SLOTS);
         RH_HIP(hipMemsetAsync(dcount.p, 0, sizeof(unsigned long long) * C_N * COUNTER_SLOTS, stream));
         diag_partial.alloc(DIAG_BLOCKS * 5);
+        {
+            std::vector<uint16_t> hl((size_t)g.next * 64, 0);
+            std::vector<uint8_t> hc((size_t)g.next, 0);
+            deposit_gather_lists(g, hl.data(), hc.data());
+            dep_list.alloc(hl.size());
+            dep_count.alloc(hc.size());
+            RH_HIP(hipMemcpy(dep_list.p, hl.data(), hl.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+            RH_HIP(hipMemcpy(dep_count.p, hc.data(), hc.size(), hipMemcpyHostToDevice));
+        }
         zero_u32.alloc(1);
         RH_HIP(hipMemsetAsync(zero_u32.p, 0, sizeof(uint32_t), stream));
         // scan levels
@@ -349,6 +362,8 @@ public:
         if (cap >= 0xFFFFFFFFull) throw CapError("particle capacity exceeds 32-bit indices");
         cap = std::max<size_t>(cap, 64);
         for (int q = 0; q < 6; ++q) { s.a[q].alloc(cap); s.dst[q].alloc(cap); }
+        s.a_lc.alloc(cap);
+        s.dst_lc.alloc(cap);
         s.cap = cap;
     }
 
@@ -412,8 +427,10 @@ public:
         std::vector<uint32_t> fillp(sub.begin(), sub.end() - 1);
         std::vector<Real> h[6];
         for (int q = 0; q < 6; ++q) h[q].resize((size_t)n);
+        std::vector<uint8_t> hl((size_t)n);
         for (int64_t p = 0; p < n; ++p) {
             const uint32_t d = fillp[key[p]]++;
+            hl[d] = (uint8_t)((key[p] >> 3) % (uint32_t)g.tc);
             h[0][d] = u[0][p]; h[1][d] = u[1][p]; h[2][d] = u[2][p];
             h[3][d] = (Real)st[7 * p + 3]; h[4][d] = (Real)st[7 * p + 4]; h[5][d] = (Real)st[7 * p + 5];
         }
@@ -422,6 +439,7 @@ public:
         ensure_capacity(S_, cap);
         for (int q = 0; q < 6; ++q)
             RH_HIP(hipMemcpyAsync(S_.a[q].p, h[q].data(), sizeof(Real) * n, hipMemcpyHostToDevice, stream));
+        RH_HIP(hipMemcpyAsync(S_.a_lc.p, hl.data(), n, hipMemcpyHostToDevice, stream));
         RH_HIP(hipMemcpyAsync(S_.sub.p, sub.data(), sizeof(uint32_t) * sub.size(), hipMemcpyHostToDevice, stream));
         RH_HIP(hipStreamSynchronize(stream));
         S_.pushed = false;
@@ -532,7 +550,7 @@ public:
         if (S_.pushed) throw StateError("accumulate between propagate and wait_particle_sends");
         const int NS = g.tc * 8;
         launch("deposit", k_deposit<Real>, tile_grid(), NS, deposit_lds_bytes<Real>(g), g, S_.A(),
-               (const uint32_t*)S_.sub.p, S_.partial.p);
+               (const uint32_t*)S_.sub.p, (const uint16_t*)dep_list.p, (const uint8_t*)dep_count.p, S_.partial.p);
         S_.deposited = true;
     }
 

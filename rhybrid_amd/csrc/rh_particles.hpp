@@ -7,8 +7,10 @@
 //   x,y,z    cell-local position in units of dx, in [0,1); x = -1 marks a
 //            particle removed by the boundary conditions
 //   vx,vy,vz velocity in m/s
+//   lc       the particle's cell inside its tile (implied by the sort order,
+//            kept so that kernels need not search the subcell ranges)
 // sorted by subcell key (rh_geom.hpp); sub[k] .. sub[k+1] is the range of
-// subcell key k, so a particle's cell is implied by its index.
+// subcell key k.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,6 +22,7 @@ namespace rh {
 template <typename Real>
 struct PArrays {
     Real *x, *y, *z, *vx, *vy, *vz;
+    uint8_t* lc;  // the particle's cell inside its tile (local index, < 64)
 };
 
 template <typename Real>
@@ -72,8 +75,7 @@ template <> __device__ __forceinline__ double below_one<double>() { return 0x1.f
 // migration is a stable counting sort).  Within a chunk, each wave ranks its
 // lanes with ballots and publishes its per-slot group sizes; a lane's rank
 // adds the slot's running count and the group sizes of the earlier waves.
-// A particle's cell comes from the tile's subcell ranges (binary search in
-// LDS); the tile's fields are staged in LDS; removed particles carry x < 0.
+// The tile's fields are staged in LDS; removed particles carry x < 0.
 constexpr int PM_THREADS = 256;
 constexpr int PM_WAVES = PM_THREADS / 64;
 
@@ -81,10 +83,8 @@ RH_HD size_t pm_align8(size_t b) { return (b + 7) & ~size_t(7); }
 
 template <typename Real>
 RH_HD size_t push_move_lds_bytes(const Geom& g) {
-    const size_t ns = (size_t)g.tc * 8;
     const size_t nh = (size_t)(g.tx + 1) * (g.ty + 1) * (g.tz + 1);
-    return pm_align8(4 * (size_t)g.extk + 4 * (ns + 1) + (size_t)PM_WAVES * g.extk) +
-           sizeof(Real) * (3 * nh + 6 * (size_t)g.tc);
+    return pm_align8(4 * (size_t)g.extk + (size_t)PM_WAVES * g.extk) + sizeof(Real) * (3 * nh + 6 * (size_t)g.tc);
 }
 
 template <typename Real>
@@ -98,10 +98,8 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
     const int NS = g.tc * 8;
     const int hx = g.tx + 1, hy = g.ty + 1, nh = (g.tx + 1) * (g.ty + 1) * (g.tz + 1);
     uint32_t* cnt = reinterpret_cast<uint32_t*>(pm_smem);     // extk running slot counts
-    uint32_t* ss = cnt + g.extk;                               // NS+1 subcell starts
-    uint8_t* grp = reinterpret_cast<uint8_t*>(ss + NS + 1);    // PM_WAVES x extk group sizes
-    Real* fl = reinterpret_cast<Real*>(pm_smem + pm_align8(4 * (size_t)g.extk + 4 * (size_t)(NS + 1) +
-                                                          (size_t)PM_WAVES * g.extk));
+    uint8_t* grp = reinterpret_cast<uint8_t*>(cnt + g.extk);   // PM_WAVES x extk group sizes
+    Real* fl = reinterpret_cast<Real*>(pm_smem + pm_align8(4 * (size_t)g.extk + (size_t)PM_WAVES * g.extk));
     Real* sbx = fl;            // faceB of local cells -1..t-1 (halo on the low side)
     Real* sby = fl + nh;
     Real* sbz = fl + 2 * nh;
@@ -112,7 +110,6 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
     const int ox = blockIdx.x << g.ltx, oy = blockIdx.y << g.lty, oz = blockIdx.z << g.ltz;
     for (int e = tid; e < g.extk; e += PM_THREADS) cnt[e] = 0;
     for (int e = tid; e < PM_WAVES * g.extk; e += PM_THREADS) grp[e] = 0;
-    for (int q = tid; q <= NS; q += PM_THREADS) ss[q] = sub[(size_t)T * NS + q];
     for (int q = tid; q < nh; q += PM_THREADS) {
         const int a = q % hx - 1, b = (q / hx) % hy - 1, c = q / (hx * hy) - 1;
         const int n = nbr(g, ox, oy, oz, a, b, c);
@@ -129,31 +126,31 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
         else { sep[q] = sep[g.tc + q] = sep[2 * g.tc + q] = Real(0); }
     }
     __syncthreads();
-    const uint32_t p0 = ss[0], p1 = ss[NS];
+    const uint32_t p0 = sub[(size_t)T * NS], p1 = sub[(size_t)(T + 1) * NS];
     const uint32_t deadkey = (uint32_t)MAX_EXTK;
     // prefetch of the first chunk
     Real nX = 0, nY = 0, nZ = 0, nvx = 0, nvy = 0, nvz = 0;
+    int nlc = 0;
     {
         const uint32_t p = p0 + tid;
-        if (p < p1) { nX = A.x[p]; nY = A.y[p]; nZ = A.z[p]; nvx = A.vx[p]; nvy = A.vy[p]; nvz = A.vz[p]; }
+        if (p < p1) { nX = A.x[p]; nY = A.y[p]; nZ = A.z[p]; nvx = A.vx[p]; nvy = A.vy[p]; nvz = A.vz[p]; nlc = A.lc[p]; }
     }
     for (uint32_t base = p0; base < p1; base += PM_THREADS) {
         const uint32_t p = base + tid;
         const bool act = p < p1;
         Real X = nX, Y = nY, Z = nZ, vx = nvx, vy = nvy, vz = nvz;
+        const int lc = nlc;
         {
             const uint32_t pn = p + PM_THREADS;
-            if (pn < p1) { nX = A.x[pn]; nY = A.y[pn]; nZ = A.z[pn]; nvx = A.vx[pn]; nvy = A.vy[pn]; nvz = A.vz[pn]; }
+            if (pn < p1) {
+                nX = A.x[pn]; nY = A.y[pn]; nZ = A.z[pn]; nvx = A.vx[pn]; nvy = A.vy[pn]; nvz = A.vz[pn];
+                nlc = A.lc[pn];
+            }
         }
         uint32_t key = deadkey;
         bool clamped = false, dropped = false;
+        uint8_t nlcell = 0;  // destination cell inside the destination tile
         if (act && X >= Real(0)) {
-            int lo = 0, hi = NS;  // ss[lo] <= p < ss[hi]
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (ss[mid] <= p) lo = mid; else hi = mid;
-            }
-            const int lc = lo >> 3;
             int li, lj, lk;
             local_ijk(g, lc, li, lj, lk);
             const int i = ox + li, j = oy + lj, k = oz + lk;
@@ -215,6 +212,9 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
                 if (Z >= Real(1)) Z = below_one<Real>();
                 const int o = (X >= Real(0.5)) + 2 * (Y >= Real(0.5)) + 4 * (Z >= Real(0.5));
                 key = (uint32_t)(((li + sx + 1) + g.ex * ((lj + sy + 1) + g.ey * (lk + sz + 1))) * 8 + o);
+                const int ni = wrap_axis(i + sx, g.nx, g.per[0]), nj = wrap_axis(j + sy, g.ny, g.per[1]),
+                          nk = wrap_axis(k + sz, g.nz, g.per[2]);
+                nlcell = (uint8_t)((ni & (g.tx - 1)) + ((nj & (g.ty - 1)) << g.ltx) + ((nk & (g.tz - 1)) << (g.ltx + g.lty)));
             } else {
                 dropped = true;
             }
@@ -240,6 +240,7 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
                 if (dst < cap) {
                     D.x[dst] = X; D.y[dst] = Y; D.z[dst] = Z;
                     D.vx[dst] = vx; D.vy[dst] = vy; D.vz[dst] = vz;
+                    D.lc[dst] = nlcell;
                 } else {
                     atomicAdd(counters + C_RANK_OVERFLOW, 1ull);  // slot 0
                 }
@@ -377,11 +378,49 @@ __device__ __forceinline__ void cic_accumulate(Real (&acc)[4][8], bool hx, bool 
 
 // LDS of the deposit: half of the subcell sums (2 quantities x 8 corners) as
 // [quantity*8+corner][octant][cell], each octant block padded by one word so
-// that neighbouring cells fall in different banks; the extended-region
-// coordinates of the tile; the first half's results.
+// that neighbouring cells fall in different banks; the first half's results.
 template <typename Real>
 RH_HD size_t deposit_lds_bytes(const Geom& g) {
-    return sizeof(Real) * 16 * 8 * (size_t)(g.tc + 1) + 4 * (size_t)g.next + sizeof(Real) * 2 * (size_t)g.next;
+    return sizeof(Real) * (16 * 8 * (size_t)(g.tc + 1) + 2 * (size_t)g.next);
+}
+
+// For every cell of the extended region, the LDS offsets (corner plane,
+// octant block, cell) of the subcell sums that reach it: cell + half - 1 +
+// corner = target along each axis.  The same for every tile of a shape;
+// built once on the host, at most 64 entries per cell.
+inline int deposit_gather_lists(const Geom& g, uint16_t* list, uint8_t* count) {
+    const int OB = g.tc + 1, QS = 8 * OB;
+    for (int e = 0; e < g.next; ++e) {
+        const int lx = e % g.ex - 1, ly = (e / g.ex) % g.ey - 1, lz = e / (g.ex * g.ey) - 1;
+        int m = 0;
+        for (int hz = 0; hz < 2; ++hz)
+            for (int kz = 0; kz < 2; ++kz) {
+                const int cz = lz + 1 - hz - kz;
+                if (cz < 0 || cz >= g.tz) continue;
+                for (int hy = 0; hy < 2; ++hy)
+                    for (int ky = 0; ky < 2; ++ky) {
+                        const int cy = ly + 1 - hy - ky;
+                        if (cy < 0 || cy >= g.ty) continue;
+                        for (int hx = 0; hx < 2; ++hx)
+                            for (int kx = 0; kx < 2; ++kx) {
+                                const int cx = lx + 1 - hx - kx;
+                                if (cx < 0 || cx >= g.tx) continue;
+                                const int cell = cx + g.tx * (cy + g.ty * cz);
+                                const int corner = kx + 2 * ky + 4 * kz, oct = hx + 2 * hy + 4 * hz;
+                                list[e * 64 + m++] = (uint16_t)(corner * QS + oct * OB + cell);
+                            }
+                    }
+            }
+        // ascending offsets: a fixed order that also walks LDS forward
+        for (int a = 1; a < m; ++a)
+            for (int b = a; b > 0 && list[e * 64 + b - 1] > list[e * 64 + b]; --b) {
+                const uint16_t t = list[e * 64 + b];
+                list[e * 64 + b] = list[e * 64 + b - 1];
+                list[e * 64 + b - 1] = t;
+            }
+        count[e] = (uint8_t)m;
+    }
+    return 0;
 }
 
 template <typename Real>
@@ -389,20 +428,16 @@ struct alignas(4 * sizeof(Real)) Real4 { Real v[4]; };
 
 template <typename Real>
 __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const uint32_t* __restrict__ sub,
-                                                Real* __restrict__ partial) {
+                                                const uint16_t* __restrict__ glist,
+                                                const uint8_t* __restrict__ gcount, Real* __restrict__ partial) {
     extern __shared__ unsigned char smem_raw[];
     const int NS = g.tc * 8;
     const int OB = g.tc + 1;          // octant block (padded)
     const int QS = 8 * OB;            // one (quantity, corner) plane
     Real* S = reinterpret_cast<Real*>(smem_raw);
-    uint32_t* extc = reinterpret_cast<uint32_t*>(smem_raw + sizeof(Real) * 16 * QS);
+    Real* R = S + 16 * QS;  // first-half results, next x 2
     const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
     const int ls = threadIdx.x;
-    for (int e = ls; e < g.next; e += blockDim.x) {
-        const int exy = g.ex * g.ey;
-        const int cz = e / exy, r = e - cz * exy, cy = r / g.ex, cx = r - cy * g.ex;
-        extc[e] = (uint32_t)cx | ((uint32_t)cy << 8) | ((uint32_t)cz << 16);
-    }
     const uint32_t key = (uint32_t)(T * NS + ls);
     const uint32_t p0 = sub[key], p1 = sub[key + 1];
     const int o = ls & 7, cell = ls >> 3;
@@ -414,7 +449,6 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
         for (int c = 0; c < 8; ++c) acc[q][c] = Real(0);
     for (uint32_t p = p0; p < p1; ++p) cic_accumulate(acc, hx, hy, hz, A.x[p], A.y[p], A.z[p], A.vx[p], A.vy[p], A.vz[p]);
     Real4<Real>* out = reinterpret_cast<Real4<Real>*>(partial) + (size_t)T * g.next;
-    Real* R = reinterpret_cast<Real*>(extc + g.next);  // first-half results, next x 2
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
         if (half) __syncthreads();  // the first half's reads are done
@@ -424,27 +458,14 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
             for (int c = 0; c < 8; ++c) S[(qq * 8 + c) * QS + o * OB + cell] = acc[2 * half + qq][c];
         __syncthreads();
         for (int e = ls; e < g.next; e += blockDim.x) {
-            const uint32_t ec = extc[e];
-            const int lx = (int)(ec & 255) - 1, ly = (int)((ec >> 8) & 255) - 1, lz = (int)(ec >> 16) - 1;
-            // per axis: (cell, half, corner) with cell + half - 1 + corner = target
-            int cx[4], bx[4], nx_ = 0, cy[4], by[4], ny_ = 0, cz[4], bz[4], nz_ = 0;
-            for (int h = 0; h < 2; ++h)
-                for (int kk = 0; kk < 2; ++kk) {
-                    const int c1 = lx + 1 - h - kk, c2 = ly + 1 - h - kk, c3 = lz + 1 - h - kk;
-                    if (c1 >= 0 && c1 < g.tx) { cx[nx_] = c1 + h * OB; bx[nx_++] = kk; }
-                    if (c2 >= 0 && c2 < g.ty) { cy[ny_] = (c2 << g.ltx) + 2 * h * OB; by[ny_++] = 2 * kk; }
-                    if (c3 >= 0 && c3 < g.tz) { cz[nz_] = (c3 << (g.ltx + g.lty)) + 4 * h * OB; bz[nz_++] = 4 * kk; }
-                }
-            Real res[2];
-#pragma unroll
-            for (int qq = 0; qq < 2; ++qq) {
-                Real sum = Real(0);
-                const Real* Sq = S + (size_t)qq * 8 * QS;
-                for (int c = 0; c < nz_; ++c)
-                    for (int b = 0; b < ny_; ++b)
-                        for (int a = 0; a < nx_; ++a)
-                            sum += Sq[(bx[a] + by[b] + bz[c]) * QS + cx[a] + cy[b] + cz[c]];
-                res[qq] = sum;
+            // the subcell terms reaching this cell: a list shared by all tiles
+            const int cnt_e = gcount[e];
+            const uint16_t* lst = glist + (size_t)e * 64;
+            Real res[2] = {Real(0), Real(0)};
+            for (int t = 0; t < cnt_e; ++t) {
+                const int off = lst[t];
+                res[0] += S[off];
+                res[1] += S[8 * QS + off];
             }
             if (half == 0) {
                 R[2 * e] = res[0];
