@@ -70,11 +70,12 @@ template <> __device__ __forceinline__ double below_one<double>() { return 0x1.f
 //           tile's extended region; counts particles per slot;
 //   pass 1: the same, then writes each particle to (slot base + rank), the
 //           slot bases being the scan of all tiles' counts.
-// The tile's particles are taken 256 at a time; the rank of a particle is its
-// position among the tile's particles with the same slot in memory order (the
-// migration is a stable counting sort).  Within a chunk, each wave ranks its
-// lanes with ballots and publishes its per-slot group sizes; a lane's rank
-// adds the slot's running count and the group sizes of the earlier waves.
+// The tile's particles are taken 256 at a time.  Pass 0 counts slots with
+// integer LDS atomics.  In pass 1 the rank of a particle is its position among
+// the tile's particles with the same slot in memory order (the migration is a
+// stable counting sort): each wave ranks its lanes with ballots and publishes
+// its per-slot group sizes; a lane's rank adds the slot's running count and
+// the group sizes of the earlier waves.
 // The tile's fields are staged in LDS; removed particles carry x < 0.
 constexpr int PM_THREADS = 256;
 constexpr int PM_WAVES = PM_THREADS / 64;
@@ -219,6 +220,13 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
                 dropped = true;
             }
         }
+        if (pass == 0) {
+            // counts only: integer LDS atomics, so the order does not matter
+            if (key != deadkey) atomicAdd(&cnt[key], 1u);
+            count_event(counters, C_MAX_VI, clamped);
+            count_event(counters, C_DROPPED, dropped);
+            continue;
+        }
         // rank among this wave's lanes holding the same slot
         unsigned long long match = __ballot(key != deadkey);
         for (int b = 0; b < kbits; ++b) {
@@ -235,15 +243,13 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
         if (key != deadkey) {
             uint32_t rank = cnt[key] + rin;
             for (int v = 0; v < w; ++v) rank += grp[v * g.extk + key];
-            if (pass == 1) {
-                const uint32_t dst = cnt_tile[(size_t)T * g.extk + key] + rank;
-                if (dst < cap) {
-                    D.x[dst] = X; D.y[dst] = Y; D.z[dst] = Z;
-                    D.vx[dst] = vx; D.vy[dst] = vy; D.vz[dst] = vz;
-                    D.lc[dst] = nlcell;
-                } else {
-                    atomicAdd(counters + C_RANK_OVERFLOW, 1ull);  // slot 0
-                }
+            const uint32_t dst = cnt_tile[(size_t)T * g.extk + key] + rank;
+            if (dst < cap) {
+                D.x[dst] = X; D.y[dst] = Y; D.z[dst] = Z;
+                D.vx[dst] = vx; D.vy[dst] = vy; D.vz[dst] = vz;
+                D.lc[dst] = nlcell;
+            } else {
+                atomicAdd(counters + C_RANK_OVERFLOW, 1ull);  // slot 0
             }
         }
         __syncthreads();
@@ -251,12 +257,9 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
             atomicAdd(&cnt[key], gsz);  // integer: the order of the four waves does not matter
             grp[w * g.extk + key] = 0;
         }
-        if (pass == 0) {
-            count_event(counters, C_MAX_VI, clamped);
-            count_event(counters, C_DROPPED, dropped);
-        }
         __syncthreads();
     }
+    __syncthreads();
     if (pass == 0)
         for (int e = tid; e < g.extk; e += PM_THREADS) cnt_tile[(size_t)T * g.extk + e] = cnt[e];
 }
@@ -266,6 +269,18 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
 // the slot counts are replaced by the first destination index of each slot
 // (out must already hold the exclusive scan of the totals).  One block per
 // tile (3D grid of tiles), one thread per subcell of the tile.
+// axis_candidates for an axis with at least three tiles or without wrap: the
+// previous tile (cell on the tile's low face), the own tile, the next tile
+// (cell on the high face), in that order.
+__device__ __forceinline__ int axis_candidates_fast(int i, int lt, int nt, int per, int* tl, int* el) {
+    const int t = 1 << lt, t0 = i >> lt, l = i & (t - 1);
+    int m = 0;
+    if (l == 0 && (t0 > 0 || per)) { tl[m] = t0 > 0 ? t0 - 1 : nt - 1; el[m++] = t + 1; }
+    tl[m] = t0; el[m++] = l + 1;
+    if (l == t - 1 && (t0 < nt - 1 || per)) { tl[m] = t0 < nt - 1 ? t0 + 1 : 0; el[m++] = 0; }
+    return m;
+}
+
 template <bool WRITE_BASE>
 __global__ __launch_bounds__(512) void k_slot_offsets(Geom g, uint32_t* __restrict__ cnt_tile,
                                                      uint32_t* __restrict__ out) {
@@ -277,9 +292,13 @@ __global__ __launch_bounds__(512) void k_slot_offsets(Geom g, uint32_t* __restri
     local_ijk(g, tid >> 3, li, lj, lk);
     const int i = (blockIdx.x << g.ltx) + li, j = (blockIdx.y << g.lty) + lj, k = (blockIdx.z << g.ltz) + lk;
     int tx[6], ex[6], ty[6], ey[6], tz[6], ez[6];
-    const int mx = axis_candidates(i, g.nx, g.ltx, g.ntx, g.per[0], tx, ex);
-    const int my = axis_candidates(j, g.ny, g.lty, g.nty, g.per[1], ty, ey);
-    const int mz = axis_candidates(k, g.nz, g.ltz, g.ntz, g.per[2], tz, ez);
+    const bool fx = !g.per[0] || g.ntx >= 3, fy = !g.per[1] || g.nty >= 3, fz = !g.per[2] || g.ntz >= 3;
+    const int mx = fx ? axis_candidates_fast(i, g.ltx, g.ntx, g.per[0], tx, ex)
+                      : axis_candidates(i, g.nx, g.ltx, g.ntx, g.per[0], tx, ex);
+    const int my = fy ? axis_candidates_fast(j, g.lty, g.nty, g.per[1], ty, ey)
+                      : axis_candidates(j, g.ny, g.lty, g.nty, g.per[1], ty, ey);
+    const int mz = fz ? axis_candidates_fast(k, g.ltz, g.ntz, g.per[2], tz, ez)
+                      : axis_candidates(k, g.nz, g.ltz, g.ntz, g.per[2], tz, ez);
     uint32_t run = WRITE_BASE ? out[key] : 0u;
     for (int c = 0; c < mz; ++c)
         for (int b = 0; b < my; ++b)
@@ -318,34 +337,53 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n,
                                                              uint32_t* __restrict__ bsum) {
     __shared__ uint32_t smem[SCAN_THREADS];
-    const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    const size_t base = (size_t)blockIdx.x * SCAN_TILE;
     uint32_t s = 0;
-    for (int q = 0; q < SCAN_ITEMS; ++q)
-        if (base + q < n) s += in[base + q];
+    for (int q = 0; q < SCAN_ITEMS; ++q) {  // coalesced: item q of thread t at base + q*threads + t
+        const size_t i = base + (size_t)q * SCAN_THREADS + threadIdx.x;
+        if (i < n) s += in[i];
+    }
     uint32_t tot;
     block_exclusive_scan(s, smem, tot);
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
+// In-place safe: a block reads its whole tile into LDS (coalesced) before any
+// write.  LDS element j sits at j + j/SCAN_ITEMS, so that the per-thread runs
+// of SCAN_ITEMS consecutive elements do not share banks.
+__device__ __forceinline__ int scan_pad(int j) { return j + j / SCAN_ITEMS; }
+
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const uint32_t* __restrict__ in, uint32_t n,
                                                             const uint32_t* __restrict__ boff,
                                                             uint32_t* __restrict__ out) {
     __shared__ uint32_t smem[SCAN_THREADS];
-    const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    __shared__ uint32_t tile[SCAN_TILE + SCAN_THREADS];
+    const size_t base = (size_t)blockIdx.x * SCAN_TILE;
+    for (int q = 0; q < SCAN_ITEMS; ++q) {
+        const size_t i = base + (size_t)q * SCAN_THREADS + threadIdx.x;
+        tile[scan_pad(q * SCAN_THREADS + threadIdx.x)] = i < n ? in[i] : 0u;
+    }
+    __syncthreads();
+    const int t0 = threadIdx.x * (SCAN_ITEMS + 1);
     uint32_t v[SCAN_ITEMS];
     uint32_t s = 0;
     for (int q = 0; q < SCAN_ITEMS; ++q) {
-        v[q] = (base + q < n) ? in[base + q] : 0u;
+        v[q] = tile[t0 + q];
         s += v[q];
     }
     uint32_t tot;
     uint32_t run = block_exclusive_scan(s, smem, tot) + boff[blockIdx.x];
     for (int q = 0; q < SCAN_ITEMS; ++q) {
-        if (base + q < n) out[base + q] = run;
+        tile[t0 + q] = run;
         run += v[q];
     }
+    __syncthreads();
+    for (int q = 0; q < SCAN_ITEMS; ++q) {
+        const size_t i = base + (size_t)q * SCAN_THREADS + threadIdx.x;
+        if (i < n) out[i] = tile[scan_pad(q * SCAN_THREADS + threadIdx.x)];
+    }
     // the grand total goes one past the end
-    if (base + SCAN_ITEMS >= n && base < n) out[n] = run;
+    if (base + SCAN_TILE >= n && threadIdx.x == SCAN_THREADS - 1) out[n] = boff[blockIdx.x] + tot;
 }
 
 // Cloud-in-cell deposit (Accumulator::accumulateCell,
@@ -428,8 +466,8 @@ struct alignas(4 * sizeof(Real)) Real4 { Real v[4]; };
 
 template <typename Real>
 __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const uint32_t* __restrict__ sub,
-                                                const uint16_t* __restrict__ glist,
-                                                const uint8_t* __restrict__ gcount, Real* __restrict__ partial) {
+                                                 const uint16_t* __restrict__ glist,
+                                                 const uint8_t* __restrict__ gcount, Real* __restrict__ partial) {
     extern __shared__ unsigned char smem_raw[];
     const int NS = g.tc * 8;
     const int OB = g.tc + 1;          // octant block (padded)
