@@ -99,6 +99,12 @@ def lib():
         L.orc_neumann_cell.argtypes = [vp, vp, C.c_int]
         L.orc_neumann_face.argtypes = [vp, vp]
         L.orc_face_curl.argtypes = [vp, vp, vp, C.c_double]
+        L.orc_set_imf_cell.argtypes = [vp, vp]
+        L.orc_set_imf_face.argtypes = [vp, vp]
+        L.orc_node_avg.argtypes = [vp, vp, vp]
+        for name in ["orc_calc_node_J", "orc_calc_cell_Ue", "orc_calc_node_Ue", "orc_upwind_node_B",
+                     "orc_calc_node_E", "orc_calc_cell_Ep"]:
+            getattr(L, name).argtypes = [vp]
         _lib = L
     return _lib
 

@@ -654,7 +654,7 @@ void orc_neumann_face(orc_sim* s, double* D) {
 }
 
 /* setIMFCell (src/hybrid_propagator.cpp:910-977). */
-static void set_imf_cell(orc_sim* s, double* cellB) {
+void orc_set_imf_cell(orc_sim* s, double* cellB) {
     const uint32_t dm[6] = {X_POS, X_NEG, Y_POS, Y_NEG, Z_POS, Z_NEG};
     for (int e = 0; e < s->nexterior; ++e) {
         const int n = s->exterior[e];
@@ -666,7 +666,7 @@ static void set_imf_cell(orc_sim* s, double* cellB) {
 }
 
 /* setIMFFace (src/hybrid_propagator.cpp:979-1040): tangential components. */
-static void set_imf_face(orc_sim* s, double* faceB) {
+void orc_set_imf_face(orc_sim* s, double* faceB) {
     const uint32_t dm[6] = {X_POS, X_NEG, Y_POS, Y_NEG, Z_POS, Z_NEG};
     for (int e = 0; e < s->nexterior; ++e) {
         const int n = s->exterior[e];
@@ -681,7 +681,7 @@ static void set_imf_face(orc_sim* s, double* faceB) {
 
 /* calcNodeJ (src/hybrid_propagator.cpp:1449-1522): edge currents from the
  * curl of faceB, averaged to nodes, with the whistler-speed limiter. */
-static void calc_node_J(orc_sim* s) {
+void orc_calc_node_J(orc_sim* s) {
     const double* faceB = s->f[ORC_FACE_B];
     const double* nodeB = s->f[ORC_NODE_B];
     const double* nodeRho = s->f[ORC_NODE_RHOQ];
@@ -716,7 +716,7 @@ static void calc_node_J(orc_sim* s) {
 }
 
 /* calcCellUe (src/hybrid_propagator.cpp:1342-1387). */
-static void calc_cell_Ue(orc_sim* s) {
+void orc_calc_cell_Ue(orc_sim* s) {
     const double* J = s->f[ORC_CELL_J];
     const double* Ji = s->f[ORC_CELL_JI];
     const double* rho = s->f[ORC_CELL_RHOQ];
@@ -748,7 +748,7 @@ static void calc_cell_Ue(orc_sim* s) {
 }
 
 /* calcNodeUe (src/hybrid_propagator.cpp:1524-1584); clamps nodeRhoQi in place. */
-static void calc_node_Ue(orc_sim* s) {
+void orc_calc_node_Ue(orc_sim* s) {
     double* rho = s->f[ORC_NODE_RHOQ];
     const double* Ji = s->f[ORC_NODE_JI];
     const double* J = s->f[ORC_NODE_J];
@@ -782,7 +782,7 @@ static void calc_node_Ue(orc_sim* s) {
 }
 
 /* upwindNodeB (src/hybrid_propagator.cpp:1268-1339). */
-static void upwind_node_B(orc_sim* s) {
+void orc_upwind_node_B(orc_sim* s) {
     const double* cellB = s->f[ORC_CELL_B];
     const double* Ue = s->f[ORC_NODE_UE];
     double* nodeB = s->f[ORC_NODE_B];
@@ -820,7 +820,7 @@ static void upwind_node_B(orc_sim* s) {
 }
 
 /* calcNodeE (src/hybrid_propagator.cpp:1391-1446): E = -Ue x B + eta J. */
-static void calc_node_E(orc_sim* s) {
+void orc_calc_node_E(orc_sim* s) {
     const double* Ue = s->f[ORC_NODE_UE];
     const double* B = s->f[ORC_NODE_B];
     const double* eta = s->f[ORC_NODE_ETA];
@@ -854,7 +854,7 @@ static void calc_node_E(orc_sim* s) {
  * the sentinel and value-initialises the rest to zero; so only a missing
  * (-1,-1,-1) neighbour's x component is recognised as missing and the other
  * missing entries count as zeros with their full weight.  Restated as is. */
-static void node_avg(orc_sim* s, const double* oldE, double* E) {
+void orc_node_avg(orc_sim* s, const double* oldE, double* E) {
     const double initVal = DBL_MAX;
     const double* C = s->p.gauss_coeffs;
 #pragma omp parallel for schedule(static)
@@ -926,7 +926,7 @@ void orc_face_curl(orc_sim* s, const double* node, double* face, double dt) {
 }
 
 /* calcCellEp (src/hybrid_propagator.cpp:1674-1737). */
-static void calc_cell_Ep(orc_sim* s) {
+void orc_calc_cell_Ep(orc_sim* s) {
     const double* nodeRho = s->f[ORC_NODE_RHOQ];
     const double* cellRho = s->f[ORC_CELL_RHOQ];
     double* Ep = s->f[ORC_CELL_EP];
@@ -965,19 +965,19 @@ void orc_propagate_B(orc_sim* s) {
     orc_neumann_cell(s, f[ORC_CELL_B], 3);
     orc_neumann_cell(s, f[ORC_CELL_JI], 3);
     orc_neumann_cell(s, f[ORC_CELL_RHOQ], 1);
-    set_imf_cell(s, f[ORC_CELL_B]);
+    orc_set_imf_cell(s, f[ORC_CELL_B]);
     orc_cell2node(s, f[ORC_CELL_B], f[ORC_NODE_B], 3);
     orc_cell2node(s, f[ORC_CELL_RHOQ], f[ORC_NODE_RHOQ], 1);
     orc_cell2node(s, f[ORC_CELL_JI], f[ORC_NODE_JI], 3);
     orc_neumann_face(s, f[ORC_FACE_B]);
-    set_imf_face(s, f[ORC_FACE_B]);
-    calc_node_J(s);
+    orc_set_imf_face(s, f[ORC_FACE_B]);
+    orc_calc_node_J(s);
     orc_node2cell(s, f[ORC_NODE_J], f[ORC_CELL_J]);
-    calc_cell_Ue(s);
+    orc_calc_cell_Ue(s);
     orc_neumann_cell(s, f[ORC_CELL_UE], 3);
-    calc_node_Ue(s);
-    upwind_node_B(s);
-    calc_node_E(s);
+    orc_calc_node_Ue(s);
+    orc_upwind_node_B(s);
+    orc_calc_node_E(s);
     for (int i = 0; i < s->p.n_linear_smoothings; ++i) {
         orc_node2cell(s, f[ORC_NODE_E], f[ORC_CELL_J]);
         orc_neumann_cell(s, f[ORC_CELL_J], 3);
@@ -987,11 +987,11 @@ void orc_propagate_B(orc_sim* s) {
     if (s->p.use_gaussian) {
         double* old = (double*)malloc(sizeof(double) * 3 * (size_t)s->nc);
         memcpy(old, f[ORC_NODE_E], sizeof(double) * 3 * (size_t)s->nc);
-        node_avg(s, old, f[ORC_NODE_E]);
+        orc_node_avg(s, old, f[ORC_NODE_E]);
         free(old);
     }
     orc_face_curl(s, f[ORC_NODE_E], f[ORC_FACE_B], s->p.dt);
-    if (s->p.use_e_pressure) calc_cell_Ep(s);
+    if (s->p.use_e_pressure) orc_calc_cell_Ep(s);
 }
 
 /* propagate() (src/user.cpp:118-134) without logging, detectors and

@@ -119,6 +119,16 @@ void orc_node2cell(orc_sim* s, const double* node, double* cell);
 void orc_neumann_cell(orc_sim* s, double* cell, int dim);
 void orc_neumann_face(orc_sim* s, double* face);
 void orc_face_curl(orc_sim* s, const double* node, double* face, double dt);
+void orc_set_imf_cell(orc_sim* s, double* cellB);
+void orc_set_imf_face(orc_sim* s, double* faceB);
+/* the stencils below read and write the simulation's own field arrays */
+void orc_calc_node_J(orc_sim* s);
+void orc_calc_cell_Ue(orc_sim* s);
+void orc_calc_node_Ue(orc_sim* s);
+void orc_upwind_node_B(orc_sim* s);
+void orc_calc_node_E(orc_sim* s);
+void orc_node_avg(orc_sim* s, const double* oldE, double* E);
+void orc_calc_cell_Ep(orc_sim* s);
 
 /* Field gather for one point (cell-local metres) as the push sees it. */
 void orc_get_fields(orc_sim* s, int64_t cell, const double r[3], double B[3], double Ue[3], double Ep[3]);

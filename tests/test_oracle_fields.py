@@ -1,0 +1,513 @@
+"""Closed-form checks of the oracle's field stencils
+(src/hybrid_propagator.cpp:42-1737 as restated in oracle/rh_oracle.c).
+
+Each stencil is checked against a property fixed by the mathematics, not by
+re-typing its formula: interpolations of (tri)linear fields are exact, the
+discrete curl of a linear field is its exact curl, the face curl keeps the
+discrete divergence zero, zero-gradient (Neumann) exteriors are copies of the
+nearest interior cell, Ohm's law is a cross product, the Gaussian average is
+a normalised symmetric stencil.  A transposed operand, a wrong sign, a wrong
+neighbour offset or a dropped term fails at least one of them.
+"""
+import numpy as np
+import pytest
+
+from rhybrid_amd import problem as P
+
+MU0 = P.MU0
+
+
+def make(nx, ny, nz, periodic=(0, 0, 0), dx=1e5, **params):
+    grid = P.Grid(nx, ny, nz, tuple(periodic), dx, 0.0, 0.0, 0.0)
+    p = P.Params(dt=params.pop("dt", 0.1), **params)
+    return P.Problem("stencil", grid, p, [], P.make_fields(grid), P.make_flags(grid), [])
+
+
+def ijk(g):
+    return [a.astype(np.float64) for a in g.ijk()]
+
+
+def exists(g, a, b, c):
+    """cell (i+a, j+b, k+c) exists for every cell (boolean per cell)."""
+    ok = np.ones(g.nc, dtype=bool)
+    for d, (idx, off, n) in enumerate(zip(g.ijk(), (a, b, c), (g.nx, g.ny, g.nz))):
+        if not g.periodic[d]:
+            ok &= (idx + off >= 0) & (idx + off <= n - 1)
+    return ok
+
+
+def upper_ok(g):
+    return exists(g, 1, 0, 0) & exists(g, 0, 1, 0) & exists(g, 0, 0, 1)
+
+
+def trilinear(rng):
+    c = rng.standard_normal(8)
+    return lambda x, y, z: (c[0] + c[1] * x + c[2] * y + c[3] * z + c[4] * x * y + c[5] * y * z
+                            + c[6] * x * z + c[7] * x * y * z)
+
+
+def linear(rng):
+    c = rng.standard_normal(4)
+    return (lambda x, y, z: c[0] + c[1] * x + c[2] * y + c[3] * z), c[1:]
+
+
+def ptr(a):
+    import ctypes
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---------------------------------------------------------------- interpolation
+
+def test_face2cell_linear_exact(oracle_mod):
+    pr = make(6, 5, 7)
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    rng = np.random.default_rng(1)
+    i, j, k = ijk(g)
+    face = np.zeros(3 * g.nc)
+    expect = np.full(3 * g.nc, 777.0)
+    inside = g.all_neighbours()
+    for l in range(3):
+        f, _ = linear(rng)
+        X = [i + 0.5, j + 0.5, k + 0.5]
+        Xf = list(X)
+        Xf[l] = Xf[l] + 0.5   # the face of component l is on the cell's upper side
+        face[l::3] = f(*Xf)
+        expect[l::3] = np.where(inside, f(*X), 777.0)
+    cell = np.full(3 * g.nc, 777.0)
+    oracle_mod.lib().orc_face2cell(o.raw(), ptr(face), ptr(cell))
+    np.testing.assert_allclose(cell, expect, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("dim", [1, 3])
+def test_cell2node_trilinear_exact(oracle_mod, dim):
+    pr = make(5, 6, 4)
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    rng = np.random.default_rng(2 + dim)
+    i, j, k = ijk(g)
+    cell = np.zeros(dim * g.nc)
+    expect = np.full(dim * g.nc, 555.0)
+    m = upper_ok(g)
+    for l in range(dim):
+        f = trilinear(rng)
+        cell[l::dim] = f(i + 0.5, j + 0.5, k + 0.5)
+        expect[l::dim] = np.where(m, f(i + 1.0, j + 1.0, k + 1.0), 555.0)  # node = upper corner
+    node = np.full(dim * g.nc, 555.0)
+    oracle_mod.lib().orc_cell2node(o.raw(), ptr(cell), ptr(node), dim)
+    np.testing.assert_allclose(node, expect, rtol=1e-12, atol=1e-12)
+
+
+def test_node2cell_trilinear_exact(oracle_mod):
+    pr = make(5, 4, 6)
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    rng = np.random.default_rng(7)
+    i, j, k = ijk(g)
+    node = np.zeros(3 * g.nc)
+    expect = np.full(3 * g.nc, 333.0)
+    inside = g.all_neighbours()
+    for l in range(3):
+        f = trilinear(rng)
+        node[l::3] = f(i + 1.0, j + 1.0, k + 1.0)
+        expect[l::3] = np.where(inside, f(i + 0.5, j + 0.5, k + 0.5), 333.0)
+    cell = np.full(3 * g.nc, 333.0)
+    oracle_mod.lib().orc_node2cell(o.raw(), ptr(node), ptr(cell))
+    np.testing.assert_allclose(cell, expect, rtol=1e-12, atol=1e-12)
+
+
+def test_interpolation_periodic_wrap(oracle_mod):
+    """In a periodic box, a cell->node average of a field that is a function of
+    one axis only equals the mean of the two wrapped neighbours along it."""
+    pr = make(4, 5, 6, periodic=(1, 1, 1))
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    i, j, k = g.ijk()
+    for axis, (idx, n) in enumerate(zip((i, j, k), (g.nx, g.ny, g.nz))):
+        vals = np.random.default_rng(axis).standard_normal(n)
+        cell = vals[idx].astype(np.float64).copy()
+        node = np.zeros(g.nc)
+        oracle_mod.lib().orc_cell2node(o.raw(), ptr(cell), ptr(node), 1)
+        np.testing.assert_allclose(node, 0.5 * (vals[idx] + vals[(idx + 1) % n]), rtol=1e-14, atol=1e-15)
+
+
+# ---------------------------------------------------------------- Neumann / IMF
+
+def clamp_source(g, mask_axes):
+    """index of the nearest interior cell, clamping only the listed axes."""
+    i, j, k = g.ijk()
+    out = []
+    for d, (idx, n) in enumerate(zip((i, j, k), (g.nx, g.ny, g.nz))):
+        if d in mask_axes and not g.periodic[d]:
+            idx = np.clip(idx, 1, n - 2)
+        out.append(idx)
+    return out[0] + g.nx * (out[1] + g.ny * out[2])
+
+
+@pytest.mark.parametrize("periodic", [(0, 0, 0), (0, 1, 0), (1, 0, 1)])
+@pytest.mark.parametrize("reverse", [False, True])
+def test_neumann_cell_is_nearest_interior_copy(oracle_mod, periodic, reverse):
+    pr = make(5, 6, 4, periodic=periodic)
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    o.set_exterior_order_reversed(reverse)
+    rng = np.random.default_rng(11)
+    for dim in (1, 3):
+        D = rng.standard_normal(dim * g.nc)
+        src = clamp_source(g, (0, 1, 2))
+        expect = D.reshape(g.nc, dim)[src].reshape(-1)
+        oracle_mod.lib().orc_neumann_cell(o.raw(), ptr(D), dim)
+        np.testing.assert_array_equal(D, expect)
+
+
+@pytest.mark.parametrize("periodic", [(0, 0, 0), (0, 0, 1), (1, 1, 0)])
+@pytest.mark.parametrize("reverse", [False, True])
+def test_neumann_face_tangential_copy(oracle_mod, periodic, reverse):
+    """Component l of faceB sits on a face normal to axis l: the exterior copy
+    clamps the two other axes into the interior and leaves axis l alone (the
+    normal component on a wall is not set by the zero-gradient rule)."""
+    pr = make(5, 4, 6, periodic=periodic)
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    o.set_exterior_order_reversed(reverse)
+    rng = np.random.default_rng(12)
+    D = rng.standard_normal(3 * g.nc)
+    expect = np.empty_like(D)
+    for l in range(3):
+        src = clamp_source(g, tuple(a for a in range(3) if a != l))
+        expect[l::3] = D[l::3][src]
+    oracle_mod.lib().orc_neumann_face(o.raw(), ptr(D))
+    np.testing.assert_array_equal(D, expect)
+
+
+WALLS = [("x+", 0, +1), ("x-", 0, -1), ("y+", 1, +1), ("y-", 1, -1), ("z+", 2, +1), ("z-", 2, -1)]
+
+
+@pytest.mark.parametrize("w", range(6))
+def test_imf_cell_and_face_on_flagged_wall(oracle_mod, w):
+    flags = [0] * 6
+    flags[w] = 1
+    imf = (1.5e-9, -2.5e-9, 0.7e-9)
+    pr = make(4, 5, 6, imf=imf, imf_cell=tuple(flags), imf_face=tuple(flags))
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    rng = np.random.default_rng(w)
+    cB = rng.standard_normal(3 * g.nc)
+    fB = rng.standard_normal(3 * g.nc)
+    c0, f0 = cB.copy(), fB.copy()
+    oracle_mod.lib().orc_set_imf_cell(o.raw(), ptr(cB))
+    oracle_mod.lib().orc_set_imf_face(o.raw(), ptr(fB))
+    _, axis, sgn = WALLS[w]
+    idx = g.ijk()[axis]
+    n = (g.nx, g.ny, g.nz)[axis]
+    on_wall = idx == (n - 1 if sgn > 0 else 0)
+    ec, ef = c0.reshape(-1, 3).copy(), f0.reshape(-1, 3).copy()
+    ec[on_wall] = imf
+    for l in range(3):
+        if l != axis:
+            ef[on_wall, l] = imf[l]
+    np.testing.assert_array_equal(cB, ec.reshape(-1))
+    np.testing.assert_array_equal(fB, ef.reshape(-1))
+
+
+# ---------------------------------------------------------------- currents, Ohm's law
+
+def linear_faceB(g, rng, scale=1e-9):
+    """faceB sampled from B(X) = B0 + M X (X in cells); returns the exact curl
+    (per metre)."""
+    i, j, k = ijk(g)
+    B0 = rng.standard_normal(3) * scale
+    M = rng.standard_normal((3, 3)) * scale
+    F = np.zeros(3 * g.nc)
+    for l in range(3):
+        X = [i + 0.5, j + 0.5, k + 0.5]
+        X[l] = X[l] + 0.5
+        F[l::3] = B0[l] + M[l, 0] * X[0] + M[l, 1] * X[1] + M[l, 2] * X[2]
+    curl = np.array([M[2, 1] - M[1, 2], M[0, 2] - M[2, 0], M[1, 0] - M[0, 1]]) / g.dx
+    return F, curl
+
+
+def test_node_J_is_curl_B_over_mu0(oracle_mod):
+    pr = make(6, 5, 5)
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    F, curl = linear_faceB(g, np.random.default_rng(21))
+    o.field("faceB")[:] = F
+    o.field("nodeJ")[:] = 99.0
+    oracle_mod.lib().orc_calc_node_J(o.raw())
+    J = o.field("nodeJ").reshape(-1, 3)
+    m = upper_ok(g)
+    np.testing.assert_allclose(J[m], np.broadcast_to(curl / MU0, J[m].shape), rtol=1e-9, atol=1e-9 * np.abs(curl / MU0).max())
+    assert np.all(J[~m] == 99.0)
+
+
+def test_node_J_whistler_limiter(oracle_mod):
+    """Where the grid whistler speed 2 pi |B| / (mu0 n e dx) exceeds the
+    limit, J is scaled down by exactly limit / speed and the node is counted."""
+    vmax = 1000e3
+    pr = make(5, 5, 5, periodic=(1, 1, 1), max_whistler=vmax)
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    rng = np.random.default_rng(22)
+    o.field("faceB")[:] = rng.standard_normal(3 * g.nc) * 1e-9
+    B = rng.standard_normal((g.nc, 3)) * 5e-9
+    ne = np.where(rng.random(g.nc) < 0.5, 1e4, 1e8)   # thin nodes exceed the limit
+    o.field("nodeB")[:] = B.reshape(-1)
+    o.field("nodeRhoQi")[:] = ne * P.QE
+    oracle_mod.lib().orc_calc_node_J(o.raw())
+    J_lim = o.field("nodeJ").reshape(-1, 3).copy()
+    o.field("nodeRhoQi")[:] = 1e30     # no limit anywhere
+    oracle_mod.lib().orc_reset_counters(o.raw())
+    oracle_mod.lib().orc_calc_node_J(o.raw())
+    J = o.field("nodeJ").reshape(-1, 3)
+    vw = 2 * np.pi * np.linalg.norm(B, axis=1) / (MU0 * ne * P.QE * g.dx)
+    factor = np.where(vw > vmax, vmax / vw, 1.0)
+    np.testing.assert_allclose(J_lim, J * factor[:, None], rtol=1e-12, atol=0)
+    assert (vw > vmax).sum() > 10
+
+
+@pytest.mark.parametrize("hall", [1, 0])
+def test_electron_velocity(oracle_mod, hall):
+    """Ue = Ui - J/(n e) with the Hall term, Ui without it (Ji = rho Ui),
+    obstacle value inside the inner boundary, 0 where rho = 0, and the speed
+    clamp keeps the direction."""
+    vmax = 300e3
+    pr = make(5, 5, 5, periodic=(1, 1, 1), use_hall=hall, max_electron_speed2=vmax ** 2,
+              obstacle_ue=(1.0, 2.0, 3.0))
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    rng = np.random.default_rng(30 + hall)
+    rho = rng.uniform(0.5, 2.0, g.nc) * 1e6 * P.QE
+    rho[::17] = 0.0
+    Ui = rng.standard_normal((g.nc, 3)) * 100e3
+    Ue_true = rng.standard_normal((g.nc, 3)) * 100e3
+    J = rho[:, None] * (Ui - Ue_true)
+    inner = np.zeros(g.nc, dtype=np.uint8)
+    inner[5::23] = 1
+    for pre, fl in (("cell", "innerFlagField"), ("node", "innerFlagNode")):
+        o.field(pre + "RhoQi")[:] = rho
+        o.field(pre + "Ji")[:] = (rho[:, None] * Ui).reshape(-1)
+        o.field(pre + "J")[:] = J.reshape(-1)
+        o.flag(fl)[:] = inner
+    oracle_mod.lib().orc_calc_cell_Ue(o.raw())
+    oracle_mod.lib().orc_calc_node_Ue(o.raw())
+    want = Ue_true if hall else Ui
+    want = np.where(rho[:, None] > 0, want, 0.0)
+    sp = np.linalg.norm(want, axis=1)
+    want = np.where(sp[:, None] > vmax, want * (vmax / np.maximum(sp, 1e-300))[:, None], want)
+    want[inner.astype(bool)] = (1.0, 2.0, 3.0)
+    for name in ("cellUe", "nodeUe"):
+        np.testing.assert_allclose(o.field(name).reshape(-1, 3), want, rtol=1e-9, atol=1e-6)
+    c = o.counters()
+    n_clamped = int(((sp > vmax) & ~inner.astype(bool)).sum())
+    assert c["max_cell_ue"] == n_clamped and c["max_node_ue"] == n_clamped and n_clamped > 0
+
+
+def test_node_rho_floor(oracle_mod):
+    pr = make(4, 4, 4, periodic=(1, 1, 1), min_rho=5.0)
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    rho = np.random.default_rng(3).uniform(0, 10, g.nc)
+    o.field("nodeRhoQi")[:] = rho
+    oracle_mod.lib().orc_calc_node_Ue(o.raw())
+    np.testing.assert_array_equal(o.field("nodeRhoQi"), np.maximum(rho, 5.0))
+    assert o.counters()["min_node_rho"] == (rho < 5.0).sum()
+
+
+def test_upwind_node_B(oracle_mod):
+    """Uniform cellB is reproduced for any Ue; for a linear cellB and Ue along
+    an axis, the inverse-distance weights seen from the point half a cell
+    upwind (4 near cells at distance sqrt(1/2), 4 far at sqrt(3/2)) give
+    B(node) + g * 0.5*(w_far - w_near)/(w_far + w_near) along that axis."""
+    pr = make(5, 5, 5)
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    rng = np.random.default_rng(40)
+    m = upper_ok(g)
+    # uniform
+    B0 = rng.standard_normal(3)
+    o.field("cellB")[:] = np.tile(B0, g.nc)
+    o.field("nodeUe")[:] = rng.standard_normal(3 * g.nc) * 1e5
+    oracle_mod.lib().orc_upwind_node_B(o.raw())
+    np.testing.assert_allclose(o.field("nodeB").reshape(-1, 3)[m], np.broadcast_to(B0, (m.sum(), 3)), rtol=1e-13)
+    # linear, axis-aligned flow
+    i, j, k = ijk(g)
+    G = rng.standard_normal((3, 3))
+    cellB = np.stack([B0[l] + G[l, 0] * (i + 0.5) + G[l, 1] * (j + 0.5) + G[l, 2] * (k + 0.5) for l in range(3)], 1)
+    nodeB = np.stack([B0[l] + G[l, 0] * (i + 1) + G[l, 1] * (j + 1) + G[l, 2] * (k + 1) for l in range(3)], 1)
+    o.field("cellB")[:] = cellB.reshape(-1)
+    wn, wf = 1 / np.sqrt(0.5), 1 / np.sqrt(1.5)
+    shift = 0.5 * (wf - wn) / (wf + wn)   # < 0: biased upwind
+    for axis in range(3):
+        for s in (+1, -1, 0):
+            ue = np.zeros(3)
+            ue[axis] = s * 2e5
+            o.field("nodeUe")[:] = np.tile(ue, g.nc)
+            oracle_mod.lib().orc_upwind_node_B(o.raw())
+            a = axis if s != 0 else 0   # Ue = 0 takes the -x side
+            sg = s if s != 0 else 1
+            want = nodeB + sg * shift * G[:, a][None, :]
+            np.testing.assert_allclose(o.field("nodeB").reshape(-1, 3)[m], want[m], rtol=1e-12, atol=1e-12)
+
+
+def test_node_E_ohms_law(oracle_mod):
+    Emax = 2e-3
+    pr = make(5, 5, 5, periodic=(1, 1, 1), max_E2=Emax ** 2)
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    rng = np.random.default_rng(50)
+    Ue = rng.standard_normal((g.nc, 3)) * 3e5
+    B = rng.standard_normal((g.nc, 3)) * 5e-9
+    J = rng.standard_normal((g.nc, 3)) * 1e-8
+    eta = rng.uniform(0, 1e4, g.nc)
+    o.field("nodeUe")[:] = Ue.reshape(-1)
+    o.field("nodeB")[:] = B.reshape(-1)
+    o.field("nodeJ")[:] = J.reshape(-1)
+    o.field("nodeEta")[:] = eta
+    oracle_mod.lib().orc_calc_node_E(o.raw())
+    E = -np.cross(Ue, B) + eta[:, None] * J
+    mag = np.linalg.norm(E, axis=1)
+    E = np.where(mag[:, None] > Emax, E * (Emax / mag)[:, None], E)
+    np.testing.assert_allclose(o.field("nodeE").reshape(-1, 3), E, rtol=1e-12, atol=1e-18)
+    assert o.counters()["max_E"] == (mag > Emax).sum() > 0
+
+
+def test_gaussian_node_average(oracle_mod):
+    """The 27-node weights are normalised and symmetric: a uniform field is
+    unchanged everywhere in a periodic box, and a linear field is unchanged
+    at nodes whose 26 neighbours all exist."""
+    C = P.gaussian_coeffs(1.0)
+    assert abs(C[0] + 6 * C[1] + 12 * C[2] + 8 * C[3] - 1.0) < 1e-15
+    pr = make(4, 5, 4, periodic=(1, 1, 1), use_gaussian=1, gauss_coeffs=C)
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    E0 = np.tile(np.array([1.0, -2.0, 3.0]), g.nc)
+    out = np.zeros_like(E0)
+    oracle_mod.lib().orc_node_avg(o.raw(), ptr(E0), ptr(out))
+    np.testing.assert_allclose(out, E0, rtol=1e-14)
+    pr = make(6, 6, 7, use_gaussian=1, gauss_coeffs=C)
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    i, j, k = ijk(g)
+    rng = np.random.default_rng(60)
+    E = np.zeros(3 * g.nc)
+    for l in range(3):
+        f, _ = linear(rng)
+        E[l::3] = f(i + 1, j + 1, k + 1)
+    out = np.full_like(E, 5.0)
+    oracle_mod.lib().orc_node_avg(o.raw(), ptr(E), ptr(out))
+    inside = g.all_neighbours()
+    np.testing.assert_allclose(out.reshape(-1, 3)[inside], E.reshape(-1, 3)[inside], rtol=1e-12, atol=1e-12)
+
+
+# ---------------------------------------------------------------- Faraday
+
+def div_faces(g, F):
+    """discrete divergence per cell (faces on the cell's upper sides)."""
+    sh = (g.nz, g.ny, g.nx)
+    out = np.zeros(sh)
+    for l in range(3):
+        a = F[l::3].reshape(sh)
+        out += a - np.roll(a, 1, axis=2 - l)
+    return out.reshape(-1)
+
+
+def test_face_curl_linear_E_and_div_free(oracle_mod):
+    dt = 0.25
+    # (a) linear E: dB = -dt curl E exactly, on the faces whose four nodes exist
+    pr = make(6, 5, 6)
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    rng = np.random.default_rng(70)
+    i, j, k = ijk(g)
+    M = rng.standard_normal((3, 3)) * 1e-3
+    E = np.zeros(3 * g.nc)
+    for l in range(3):
+        E[l::3] = M[l, 0] * (i + 1) + M[l, 1] * (j + 1) + M[l, 2] * (k + 1) + rng.standard_normal()
+    curl = np.array([M[2, 1] - M[1, 2], M[0, 2] - M[2, 0], M[1, 0] - M[0, 1]]) / g.dx
+    F0 = rng.standard_normal(3 * g.nc) * 1e-9
+    F = F0.copy()
+    oracle_mod.lib().orc_face_curl(o.raw(), ptr(E), ptr(F), dt)
+    dF = (F - F0).reshape(-1, 3)
+    up = upper_ok(g)
+    own = {0: (exists(g, 0, -1, 0) & exists(g, 0, 0, -1)),
+           1: (exists(g, -1, 0, 0) & exists(g, 0, 0, -1)),
+           2: (exists(g, -1, 0, 0) & exists(g, 0, -1, 0))}
+    for l in range(3):
+        upd = up & own[l]
+        np.testing.assert_allclose(dF[upd, l], -dt * curl[l], rtol=1e-9, atol=1e-9 * abs(dt * curl).max())
+        assert np.all(dF[~upd, l] == 0.0)
+    # (b) any node E in a periodic box: the face curl has zero divergence
+    pr = make(5, 6, 7, periodic=(1, 1, 1))
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    E = rng.standard_normal(3 * g.nc)
+    F = np.zeros(3 * g.nc)
+    oracle_mod.lib().orc_face_curl(o.raw(), ptr(E), ptr(F), dt)
+    assert np.abs(F).max() > 1e-8
+    assert np.abs(div_faces(g, F)).max() < 1e-12 * np.abs(F).max() * 10
+
+
+def test_cell_Ep_linear_rho(oracle_mod):
+    coeff = 8.6
+    for iso in (1, 0):
+        pr = make(5, 6, 5, use_e_pressure=1, isothermal=iso, e_pressure_coeff=coeff)
+        g = pr.grid
+        o = oracle_mod.Oracle(pr)
+        rng = np.random.default_rng(80 + iso)
+        i, j, k = ijk(g)
+        f, grad = linear(rng)
+        o.field("nodeRhoQi")[:] = 10.0 + f(i + 1, j + 1, k + 1)
+        crho = rng.uniform(1.0, 2.0, g.nc)
+        o.field("cellRhoQi")[:] = crho
+        inner = np.zeros(g.nc, dtype=np.uint8)
+        inner[3::11] = 1
+        o.flag("innerFlagCellEp")[:] = inner
+        o.field("cellEp")[:] = 4.0
+        oracle_mod.lib().orc_calc_cell_Ep(o.raw())
+        Ep = o.field("cellEp").reshape(-1, 3)
+        inside = g.all_neighbours()
+        want = -coeff * grad[None, :] / g.dx / (crho[:, None] if iso else 1.0)
+        want = np.where(inner[:, None].astype(bool), 0.0, want)
+        np.testing.assert_allclose(Ep[inside], want[inside], rtol=1e-9, atol=1e-20)
+        assert np.all(Ep[~inside] == 4.0)
+
+
+# ---------------------------------------------------------------- the whole update
+
+def test_propagate_B_uniform_plasma(oracle_mod):
+    """Uniform n, U and B in a periodic box: J = 0, Ue = U, E = -U x B,
+    curl E = 0 so B is unchanged, and no pressure gradient."""
+    pr = make(4, 5, 6, periodic=(1, 1, 1), use_e_pressure=1, e_pressure_coeff=8.6,
+              n_linear_smoothings=1, use_gaussian=1, gauss_coeffs=P.gaussian_coeffs(1.0))
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    B = np.array([-1.5e-9, 2.5e-9, 0.7e-9])
+    U = np.array([-450e3, 20e3, -5e3])
+    rho = 3e6 * P.QE
+    o.field("faceB")[:] = np.tile(B, g.nc)
+    o.field("cellRhoQi")[:] = rho
+    o.field("cellJi")[:] = np.tile(rho * U, g.nc)
+    o.propagate_B()
+    np.testing.assert_allclose(o.field("faceB"), np.tile(B, g.nc), rtol=1e-13)
+    J_scale = np.abs(B).max() / (MU0 * g.dx)   # one cell's worth of |B| gradient
+    np.testing.assert_allclose(o.field("nodeJ"), 0.0, atol=1e-14 * J_scale)
+    np.testing.assert_allclose(o.field("cellUe").reshape(-1, 3), np.broadcast_to(U, (g.nc, 3)), rtol=1e-13)
+    np.testing.assert_allclose(o.field("nodeE").reshape(-1, 3), np.broadcast_to(-np.cross(U, B), (g.nc, 3)),
+                               rtol=1e-12)
+    np.testing.assert_allclose(o.field("cellEp"), 0.0, atol=1e-20)
+
+
+def test_propagate_B_keeps_div_B(oracle_mod):
+    pr = P.perturb(make(6, 5, 7, periodic=(1, 1, 1), use_e_pressure=1, e_pressure_coeff=8.6), seed=3)
+    g = pr.grid
+    rng = np.random.default_rng(90)
+    pr.fields["cellRhoQi"][:] = rng.uniform(0.5, 1.5, g.nc) * 3e6 * P.QE
+    pr.fields["cellJi"][:] = rng.standard_normal(3 * g.nc) * 3e6 * P.QE * 1e5
+    pr.fields["nodeEta"][:] = 1e3
+    o = oracle_mod.Oracle(pr)
+    d0 = div_faces(g, o.field("faceB").copy())
+    o.propagate_B()
+    F = o.field("faceB").copy()
+    assert np.abs(F - pr.fields["faceB"]).max() > 1e-12
+    np.testing.assert_allclose(div_faces(g, F), d0, atol=1e-12 * np.abs(F).max())
