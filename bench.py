@@ -128,10 +128,23 @@ def gpu_bench(args):
     return res
 
 
+def _rel_errors(x_gpu: np.ndarray, x_orc: np.ndarray, dim: int) -> dict:
+    a = x_gpu.reshape(-1, dim)
+    b = x_orc.reshape(-1, dim)
+    nb = np.linalg.norm(b, axis=1)
+    big = nb > 1e-3 * nb.max()
+    d = np.linalg.norm(a - b, axis=1)
+    return {"rel_l2": float(np.linalg.norm(d) / np.linalg.norm(nb)),
+            "max_rel": float((d[big] / nb[big]).max())}
+
+
 def cpu_baseline(threads: int):
-    """FP64 oracle (the reference algorithm in the reference's data layout),
-    on a 64x48x48-cell box of the same workload (1/8 of the particles), timed
-    over 2 steps after 1 warm-up step."""
+    """The FP64 oracle (the reference algorithm in the reference's data
+    layout): timed on a 64x48x48-cell box of the same workload (1/8 of the
+    particles) over 2 steps after 1 warm-up step, on `threads` cores and on
+    one core over 1 step; and, as the checker, the matched-error table of the
+    FP32 GPU path against it on the 32^3 periodic box (64 ppc, 2.1 M
+    particles) after 10 full steps from the same upload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.build()
@@ -143,9 +156,33 @@ def cpu_baseline(threads: int):
     steps = 2
     o.step(steps)
     dt = time.perf_counter() - t0
-    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": "FP64 oracle, 64x48x48-cell box of the same workload (%d particles), %d steps, "
-                      "full step incl. field update" % (n, steps)}
+    del o
+    o1 = O.Oracle(pr, threads=1)
+    n1 = sum(o1.num_particles(s) for s in range(o1.nspecies))
+    t0 = time.perf_counter()
+    o1.step(1)
+    dt1 = time.perf_counter() - t0
+    del o1, pr
+    cpu = {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+           "sample": "FP64 oracle, 64x48x48-cell box of the same workload (%d particles), %d steps, "
+                     "full step incl. field update" % (n, steps),
+           "value_1core": n1 / dt1}
+    # matched error of the FP32 GPU path
+    from rhybrid_amd import problem as P
+    from rhybrid_amd.gpu import RH_FP32, Context
+    err_steps = 10
+    pr = P.config_periodic(n=32, ppc=64)
+    ctx = Context(pr, precision=RH_FP32)
+    ctx.step(err_steps)
+    g = {nm: ctx.field(nm) for nm in ("cellRhoQi", "cellJi", "nodeE", "faceB")}
+    ctx.close()
+    o = O.Oracle(pr, threads=threads)
+    o.step(err_steps)
+    err = {"workload": "32^3 periodic box, 64 ppc H+, %d full steps, FP32 GPU vs FP64 oracle; rel_l2 = "
+                       "|X_gpu - X_oracle|_2 / |X_oracle|_2, max_rel over cells with |X| > 1e-3 max|X|" % err_steps}
+    for key, nm, dim in (("n", "cellRhoQi", 1), ("J", "cellJi", 3), ("E", "nodeE", 3), ("B", "faceB", 3)):
+        err[key] = _rel_errors(g[nm], o.field(nm).copy(), dim)
+    return cpu, err
 
 
 def main():
@@ -160,10 +197,10 @@ def main():
               "the halo and migration exchange between GPUs is not built).", file=sys.stderr)
         sys.exit(2)
     res = gpu_bench(args)
-    cpu = None
+    cpu, err = None, None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(threads)
+        cpu, err = cpu_baseline(threads)
     line = {
         "metric": "particle-steps/s (push+deposit) & step ms at matched n/J error vs FP64 oracle",
         "value": res["value"],
@@ -185,6 +222,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": res["roofline"],
         "cpu_baseline": cpu,
+        "error_vs_oracle": err,
         "push_deposit": res["push_deposit"],
         "kernels": res["kernels"],
     }

@@ -1,0 +1,75 @@
+/* rh_denoise.h -- C ABI of the learned moment denoiser on MI355X (gfx950).
+ *
+ * The denoiser has no counterpart in the reference: it is the optional
+ * post-deposit correction of the noisy ion moments (n, J) of a low-ppc run
+ * towards a high-ppc run, applied before the field update.  Network: four
+ * 3x3x3 convolutions, 7 -> 32 -> 32 -> 32 -> 4 channels, no biases (64,800
+ * weights), ReLU after the first three, residual on (n, Jx, Jy, Jz):
+ *
+ *   x = (n, Jx, Jy, Jz, Bx, By, Bz) / scale          (7 channels, per voxel)
+ *   y = x[0:4] + conv4(relu(conv3(relu(conv2(relu(conv1(x)))))))
+ *   output = y * scale[0:4]
+ *
+ * Convolutions are zero-padded ("same") over each volume of X*Y*Z voxels.
+ * Activations and weights are BF16 on the device, products accumulate in
+ * FP32 on the matrix cores; the residual and the loss are FP32.
+ *
+ * Layouts (all device pointers are hipMalloc'd memory of the caller):
+ *   volumes   float [P][Z][Y][X][C], C = 7 for inputs, 4 for outputs/targets
+ *   weights   float, the four layers concatenated, each [cout][cin][kz][ky][kx]
+ *             (64,800 values: 6,048 + 27,648 + 27,648 + 3,456)
+ * X must be a multiple of 16, Y and Z multiples of 4.
+ *
+ * Every call returns RH_DN_OK (0) or a negative code; rh_dn_last_error()
+ * gives the message.  All work of one denoiser is ordered on one HIP stream.
+ */
+#ifndef RH_DENOISE_H
+#define RH_DENOISE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { RH_DN_OK = 0, RH_DN_ERR_ARG = -1, RH_DN_ERR_HIP = -2, RH_DN_ERR_CAPACITY = -4 };
+enum { RH_DN_N_WEIGHTS = 64800, RH_DN_IN_CH = 7, RH_DN_OUT_CH = 4 };
+
+typedef struct rh_denoiser rh_denoiser;
+
+/* max_voxels bounds P*X*Y*Z of any later call (activation buffers). */
+int rh_dn_create(int64_t max_voxels, rh_denoiser** out);
+void rh_dn_destroy(rh_denoiser* d);
+const char* rh_dn_last_error(const rh_denoiser* d);
+
+/* host FP32 weights in the layout above; resets the Adam state on set */
+int rh_dn_set_weights(rh_denoiser* d, const float* w, int64_t n);
+int rh_dn_get_weights(rh_denoiser* d, float* w, int64_t n);
+/* per-channel normalisation of the 7 input channels (n, J, B) */
+int rh_dn_set_scales(rh_denoiser* d, const float scale[7]);
+
+/* y = denoise(x) on device volumes */
+int rh_dn_forward(rh_denoiser* d, const float* x, float* y, int P, int X, int Y, int Z);
+
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the mean squared error
+ * between the normalised output and target/scale over the 4 output channels;
+ * loss (before the step) is written to *loss_host. */
+int rh_dn_train_step(rh_denoiser* d, const float* x, const float* target, int P, int X, int Y, int Z,
+                     float lr, float* loss_host);
+
+/* gradient of that loss w.r.t. the 64,800 weights (host FP32), without a step */
+int rh_dn_gradient(rh_denoiser* d, const float* x, const float* target, int P, int X, int Y, int Z,
+                   float* grad_host, float* loss_host);
+
+int rh_dn_synchronize(rh_denoiser* d);
+
+/* per-kernel device time of the denoiser's launches since the last reset */
+int rh_dn_set_timing(rh_denoiser* d, int on);
+int rh_dn_kernel_time(rh_denoiser* d, const char* name, double* ms, int64_t* launches);
+int rh_dn_reset_timing(rh_denoiser* d);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RH_DENOISE_H */

@@ -7,7 +7,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "rh_capi.hip")
+SRCS = [os.path.join(HERE, "csrc", "rh_capi.hip"), os.path.join(HERE, "csrc", "rh_denoise.hip")]
 OUT = os.path.join(HERE, "librhybrid_amd.so")
 
 HIPCC_FLAGS = [
@@ -20,11 +20,11 @@ HIPCC_FLAGS = [
 
 def build(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc] + HIPCC_FLAGS + ["-o", OUT, SRC]
+    cmd = [hipcc] + HIPCC_FLAGS + ["-o", OUT] + SRCS
     r = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed for %s" % SRC)
+        raise RuntimeError("hipcc failed for %s" % " ".join(SRCS))
     if verbose:
         sys.stderr.write(r.stderr)
     return OUT

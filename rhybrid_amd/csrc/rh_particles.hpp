@@ -390,11 +390,14 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const uint32_t* __r
 // src/particle_accumulator.cpp:43-156).  One workgroup per tile, one thread
 // per subcell: a subcell's particles share one 2x2x2 footprint, so the thread
 // sums, in particle order, the 8 corner weights times (1, vx, vy, vz).  The
-// subcell sums are staged in LDS (two quantities at a time) and every cell of
-// the tile's extended region adds the (up to 64) subcell terms that reach it
-// in a fixed order.  The result is the tile's ghost partial of this species
-// (unscaled); k_fold sums partials over tiles and species in FP64.  No
-// atomics, so every run gives the same bits.
+// threads take the subcells in ascending order of their particle counts (a
+// counting sort in LDS), so that the lanes of a wave loop about equally long;
+// which thread sums a subcell does not change the sum.  The subcell sums are
+// staged in LDS (two quantities at a time) and every cell of the tile's
+// extended region adds the (up to 64) subcell terms that reach it in a fixed
+// order.  The result is the tile's ghost partial of this species (unscaled);
+// k_fold sums partials over tiles and species in FP64.  No float atomics, so
+// every run gives the same bits.
 template <typename Real>
 __device__ __forceinline__ void cic_accumulate(Real (&acc)[4][8], bool hx, bool hy, bool hz, Real X, Real Y, Real Z,
                                                Real vx, Real vy, Real vz) {
@@ -416,10 +419,14 @@ __device__ __forceinline__ void cic_accumulate(Real (&acc)[4][8], bool hx, bool 
 
 // LDS of the deposit: half of the subcell sums (2 quantities x 8 corners) as
 // [quantity*8+corner][octant][cell], each octant block padded by one word so
-// that neighbouring cells fall in different banks; the first half's results.
+// that neighbouring cells fall in different banks; the first half's results;
+// the count histogram and, per subcell, first particle, count and the
+// count-ordered subcell list.
+constexpr int DEP_BINS = 64;
 template <typename Real>
 RH_HD size_t deposit_lds_bytes(const Geom& g) {
-    return sizeof(Real) * (16 * 8 * (size_t)(g.tc + 1) + 2 * (size_t)g.next);
+    return sizeof(Real) * (16 * 8 * (size_t)(g.tc + 1) + 2 * (size_t)g.next) +
+           sizeof(uint32_t) * (DEP_BINS + 3 * 8 * (size_t)g.tc);
 }
 
 // For every cell of the extended region, the LDS offsets (corner plane,
@@ -474,10 +481,39 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
     const int QS = 8 * OB;            // one (quantity, corner) plane
     Real* S = reinterpret_cast<Real*>(smem_raw);
     Real* R = S + 16 * QS;  // first-half results, next x 2
+    uint32_t* hist = reinterpret_cast<uint32_t*>(R + 2 * g.next);
+    uint32_t* first = hist + DEP_BINS;  // first particle of each subcell
+    uint32_t* num = first + NS;         // particle count of each subcell
+    uint32_t* order = num + NS;         // subcells in ascending count order
     const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
-    const int ls = threadIdx.x;
-    const uint32_t key = (uint32_t)(T * NS + ls);
-    const uint32_t p0 = sub[key], p1 = sub[key + 1];
+    const int tid = threadIdx.x;
+    // counting sort of the subcells by particle count (bins clamp at NB-1)
+    const int NB = NS < DEP_BINS ? NS : DEP_BINS;
+    if (tid < NB) hist[tid] = 0u;
+    {
+        const uint32_t key = (uint32_t)(T * NS + tid);
+        const uint32_t a = sub[key], n = sub[key + 1] - a;
+        first[tid] = a;
+        num[tid] = n;
+        __syncthreads();
+        const int bin = n < (uint32_t)NB ? (int)n : NB - 1;
+        const uint32_t r = atomicAdd(&hist[bin], 1u);
+        __syncthreads();
+        if (tid < NB) {  // exclusive scan of the bins inside the first wave
+            const uint32_t h = hist[tid];
+            uint32_t v = h;
+            for (int off = 1; off < NB; off <<= 1) {
+                const uint32_t u = __shfl_up(v, off);
+                if (tid >= off) v += u;
+            }
+            hist[tid] = v - h;
+        }
+        __syncthreads();
+        order[hist[bin] + r] = (uint32_t)tid;
+        __syncthreads();
+    }
+    const int ls = (int)order[tid];
+    const uint32_t p0 = first[ls], p1 = p0 + num[ls];
     const int o = ls & 7, cell = ls >> 3;
     const bool hx = o & 1, hy = (o >> 1) & 1, hz = (o >> 2) & 1;
     Real acc[4][8];
@@ -495,7 +531,7 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
 #pragma unroll
             for (int c = 0; c < 8; ++c) S[(qq * 8 + c) * QS + o * OB + cell] = acc[2 * half + qq][c];
         __syncthreads();
-        for (int e = ls; e < g.next; e += blockDim.x) {
+        for (int e = tid; e < g.next; e += blockDim.x) {
             // the subcell terms reaching this cell: a list shared by all tiles
             const int cnt_e = gcount[e];
             const uint16_t* lst = glist + (size_t)e * 64;
