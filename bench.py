@@ -26,7 +26,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+HBM_PEAK_GBS = 8000.0      # MI355X published HBM3E bandwidth
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
 
 
 def workload(scale: int = 1, seed: int = 6461674):
@@ -128,6 +129,63 @@ def gpu_bench(args):
     return res
 
 
+def denoiser_bench(patches: int = 8, reps: int = 10):
+    """The learned moment denoiser on `patches` volumes of 32^3 voxels x 7
+    channels (the training patch shape): forward passes and training steps
+    (forward, MSE loss, backward-data, backward-weights, Adam), timed with
+    HIP events; FLOPs count the 3x3x3 convolutions only."""
+    import torch
+    from rhybrid_amd import denoise as D
+    P, S = patches, 32
+    V = P * S ** 3
+    g = torch.Generator(device="cuda").manual_seed(11)
+    x = torch.randn((P, S, S, S, 7), device="cuda", generator=g)
+    t = x[..., :4] + 0.1 * torch.randn((P, S, S, S, 4), device="cuda", generator=g)
+    d = D.Denoiser(V, scales=[1.0] * 7, weights=D.init_weights(seed=0))
+    units_fwd = sum(ci * co for ci, co in D.LAYERS)                     # 2400
+    units_train = units_fwd + sum(ci * co for ci, co in D.LAYERS[1:]) + units_fwd  # fwd + dgrad + wgrad
+    flop_unit = 2.0 * V * 27
+    for _ in range(2):
+        d.forward(x)
+        d.train_step(x, t, lr=1e-4)
+    d.synchronize()
+    d.reset_timing()
+    d.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        d.forward(x)
+    d.synchronize()
+    t_fwd = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        d.train_step(x, t, lr=1e-4)
+    d.synchronize()
+    t_train = (time.perf_counter() - t0) / reps
+    d.set_timing(False)
+    kern = {}
+    for nm in ("pack_input", "conv1", "conv2", "conv3", "conv4", "loss", "dgrad4", "dgrad3", "dgrad2",
+               "wgrad4", "wgrad3", "wgrad2", "wgrad1", "wgrad_reduce", "adam", "pack"):
+        ms, n = d.kernel_time(nm)
+        if n:
+            kern[nm] = {"us_avg": round(ms * 1e3 / n, 2), "launches": n}
+    d.close()
+    conv_units = {"conv1": 7 * 32, "conv2": 1024, "conv3": 1024, "conv4": 128, "dgrad4": 128, "dgrad3": 1024,
+                  "dgrad2": 1024, "wgrad4": 128, "wgrad3": 1024, "wgrad2": 1024, "wgrad1": 7 * 32}
+    for nm, u in conv_units.items():
+        if nm in kern:
+            kern[nm]["TFLOPs"] = round(u * flop_unit / (kern[nm]["us_avg"] * 1e-6) / 1e12, 1)
+    dom = max(conv_units, key=lambda nm: kern.get(nm, {}).get("us_avg", 0) * kern.get(nm, {}).get("launches", 0))
+    ach = kern[dom]["TFLOPs"]
+    return {"workload": "%d volumes of 32^3 voxels x 7 channels, conv3d 7-32-32-32-4 (3^3), BF16 MFMA, "
+                        "FP32 accumulation" % P,
+            "forward_tflops": units_fwd * flop_unit / t_fwd / 1e12,
+            "train_step_tflops": units_train * flop_unit / t_train / 1e12,
+            "ms_forward": t_fwd * 1e3, "ms_train_step": t_train * 1e3,
+            "roofline": {"bound": "mfma", "kernel": dom, "achieved": ach, "peak": BF16_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(ach / BF16_PEAK_TFLOPS, 4)},
+            "kernels": kern}
+
+
 def _rel_errors(x_gpu: np.ndarray, x_orc: np.ndarray, dim: int) -> dict:
     a = x_gpu.reshape(-1, dim)
     b = x_orc.reshape(-1, dim)
@@ -191,12 +249,14 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-denoiser", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one domain per GPU; "
               "the halo and migration exchange between GPUs is not built).", file=sys.stderr)
         sys.exit(2)
     res = gpu_bench(args)
+    dn = None if args.no_denoiser else denoiser_bench()
     cpu, err = None, None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
@@ -223,6 +283,7 @@ def main():
         "roofline": res["roofline"],
         "cpu_baseline": cpu,
         "error_vs_oracle": err,
+        "denoiser": dn,
         "push_deposit": res["push_deposit"],
         "kernels": res["kernels"],
     }

@@ -99,7 +99,7 @@ enum rh_flag_id {
 /* Constraint counters, as Hybrid::logCounter* (src/hybrid.h:306-330). */
 enum rh_counter_id {
     RH_CNT_MAX_VI = 0, RH_CNT_MIN_CELL_RHO, RH_CNT_MAX_CELL_UE, RH_CNT_MIN_NODE_RHO,
-    RH_CNT_MAX_NODE_UE, RH_CNT_MAX_E, RH_CNT_MAX_VW, RH_CNT_DROPPED, RH_N_COUNTERS
+    RH_CNT_MAX_NODE_UE, RH_CNT_MAX_E, RH_CNT_MAX_VW, RH_CNT_DROPPED, RH_CNT_DENOISE_FALLBACK, RH_N_COUNTERS
 };
 
 /* ---- context -------------------------------------------------------- */
@@ -153,6 +153,20 @@ int rh_apply_boundary_conditions(rh_ctx* ctx, int32_t species);
 int rh_propagate_B(rh_ctx* ctx);
 /* propagate() (src/user.cpp:118-134) without logging or injection, nsteps times. */
 int rh_step(rh_ctx* ctx, int32_t nsteps);
+
+/* ---- learned moment denoiser (include/rh_denoise.h) -------------------- */
+
+/* No counterpart in the reference: replaces the deposited ion moments
+ * (cellRhoQi, cellJi) by the denoiser's output, with cellB as the magnetic
+ * input channels, for callers that run at low ppc.  Call between
+ * rh_add_remote_updates and rh_propagate_B.  Fallback bound: a cell whose
+ * denoised charge density differs from the deposited one by more than
+ * max_rel_change * |rho| keeps its deposited moments and is counted in
+ * RH_CNT_DENOISE_FALLBACK; the rho floor is applied again afterwards.
+ * Needs nx % 16 == 0, ny % 4 == 0, nz % 4 == 0 and a denoiser created for at
+ * least nx*ny*nz voxels. */
+typedef struct rh_denoiser rh_denoiser;
+int rh_denoise_moments(rh_ctx* ctx, rh_denoiser* dn, double max_rel_change);
 
 /* ---- diagnostics ------------------------------------------------------ */
 

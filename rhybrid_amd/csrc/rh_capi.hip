@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/rh_gpu.h"
+#include "../../include/rh_denoise.h"
 #include "rh_fields.hpp"
 #include "rh_geom.hpp"
 #include "rh_particles.hpp"
@@ -95,12 +96,43 @@ public:
     virtual void step(int n) = 0;
     virtual void diagnostics(int s, double* out) = 0;
     virtual void counters(double* out) = 0;
+    virtual void denoise(rh_denoiser* dn, double max_rel) = 0;
     virtual void synchronize() = 0;
     virtual void set_timing(bool on) = 0;
     virtual void kernel_time(const char* name, double* ms, int64_t* launches) = 0;
     virtual void reset_timing() = 0;
     virtual void layout(int64_t* out) = 0;
 };
+
+// Moments <-> denoiser volume [Z][Y][X][7] (n, Ji, cellB) and back; the
+// cell index n = i + nx*(j + ny*k) is the voxel index of the volume.
+template <typename Real>
+__global__ void k_dn_gather(int nc, const Real* __restrict__ rho, V3<Real> ji, V3<Real> b, float* __restrict__ out) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nc) return;
+    float* o = out + (size_t)n * 7;
+    o[0] = (float)rho[n];
+    o[1] = (float)ji.x[n]; o[2] = (float)ji.y[n]; o[3] = (float)ji.z[n];
+    o[4] = (float)b.x[n]; o[5] = (float)b.y[n]; o[6] = (float)b.z[n];
+}
+
+template <typename Real>
+__global__ void k_dn_scatter(int nc, const float* __restrict__ y, double max_rel, double min_rho, Real* __restrict__ rho,
+                             V3<Real> ji, unsigned long long* __restrict__ counters) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    bool fallback = false;
+    if (n < nc) {
+        const float* q = y + (size_t)n * 4;
+        const double r0 = (double)rho[n], r1 = (double)q[0];
+        if (!(fabs(r1 - r0) <= max_rel * fabs(r0))) {
+            fallback = true;  // keep the deposited moments (also for a non-finite output)
+        } else {
+            rho[n] = (Real)(r1 < min_rho ? min_rho : r1);
+            ji.x[n] = (Real)q[1]; ji.y[n] = (Real)q[2]; ji.z[n] = (Real)q[3];
+        }
+    }
+    count_event(counters, C_DN_FALLBACK, fallback);
+}
 
 template <typename Real>
 class Sim final : public Ctx {
@@ -568,6 +600,27 @@ public:
                fld[RH_CELL_RHOQ].p, ji.x, ji.y, ji.z, dcount.p);
     }
 
+    DevBuf<float> dn_in, dn_out;  // denoiser volumes (allocated on first use)
+
+    void denoise(rh_denoiser* dn, double max_rel) override {
+        if (!dn) throw ArgError("null denoiser");
+        if (!(max_rel >= 0.0)) throw ArgError("max_rel_change must be >= 0");
+        if (g.nx % 16 || g.ny % 4 || g.nz % 4) throw ArgError("denoiser needs nx % 16 == 0, ny % 4 == 0, nz % 4 == 0");
+        if (!dn_in.p) {
+            dn_in.alloc((size_t)g.nc * 7);
+            dn_out.alloc((size_t)g.nc * 4);
+        }
+        const V3<Real> ji = v3(RH_CELL_JI), b = v3(RH_CELL_B);
+        launch("dn_gather", k_dn_gather<Real>, cdiv(g.nc, 256), 256, 0, g.nc, (const Real*)fld[RH_CELL_RHOQ].p, ji, b,
+               dn_in.p);
+        RH_HIP(hipStreamSynchronize(stream));  // the denoiser runs on its own stream
+        if (rh_dn_forward(dn, dn_in.p, dn_out.p, 1, g.nx, g.ny, g.nz) != RH_DN_OK ||
+            rh_dn_synchronize(dn) != RH_DN_OK)
+            throw ArgError(std::string("denoiser: ") + rh_dn_last_error(dn));
+        launch("dn_scatter", k_dn_scatter<Real>, cdiv(g.nc, 256), 256, 0, g.nc, (const float*)dn_out.p, max_rel,
+               P.min_rho, fld[RH_CELL_RHOQ].p, ji, dcount.p);
+    }
+
     void apply_bcs(int s) override {
         Species& S_ = S(s);
         if (S_.pushed) throw StateError("boundary conditions between propagate and wait_particle_sends");
@@ -835,6 +888,9 @@ int rh_kernel_times(rh_ctx* ctx, const char* kernel, double* ms, int64_t* launch
 }
 int rh_reset_timing(rh_ctx* ctx) {
     return guarded(ctx, [&](rh::Ctx& c) { c.reset_timing(); });
+}
+int rh_denoise_moments(rh_ctx* ctx, rh_denoiser* dn, double max_rel_change) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.denoise(dn, max_rel_change); });
 }
 int rh_layout_info(rh_ctx* ctx, int64_t out[6]) {
     if (!out) return RH_ERR_ARG;

@@ -41,7 +41,7 @@ struct PushParams {
 };
 
 enum DevCounter { C_MAX_VI = 0, C_MIN_CELL_RHO, C_MAX_CELL_UE, C_MIN_NODE_RHO, C_MAX_NODE_UE,
-                  C_MAX_E, C_MAX_VW, C_DROPPED, C_RANK_OVERFLOW, C_N };
+                  C_MAX_E, C_MAX_VW, C_DROPPED, C_DN_FALLBACK, C_RANK_OVERFLOW, C_N };
 
 // Counters are spread over COUNTER_SLOTS copies (summed when read) so that
 // waves of different blocks do not all hit one address.

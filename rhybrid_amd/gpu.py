@@ -26,7 +26,7 @@ LIB_PATH = os.path.join(_HERE, "librhybrid_amd.so")
 RH_OK, RH_ERR_ARG, RH_ERR_HIP, RH_ERR_STATE, RH_ERR_CAPACITY, RH_ERR_NO_DEVICE = 0, -1, -2, -3, -4, -5
 RH_FP32, RH_FP64 = 0, 1
 COUNTER_NAMES = ["max_vi", "min_cell_rho", "max_cell_ue", "min_node_rho", "max_node_ue",
-                 "max_E", "max_vw", "dropped"]
+                 "max_E", "max_vw", "dropped", "denoise_fallback"]
 
 #: every symbol include/rh_gpu.h declares
 EXPORTS = ["rh_create", "rh_destroy", "rh_last_error", "rh_add_species", "rh_upload_field",
@@ -35,7 +35,7 @@ EXPORTS = ["rh_create", "rh_destroy", "rh_last_error", "rh_add_species", "rh_upl
            "rh_clear_accumulation_arrays", "rh_accumulate", "rh_add_remote_updates",
            "rh_apply_boundary_conditions", "rh_propagate_B", "rh_step", "rh_diagnostics",
            "rh_counters", "rh_synchronize", "rh_set_timing", "rh_kernel_times", "rh_reset_timing",
-           "rh_layout_info"]
+           "rh_layout_info", "rh_denoise_moments"]
 
 
 class GridDesc(C.Structure):
@@ -79,6 +79,15 @@ def load_library(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise RuntimeError("HIP library %s is missing: build it with __graft_entry__.build()" % path)
+    # One HIP runtime per process: torch's build carries its own
+    # libamdhip64 (loaded by the unversioned name), the library needs
+    # libamdhip64.so.7.  With torch loaded first, the library binds to the
+    # runtime already in the process; loaded the other way round, the process
+    # would hold two runtimes and torch would see no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(path)
     vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
     L.rh_create.argtypes = [C.POINTER(GridDesc), C.POINTER(Params), i32, C.POINTER(vp)]
@@ -100,6 +109,7 @@ def load_library(path: str = LIB_PATH):
         getattr(L, name).argtypes = [vp, i32]
     L.rh_diagnostics.argtypes = [vp, i32, vp]
     L.rh_counters.argtypes = [vp, vp]
+    L.rh_denoise_moments.argtypes = [vp, vp, C.c_double]
     L.rh_kernel_times.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(i64)]
     L.rh_layout_info.argtypes = [vp, vp]
     for name in EXPORTS:
@@ -220,6 +230,11 @@ class Context:
         out = np.zeros(5)
         self._check(load_library().rh_diagnostics(self._h, s, _ptr(out)))
         return out
+
+    def denoise_moments(self, denoiser, max_rel_change: float) -> None:
+        """Learned denoiser on the deposited moments (no counterpart in the
+        reference); call between add_remote_updates and propagate_B."""
+        self._check(load_library().rh_denoise_moments(self._h, denoiser._h, float(max_rel_change)))
 
     def counters(self) -> Dict[str, float]:
         out = np.zeros(len(COUNTER_NAMES))
