@@ -45,10 +45,15 @@ def kernel_bytes(name: str, ctx, np_total: float, nspecies: int) -> float:
     extk = next_ * 8
     nkeys = ntiles * lay["tx"] * lay["ty"] * lay["tz"] * 8
     npp = np_total / nspecies  # particles per species launch (mean)
-    if name == "push":       # pass 0: read 6 f32 + local cell byte; 9 f32 fields per cell; write slot counts
-        return 25.0 * npp + 36.0 * nc + 4.0 * extk * ntiles + 4.0 * ntiles
-    if name == "move":       # pass 1: read and write 6 f32 + local cell byte; fields; slot bases
-        return 50.0 * npp + 36.0 * nc + 4.0 * extk * ntiles + 4.0 * ntiles
+    nh = (lay["tx"] + 1) * (lay["ty"] + 1) * (lay["tz"] + 1)
+    tc = lay["tx"] * lay["ty"] * lay["tz"]
+    fblock = 4.0 * (3 * nh + 7 * tc) * ntiles   # per-tile field blocks (FP32 + neighbourhood masks)
+    if name == "push_fields":  # read 9 f32 fields per cell, write the per-tile blocks
+        return 36.0 * nc + fblock
+    if name == "push":       # pass 0: read 6 f32 + local cell byte and the field blocks; write slot counts
+        return 25.0 * npp + fblock + 4.0 * extk * ntiles + 4.0 * ntiles
+    if name == "move":       # pass 1: read and write 6 f32 + local cell byte; field blocks; slot bases
+        return 50.0 * npp + fblock + 4.0 * extk * ntiles + 4.0 * ntiles
     if name == "deposit":    # read 6 f32 per particle, subcell starts; read+write FP64 tile ghosts
         return 24.0 * npp + 4.0 * nkeys + 64.0 * next_ * ntiles
     if name == "fold":       # read FP64 tile ghosts once, write rho + J (f32)
@@ -56,6 +61,28 @@ def kernel_bytes(name: str, ctx, np_total: float, nspecies: int) -> float:
     if name in ("sort_count", "sort_base"):
         return 4.0 * extk * ntiles * (2 if name == "sort_base" else 1) + 4.0 * nkeys
     return 0.0
+
+
+PROFILE_KERNELS = {"push": "rh::k_push_move<float>_[pass_0:_push]", "move": "rh::k_push_move<float>_[pass_1:_move]",
+                   "deposit": "rh::k_deposit<float>", "fold": "rh::k_fold<float>",
+                   "sort_count": "rh::k_slot_offsets<false>", "sort_base": "rh::k_slot_offsets<true>"}
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the factor a 1 GiB copy
+    gives); None if the summary does not list the kernel."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_obstacle128x96x96_ppc32.txt")
+    name = PROFILE_KERNELS.get(kernel)
+    if name is None or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("#"):
+            continue
+        f = line.rstrip("\n").split("\t")
+        if f[0] == name:
+            return float(f[6])
+    return None
 
 
 def gpu_bench(args):
@@ -84,7 +111,7 @@ def gpu_bench(args):
     ctx.set_timing(False)
     np_after = sum(ctx.num_particles(s) for s in range(nspec))
     np_mean = 0.5 * (np_before + np_after)
-    names = ["push", "sort_count", "scan", "sort_base", "move", "deposit", "fold", "boundary",
+    names = ["push_fields", "push", "sort_count", "scan", "sort_base", "move", "deposit", "fold", "boundary",
              "face2cell", "neumann", "imf", "cell2node", "node_J", "node2cell", "cell_Ue", "node_Ue",
              "upwind_B", "node_E", "node_avg", "face_curl", "cell_Ep", "fill", "diag"]
     ktimes = {}
@@ -93,7 +120,7 @@ def gpu_bench(args):
         if n:
             ktimes[nm] = (ms, n)
     total_k = sum(v[0] for v in ktimes.values())
-    pd_names = ("push", "sort_count", "scan", "sort_base", "move", "deposit", "fold")
+    pd_names = ("push_fields", "push", "sort_count", "scan", "sort_base", "move", "deposit", "fold")
     t_pd = sum(ktimes[k][0] for k in pd_names if k in ktimes) / 1e3
     dom = max(ktimes, key=lambda k: ktimes[k][0])
     per_kernel = {}
@@ -117,7 +144,8 @@ def gpu_bench(args):
         "np": np_mean,
         "wall_s": wall,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
+                     "algorithmic_bytes": dom_bytes},
         "push_deposit": {"particle_steps_per_s": np_mean * args.steps / t_pd if t_pd > 0 else None,
                          "ms_per_step": t_pd / args.steps * 1e3,
                          "effective_GBps_vs_56B_per_particle": round(pd_bytes * args.steps / t_pd / 1e9, 1)},

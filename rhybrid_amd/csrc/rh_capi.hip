@@ -407,6 +407,7 @@ public:
             for (int n = 0; n < g.nc; ++n) h[(size_t)l * g.nc + n] = (Real)data[(size_t)n * dim + l];
         RH_HIP(hipMemcpyAsync(fld[id].p, h.data(), h.size() * sizeof(Real), hipMemcpyHostToDevice, stream));
         RH_HIP(hipStreamSynchronize(stream));
+        pf_dirty = true;
     }
 
     void download_field(int id, double* data) override {
@@ -537,9 +538,9 @@ public:
         pp.use_e_pressure = P.use_e_pressure;
         pp.use_gravity = P.use_gravity;
         pp.initial_flow_through = P.initial_flow_through;
-        const V3<Real> fb = v3(RH_FACE_B), ue = v3(RH_CELL_UE), ep = v3(RH_CELL_EP);
-        PushFields<Real> pf{fb.x, fb.y, fb.z, ue.x, ue.y, ue.z, ep.x, ep.y, ep.z};
-        launch("push", k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, pp, pf,
+        prepare_push_fields();
+        launch("push", k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, pp,
+               (const uint32_t*)pf_tile.p,
                S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.cnt_tile.p, dcount.p, kbits, 0, (uint32_t)S_.cap);
         S_.pp = pp;
         S_.pushed = true;
@@ -564,9 +565,8 @@ public:
         launch("sort_count", k_slot_offsets<false>, tile_grid(), g.tc * 8, 0, g, S_.cnt_tile.p, S_.sub_new.p);
         scan(S_.sub_new.p, g.nkeys, S_.sub_new.p, 0);
         launch("sort_base", k_slot_offsets<true>, tile_grid(), g.tc * 8, 0, g, S_.cnt_tile.p, S_.sub_new.p);
-        const V3<Real> fb = v3(RH_FACE_B), ue = v3(RH_CELL_UE), ep = v3(RH_CELL_EP);
-        PushFields<Real> pf{fb.x, fb.y, fb.z, ue.x, ue.y, ue.z, ep.x, ep.y, ep.z};
-        launch("move", k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, S_.pp, pf,
+        launch("move", k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, S_.pp,
+               (const uint32_t*)pf_tile.p,
                S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.cnt_tile.p, dcount.p, kbits, 1, (uint32_t)S_.cap);
         S_.swap_stores();
         S_.pushed = false;
@@ -601,6 +601,21 @@ public:
     }
 
     DevBuf<float> dn_in, dn_out;  // denoiser volumes (allocated on first use)
+    DevBuf<uint32_t> pf_tile;     // push fields, one block per tile (k_push_fields)
+    bool pf_dirty = true;         // fields changed since the blocks were written
+
+    // The blocks of the push fields are rewritten when the fields have
+    // changed; both passes of every species then read the same blocks.
+    void prepare_push_fields() {
+        if (!pf_dirty) return;
+        for (auto& s_ : sp)
+            if (s_->pushed) throw StateError("fields changed between propagate and wait_particle_sends");
+        if (!pf_tile.p) pf_tile.alloc((size_t)g.ntiles * push_field_words<Real>(g));
+        const V3<Real> fb = v3(RH_FACE_B), ue = v3(RH_CELL_UE), ep = v3(RH_CELL_EP);
+        PushFields<Real> pf{fb.x, fb.y, fb.z, ue.x, ue.y, ue.z, ep.x, ep.y, ep.z};
+        launch("push_fields", k_push_fields<Real>, tile_grid(), 256, 0, g, pf, (int)P.use_e_pressure, pf_tile.p);
+        pf_dirty = false;
+    }
 
     void denoise(rh_denoiser* dn, double max_rel) override {
         if (!dn) throw ArgError("null denoiser");
@@ -640,6 +655,7 @@ public:
     void propagate_B() override {
         for (auto& sp_ : sp)
             if (sp_->pushed) throw StateError("propagate_B between propagate and wait_particle_sends");
+        pf_dirty = true;  // faceB, cellUe and cellEp change
         const dim3 cb = cell_grid(), bb = cell_block();
         const unsigned eb = cdiv(n_ext, 256);
         const int imf_cell = P.imf_cell[0] | P.imf_cell[1] << 1 | P.imf_cell[2] << 2 | P.imf_cell[3] << 3 |

@@ -99,6 +99,8 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[my][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
     if constexpr (CIN == 32) {
+        // fully unrolled, so that the weight loads of later taps issue early
+#pragma unroll
         for (int tap = 0; tap < NTAP; ++tap) {
             const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
             bf16x8 b[NT];
@@ -115,6 +117,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
     } else {
         // four taps per instruction: lane group kg takes tap 4g+kg (the
         // weights of the padding tap 27 are zero)
+#pragma unroll
         for (int g = 0; g < 7; ++g) {
             int tap = 4 * g + kg;
             tap = tap < NTAP ? tap : NTAP - 1;
@@ -269,17 +272,30 @@ __global__ __launch_bounds__(256) void k_wgrad(const bf16* __restrict__ in, cons
     }
 }
 
-// grad[offset + (co*cin + ci)*27 + tap] = sum over workgroups, in order
-__global__ void k_wgrad_reduce(const float* __restrict__ part, int nblk, int CI, int CO, int cin, int cout,
-                               float* __restrict__ grad) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= cout * cin * NTAP) return;
-    const int tap = e % NTAP, ci = (e / NTAP) % cin, co = e / (NTAP * cin);
+// grad[offset + (co*cin + ci)*27 + tap] = sum over workgroups in a fixed
+// order: a block of 256 threads takes 8 consecutive weights (co fastest, the
+// partials' memory order) x 32 groups of workgroups; group g sums workgroups
+// g, g+32, ... and the 32 group sums are added in order of g.
+constexpr int RED_W = 8, RED_G = 32;
+__global__ __launch_bounds__(RED_W * RED_G) void k_wgrad_reduce(const float* __restrict__ part, int nblk, int CI,
+                                                                int CO, int cin, int cout, float* __restrict__ grad) {
+    __shared__ double red[RED_G][RED_W];
+    const int wl = threadIdx.x % RED_W, gi = threadIdx.x / RED_W;
+    const int e = blockIdx.x * RED_W + wl;
+    const int ne = cout * cin * NTAP;
+    const int co = e % cout, ci = (e / cout) % cin, tap = e / (cout * cin);
     const size_t stride = (size_t)NTAP * CI * CO;
     const size_t off = ((size_t)tap * CI + ci) * CO + co;
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += part[b * stride + off];
-    grad[e] = (float)s;
+    if (e < ne)
+        for (int b = gi; b < nblk; b += RED_G) s += part[b * stride + off];
+    red[gi][wl] = s;
+    __syncthreads();
+    if (gi == 0 && e < ne) {
+        double t = 0.0;
+        for (int g = 0; g < RED_G; ++g) t += red[g][wl];
+        grad[((size_t)co * cin + ci) * NTAP + tap] = (float)t;
+    }
 }
 
 // packed B fragments of one layer.  KP = 32: fragment f = tap*NT + nt,
@@ -353,11 +369,19 @@ __global__ __launch_bounds__(LOSS_THREADS) void k_loss(const float* __restrict__
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-__global__ void k_sum(const double* __restrict__ part, int n, double scale, float* __restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// one block of LOSS_THREADS: strided partial sums, then a fixed-order tree
+__global__ __launch_bounds__(LOSS_THREADS) void k_sum(const double* __restrict__ part, int n, double scale,
+                                                      float* __restrict__ out) {
+    __shared__ double red[LOSS_THREADS];
     double s = 0.0;
-    for (int i = 0; i < n; ++i) s += part[i];
-    out[0] = (float)(s * scale);
+    for (int i = threadIdx.x; i < n; i += LOSS_THREADS) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = LOSS_THREADS / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = (float)(red[0] * scale);
 }
 
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m1,
@@ -430,7 +454,7 @@ struct rh_denoiser {
     Buf<float> part;
     Buf<double> lpart;
     int adam_step = 0;
-    int wgrad_blocks = 512;
+    int wgrad_blocks = 512;  // persistent workgroups (two rounds of one per CU)
     // timing
     bool timing = false;
     std::map<std::string, std::pair<double, int64_t>> times;
@@ -562,7 +586,7 @@ struct rh_denoiser {
         const int nblk = ntiles < wgrad_blocks ? ntiles : wgrad_blocks;
         launch(name, k_wgrad<CIN, COUT>, dim3(nblk), dim3(256), in, G, X, Y, Z, P, part.p);
         const Layer& L = LAYERS[layer];
-        launch("wgrad_reduce", k_wgrad_reduce, dim3(cdiv((size_t)L.cin * L.cout * NTAP, 256)), dim3(256),
+        launch("wgrad_reduce", k_wgrad_reduce, dim3(cdiv((size_t)L.cin * L.cout * NTAP, RED_W)), dim3(RED_W * RED_G),
                (const float*)part.p, nblk, CI, CO, L.cin, L.cout, g.p + L.offset);
     }
 
@@ -574,7 +598,7 @@ struct rh_denoiser {
         const float gscale = (float)(2.0 / (4.0 * (double)V));
         launch("loss", k_loss, dim3(lb), dim3(LOSS_THREADS), (const float*)y.p, t, V, 1.0f / scale[0],
                1.0f / scale[1], 1.0f / scale[2], 1.0f / scale[3], gscale, reinterpret_cast<bf16x8*>(g4.p), lpart.p);
-        launch("loss_sum", k_sum, dim3(1), dim3(64), (const double*)lpart.p, lb, 1.0 / (4.0 * (double)V), loss.p);
+        launch("loss_sum", k_sum, dim3(1), dim3(LOSS_THREADS), (const double*)lpart.p, lb, 1.0 / (4.0 * (double)V), loss.p);
         // backward-data with the ReLU masks of the forward activations
         conv<8, 2, EP_MASK>("dgrad4", g4.p, bw[3].p, P, X, Y, Z, g3.p, a3.p, nullptr, nullptr);
         conv<32, 2, EP_MASK>("dgrad3", g3.p, bw[2].p, P, X, Y, Z, g2.p, a2.p, nullptr, nullptr);

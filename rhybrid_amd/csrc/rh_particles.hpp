@@ -82,14 +82,76 @@ constexpr int PM_WAVES = PM_THREADS / 64;
 
 RH_HD size_t pm_align8(size_t b) { return (b + 7) & ~size_t(7); }
 
+// A tile's push fields as one contiguous block (32-bit words): faceB of the
+// (t+1)^3 cells -1..t-1 (halo on the low side), cellUe and cellEp of the
+// tile's cells, and the cells' neighbourhood masks.  Written once per field
+// update by k_push_fields and copied to LDS by both passes of k_push_move.
 template <typename Real>
-RH_HD size_t push_move_lds_bytes(const Geom& g) {
+RH_HD size_t push_field_words(const Geom& g) {
     const size_t nh = (size_t)(g.tx + 1) * (g.ty + 1) * (g.tz + 1);
-    return pm_align8(4 * (size_t)g.extk + (size_t)PM_WAVES * g.extk) + sizeof(Real) * (3 * nh + 6 * (size_t)g.tc);
+    const size_t words = (sizeof(Real) * (3 * nh + 6 * (size_t)g.tc) + 4 * (size_t)g.tc) / 4;
+    return (words + 3) & ~size_t(3);  // 16-byte blocks keep every tile's block aligned
 }
 
 template <typename Real>
-__global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp, PushFields<Real> f,
+RH_HD size_t push_move_lds_bytes(const Geom& g) {
+    return pm_align8(4 * (size_t)g.extk + (size_t)PM_WAVES * g.extk) + 4 * push_field_words<Real>(g);
+}
+
+// Neighbourhood mask of a cell: bit (a+1) + 3(b+1) + 9(c+1) is set when the
+// neighbour at offset (a,b,c) exists; bit 27 when all 26 exist
+// (pargrid::ALL_NEIGHBOURS_EXIST).
+constexpr int NMASK_ALL = 27;
+RH_HD uint32_t neighbour_mask(const Geom& g, int i, int j, int k) {
+    // existence is separable: offset (a,b,c) exists iff each axis offset does
+    uint32_t ax[3] = {0, 0, 0};
+    const int idx[3] = {i, j, k}, n[3] = {g.nx, g.ny, g.nz};
+    for (int d = 0; d < 3; ++d)
+        for (int a = -1; a <= 1; ++a)
+            if (g.per[d] || (idx[d] + a >= 0 && idx[d] + a < n[d])) ax[d] |= 1u << (a + 1);
+    uint32_t m = 0;
+    for (int c = 0; c < 3; ++c)
+        for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a)
+                if ((ax[0] >> a) & (ax[1] >> b) & (ax[2] >> c) & 1u) m |= 1u << (a + 3 * b + 9 * c);
+    if (ax[0] == 7u && ax[1] == 7u && ax[2] == 7u) m |= 1u << NMASK_ALL;
+    return m;
+}
+
+// Fields of the push in LDS order (see push_field_words).
+template <typename Real>
+__global__ __launch_bounds__(256) void k_push_fields(Geom g, PushFields<Real> f, int use_e_pressure,
+                                                     uint32_t* __restrict__ out) {
+    const int hx = g.tx + 1, hy = g.ty + 1, nh = (g.tx + 1) * (g.ty + 1) * (g.tz + 1);
+    const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
+    const int ox = blockIdx.x << g.ltx, oy = blockIdx.y << g.lty, oz = blockIdx.z << g.ltz;
+    uint32_t* blk = out + (size_t)T * push_field_words<Real>(g);
+    Real* sbx = reinterpret_cast<Real*>(blk);
+    Real* sby = sbx + nh;
+    Real* sbz = sbx + 2 * nh;
+    Real* sue = sbx + 3 * nh;
+    Real* sep = sue + 3 * g.tc;
+    uint32_t* nmask = reinterpret_cast<uint32_t*>(sep + 3 * g.tc);
+    for (int q = threadIdx.x; q < nh; q += blockDim.x) {
+        const int a = q % hx - 1, b = (q / hx) % hy - 1, c = q / (hx * hy) - 1;
+        const int n = nbr(g, ox, oy, oz, a, b, c);
+        sbx[q] = n >= 0 ? f.fbx[n] : Real(0);
+        sby[q] = n >= 0 ? f.fby[n] : Real(0);
+        sbz[q] = n >= 0 ? f.fbz[n] : Real(0);
+    }
+    for (int q = threadIdx.x; q < g.tc; q += blockDim.x) {
+        int li, lj, lk;
+        local_ijk(g, q, li, lj, lk);
+        const int n = cell_index(g, ox + li, oy + lj, oz + lk);
+        sue[q] = f.uex[n]; sue[g.tc + q] = f.uey[n]; sue[2 * g.tc + q] = f.uez[n];
+        if (use_e_pressure) { sep[q] = f.epx[n]; sep[g.tc + q] = f.epy[n]; sep[2 * g.tc + q] = f.epz[n]; }
+        else { sep[q] = sep[g.tc + q] = sep[2 * g.tc + q] = Real(0); }
+        nmask[q] = neighbour_mask(g, ox + li, oy + lj, oz + lk);
+    }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp, const uint32_t* __restrict__ pfb,
                                                          PArrays<Real> A, PArrays<Real> D,
                                                          const uint32_t* __restrict__ sub,
                                                          uint32_t* __restrict__ cnt_tile,
@@ -106,25 +168,17 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
     Real* sbz = fl + 2 * nh;
     Real* sue = fl + 3 * nh;   // cellUe, cellEp of the tile's cells
     Real* sep = sue + 3 * g.tc;
+    uint32_t* nmask = reinterpret_cast<uint32_t*>(sep + 3 * g.tc);  // neighbourhood of each tile cell
     const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int ox = blockIdx.x << g.ltx, oy = blockIdx.y << g.lty, oz = blockIdx.z << g.ltz;
     for (int e = tid; e < g.extk; e += PM_THREADS) cnt[e] = 0;
     for (int e = tid; e < PM_WAVES * g.extk; e += PM_THREADS) grp[e] = 0;
-    for (int q = tid; q < nh; q += PM_THREADS) {
-        const int a = q % hx - 1, b = (q / hx) % hy - 1, c = q / (hx * hy) - 1;
-        const int n = nbr(g, ox, oy, oz, a, b, c);
-        sbx[q] = n >= 0 ? f.fbx[n] : Real(0);
-        sby[q] = n >= 0 ? f.fby[n] : Real(0);
-        sbz[q] = n >= 0 ? f.fbz[n] : Real(0);
-    }
-    for (int q = tid; q < g.tc; q += PM_THREADS) {
-        int li, lj, lk;
-        local_ijk(g, q, li, lj, lk);
-        const int n = cell_index(g, ox + li, oy + lj, oz + lk);
-        sue[q] = f.uex[n]; sue[g.tc + q] = f.uey[n]; sue[2 * g.tc + q] = f.uez[n];
-        if (pp.use_e_pressure) { sep[q] = f.epx[n]; sep[g.tc + q] = f.epy[n]; sep[2 * g.tc + q] = f.epz[n]; }
-        else { sep[q] = sep[g.tc + q] = sep[2 * g.tc + q] = Real(0); }
+    {
+        const int nw = (int)push_field_words<Real>(g);
+        const uint32_t* src = pfb + (size_t)T * nw;
+        uint32_t* dst = reinterpret_cast<uint32_t*>(fl);
+        for (int q = tid; q < nw; q += PM_THREADS) dst[q] = src[q];
     }
     __syncthreads();
     const uint32_t p0 = sub[(size_t)T * NS], p1 = sub[(size_t)(T + 1) * NS];
@@ -154,8 +208,8 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
         if (act && X >= Real(0)) {
             int li, lj, lk;
             local_ijk(g, lc, li, lj, lk);
-            const int i = ox + li, j = oy + lj, k = oz + lk;
-            const bool accel = pp.accelerate && all_nbrs(g, i, j, k) && !pp.initial_flow_through;
+            const uint32_t nm = nmask[lc];
+            const bool accel = pp.accelerate && ((nm >> NMASK_ALL) & 1u) && !pp.initial_flow_through;
             if (accel) {
                 const int h = (li + 1) + hx * ((lj + 1) + hy * (lk + 1));
                 // face2r: each component linear along its own axis only
@@ -182,9 +236,9 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
                 vy = vmy + v0z * sx - v0x * sz + dvy;
                 vz = vmz + v0x * sy - v0y * sx + dvz;
                 if (pp.use_gravity) {
-                    const double rx = g.x0 + (i + (double)X) * g.dx;
-                    const double ry = g.y0 + (j + (double)Y) * g.dx;
-                    const double rz = g.z0 + (k + (double)Z) * g.dx;
+                    const double rx = g.x0 + (ox + li + (double)X) * g.dx;
+                    const double ry = g.y0 + (oy + lj + (double)Y) * g.dx;
+                    const double rz = g.z0 + (oz + lk + (double)Z) * g.dx;
                     const double r = sqrt(rx * rx + ry * ry + rz * rz);
                     const double r3 = r * r * r;
                     if (r3 > 0) {
@@ -206,16 +260,16 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
             const double fx = floor(Xd), fy = floor(Yd), fz = floor(Zd);
             const bool ok = (fx >= -1.0 && fx <= 1.0 && fy >= -1.0 && fy <= 1.0 && fz >= -1.0 && fz <= 1.0);
             const int sx = (int)fx, sy = (int)fy, sz = (int)fz;
-            if (ok && nbr(g, i, j, k, sx, sy, sz) >= 0) {
+            if (ok && ((nm >> ((sx + 1) + 3 * (sy + 1) + 9 * (sz + 1))) & 1u)) {
                 X = (Real)(Xd - fx); Y = (Real)(Yd - fy); Z = (Real)(Zd - fz);
                 if (X >= Real(1)) X = below_one<Real>();
                 if (Y >= Real(1)) Y = below_one<Real>();
                 if (Z >= Real(1)) Z = below_one<Real>();
                 const int o = (X >= Real(0.5)) + 2 * (Y >= Real(0.5)) + 4 * (Z >= Real(0.5));
                 key = (uint32_t)(((li + sx + 1) + g.ex * ((lj + sy + 1) + g.ey * (lk + sz + 1))) * 8 + o);
-                const int ni = wrap_axis(i + sx, g.nx, g.per[0]), nj = wrap_axis(j + sy, g.ny, g.per[1]),
-                          nk = wrap_axis(k + sz, g.nz, g.per[2]);
-                nlcell = (uint8_t)((ni & (g.tx - 1)) + ((nj & (g.ty - 1)) << g.ltx) + ((nk & (g.tz - 1)) << (g.ltx + g.lty)));
+                // tiles divide the grid, so a periodic wrap keeps the local index
+                nlcell = (uint8_t)(((li + sx) & (g.tx - 1)) + (((lj + sy) & (g.ty - 1)) << g.ltx) +
+                                   (((lk + sz) & (g.tz - 1)) << (g.ltx + g.lty)));
             } else {
                 dropped = true;
             }
@@ -390,14 +444,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const uint32_t* __r
 // src/particle_accumulator.cpp:43-156).  One workgroup per tile, one thread
 // per subcell: a subcell's particles share one 2x2x2 footprint, so the thread
 // sums, in particle order, the 8 corner weights times (1, vx, vy, vz).  The
-// threads take the subcells in ascending order of their particle counts (a
-// counting sort in LDS), so that the lanes of a wave loop about equally long;
-// which thread sums a subcell does not change the sum.  The subcell sums are
-// staged in LDS (two quantities at a time) and every cell of the tile's
-// extended region adds the (up to 64) subcell terms that reach it in a fixed
-// order.  The result is the tile's ghost partial of this species (unscaled);
-// k_fold sums partials over tiles and species in FP64.  No float atomics, so
-// every run gives the same bits.
+// subcell sums are staged in LDS (two quantities at a time) and every cell of
+// the tile's extended region adds the (up to 64) subcell terms that reach it
+// in a fixed order.  The result is the tile's ghost partial of this species
+// (unscaled); k_fold sums partials over tiles and species in FP64.  No
+// atomics, so every run gives the same bits.
 template <typename Real>
 __device__ __forceinline__ void cic_accumulate(Real (&acc)[4][8], bool hx, bool hy, bool hz, Real X, Real Y, Real Z,
                                                Real vx, Real vy, Real vz) {
@@ -419,14 +470,10 @@ __device__ __forceinline__ void cic_accumulate(Real (&acc)[4][8], bool hx, bool 
 
 // LDS of the deposit: half of the subcell sums (2 quantities x 8 corners) as
 // [quantity*8+corner][octant][cell], each octant block padded by one word so
-// that neighbouring cells fall in different banks; the first half's results;
-// the count histogram and, per subcell, first particle, count and the
-// count-ordered subcell list.
-constexpr int DEP_BINS = 64;
+// that neighbouring cells fall in different banks; the first half's results.
 template <typename Real>
 RH_HD size_t deposit_lds_bytes(const Geom& g) {
-    return sizeof(Real) * (16 * 8 * (size_t)(g.tc + 1) + 2 * (size_t)g.next) +
-           sizeof(uint32_t) * (DEP_BINS + 3 * 8 * (size_t)g.tc);
+    return sizeof(Real) * (16 * 8 * (size_t)(g.tc + 1) + 2 * (size_t)g.next);
 }
 
 // For every cell of the extended region, the LDS offsets (corner plane,
@@ -481,39 +528,10 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
     const int QS = 8 * OB;            // one (quantity, corner) plane
     Real* S = reinterpret_cast<Real*>(smem_raw);
     Real* R = S + 16 * QS;  // first-half results, next x 2
-    uint32_t* hist = reinterpret_cast<uint32_t*>(R + 2 * g.next);
-    uint32_t* first = hist + DEP_BINS;  // first particle of each subcell
-    uint32_t* num = first + NS;         // particle count of each subcell
-    uint32_t* order = num + NS;         // subcells in ascending count order
     const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
-    const int tid = threadIdx.x;
-    // counting sort of the subcells by particle count (bins clamp at NB-1)
-    const int NB = NS < DEP_BINS ? NS : DEP_BINS;
-    if (tid < NB) hist[tid] = 0u;
-    {
-        const uint32_t key = (uint32_t)(T * NS + tid);
-        const uint32_t a = sub[key], n = sub[key + 1] - a;
-        first[tid] = a;
-        num[tid] = n;
-        __syncthreads();
-        const int bin = n < (uint32_t)NB ? (int)n : NB - 1;
-        const uint32_t r = atomicAdd(&hist[bin], 1u);
-        __syncthreads();
-        if (tid < NB) {  // exclusive scan of the bins inside the first wave
-            const uint32_t h = hist[tid];
-            uint32_t v = h;
-            for (int off = 1; off < NB; off <<= 1) {
-                const uint32_t u = __shfl_up(v, off);
-                if (tid >= off) v += u;
-            }
-            hist[tid] = v - h;
-        }
-        __syncthreads();
-        order[hist[bin] + r] = (uint32_t)tid;
-        __syncthreads();
-    }
-    const int ls = (int)order[tid];
-    const uint32_t p0 = first[ls], p1 = p0 + num[ls];
+    const int ls = threadIdx.x;
+    const uint32_t key = (uint32_t)(T * NS + ls);
+    const uint32_t p0 = sub[key], p1 = sub[key + 1];
     const int o = ls & 7, cell = ls >> 3;
     const bool hx = o & 1, hy = (o >> 1) & 1, hz = (o >> 2) & 1;
     Real acc[4][8];
@@ -531,7 +549,7 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
 #pragma unroll
             for (int c = 0; c < 8; ++c) S[(qq * 8 + c) * QS + o * OB + cell] = acc[2 * half + qq][c];
         __syncthreads();
-        for (int e = tid; e < g.next; e += blockDim.x) {
+        for (int e = ls; e < g.next; e += blockDim.x) {
             // the subcell terms reaching this cell: a list shared by all tiles
             const int cnt_e = gcount[e];
             const uint16_t* lst = glist + (size_t)e * 64;
