@@ -610,10 +610,15 @@ public:
         if (!pf_dirty) return;
         for (auto& s_ : sp)
             if (s_->pushed) throw StateError("fields changed between propagate and wait_particle_sends");
-        if (!pf_tile.p) pf_tile.alloc((size_t)g.ntiles * push_field_words<Real>(g));
+        if (!pf_tile.p) {
+            pf_tile.alloc((size_t)g.ntiles * push_field_words<Real>(g));
+            // halo entries of cells outside a non-periodic grid stay zero
+            launch("fill", k_fill<uint32_t>, cdiv(pf_tile.n, 256), 256, 0, pf_tile.p, 0u, pf_tile.n);
+        }
         const V3<Real> fb = v3(RH_FACE_B), ue = v3(RH_CELL_UE), ep = v3(RH_CELL_EP);
         PushFields<Real> pf{fb.x, fb.y, fb.z, ue.x, ue.y, ue.z, ep.x, ep.y, ep.z};
-        launch("push_fields", k_push_fields<Real>, tile_grid(), 256, 0, g, pf, (int)P.use_e_pressure, pf_tile.p);
+        launch("push_fields", k_push_fields<Real>, cell_grid(), cell_block(), 0, g, pf, (int)P.use_e_pressure,
+               pf_tile.p);
         pf_dirty = false;
     }
 

@@ -118,36 +118,49 @@ RH_HD uint32_t neighbour_mask(const Geom& g, int i, int j, int k) {
     return m;
 }
 
-// Fields of the push in LDS order (see push_field_words).
+// Fields of the push in LDS order (see push_field_words).  One thread per
+// grid cell (coalesced reads of the nine arrays): the cell writes its faceB,
+// cellUe, cellEp and neighbourhood mask into its own tile's block, and its
+// faceB into the low-side halo of the tiles above it when it lies on its
+// tile's upper faces (up to seven, periodic wrap included).  Halo entries of
+// cells that do not exist are never written and keep the zero of the
+// allocation.
 template <typename Real>
 __global__ __launch_bounds__(256) void k_push_fields(Geom g, PushFields<Real> f, int use_e_pressure,
                                                      uint32_t* __restrict__ out) {
+    const int i = blockIdx.x * 32 + threadIdx.x, j = blockIdx.y * 8 + threadIdx.y, k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int n = cell_index(g, i, j, k);
     const int hx = g.tx + 1, hy = g.ty + 1, nh = (g.tx + 1) * (g.ty + 1) * (g.tz + 1);
-    const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
-    const int ox = blockIdx.x << g.ltx, oy = blockIdx.y << g.lty, oz = blockIdx.z << g.ltz;
-    uint32_t* blk = out + (size_t)T * push_field_words<Real>(g);
-    Real* sbx = reinterpret_cast<Real*>(blk);
-    Real* sby = sbx + nh;
-    Real* sbz = sbx + 2 * nh;
-    Real* sue = sbx + 3 * nh;
-    Real* sep = sue + 3 * g.tc;
-    uint32_t* nmask = reinterpret_cast<uint32_t*>(sep + 3 * g.tc);
-    for (int q = threadIdx.x; q < nh; q += blockDim.x) {
-        const int a = q % hx - 1, b = (q / hx) % hy - 1, c = q / (hx * hy) - 1;
-        const int n = nbr(g, ox, oy, oz, a, b, c);
-        sbx[q] = n >= 0 ? f.fbx[n] : Real(0);
-        sby[q] = n >= 0 ? f.fby[n] : Real(0);
-        sbz[q] = n >= 0 ? f.fbz[n] : Real(0);
+    const size_t nw = push_field_words<Real>(g);
+    const int li = i & (g.tx - 1), lj = j & (g.ty - 1), lk = k & (g.tz - 1);
+    const int lc = li + (lj << g.ltx) + (lk << (g.ltx + g.lty));
+    const Real bx = f.fbx[n], by = f.fby[n], bz = f.fbz[n];
+    {
+        Real* blk = reinterpret_cast<Real*>(out + (size_t)tile_of(g, i, j, k) * nw);
+        const int h = (li + 1) + hx * ((lj + 1) + hy * (lk + 1));
+        blk[h] = bx; blk[nh + h] = by; blk[2 * nh + h] = bz;
+        Real* sue = blk + 3 * nh;
+        Real* sep = sue + 3 * g.tc;
+        sue[lc] = f.uex[n]; sue[g.tc + lc] = f.uey[n]; sue[2 * g.tc + lc] = f.uez[n];
+        if (use_e_pressure) { sep[lc] = f.epx[n]; sep[g.tc + lc] = f.epy[n]; sep[2 * g.tc + lc] = f.epz[n]; }
+        else { sep[lc] = sep[g.tc + lc] = sep[2 * g.tc + lc] = Real(0); }
+        reinterpret_cast<uint32_t*>(sep + 3 * g.tc)[lc] = neighbour_mask(g, i, j, k);
     }
-    for (int q = threadIdx.x; q < g.tc; q += blockDim.x) {
-        int li, lj, lk;
-        local_ijk(g, q, li, lj, lk);
-        const int n = cell_index(g, ox + li, oy + lj, oz + lk);
-        sue[q] = f.uex[n]; sue[g.tc + q] = f.uey[n]; sue[2 * g.tc + q] = f.uez[n];
-        if (use_e_pressure) { sep[q] = f.epx[n]; sep[g.tc + q] = f.epy[n]; sep[2 * g.tc + q] = f.epz[n]; }
-        else { sep[q] = sep[g.tc + q] = sep[2 * g.tc + q] = Real(0); }
-        nmask[q] = neighbour_mask(g, ox + li, oy + lj, oz + lk);
-    }
+    // halo copies: the cell is local coordinate -1 of the tile above along
+    // each axis where it is the tile's last cell
+    const int ux = li == g.tx - 1, uy = lj == g.ty - 1, uz = lk == g.tz - 1;
+    for (int c = 0; c <= uz; ++c)
+        for (int b = 0; b <= uy; ++b)
+            for (int a = 0; a <= ux; ++a) {
+                if (a + b + c == 0) continue;
+                const int ti = wrap_axis(i + a, g.nx, g.per[0]), tj = wrap_axis(j + b, g.ny, g.per[1]),
+                          tk = wrap_axis(k + c, g.nz, g.per[2]);
+                if (ti < 0 || tj < 0 || tk < 0) continue;
+                Real* blk = reinterpret_cast<Real*>(out + (size_t)tile_of(g, ti, tj, tk) * nw);
+                const int h = (a ? 0 : li + 1) + hx * ((b ? 0 : lj + 1) + hy * (c ? 0 : lk + 1));
+                blk[h] = bx; blk[nh + h] = by; blk[2 * nh + h] = bz;
+            }
 }
 
 template <typename Real>
