@@ -58,14 +58,14 @@ def kernel_bytes(name: str, ctx, np_total: float, nspecies: int) -> float:
         return 24.0 * npp + 4.0 * nkeys + 64.0 * next_ * ntiles
     if name == "fold":       # read FP64 tile ghosts once, write rho + J (f32)
         return 32.0 * next_ * ntiles + 16.0 * nc
-    if name in ("sort_count", "sort_base"):
-        return 4.0 * extk * ntiles * (2 if name == "sort_base" else 1) + 4.0 * nkeys
+    if name == "sort_slots":  # read and write the 16-bit slot table, write the totals
+        return 4.0 * extk * ntiles + 4.0 * nkeys
     return 0.0
 
 
 PROFILE_KERNELS = {"push": "rh::k_push_move<float>_[pass_0:_push]", "move": "rh::k_push_move<float>_[pass_1:_move]",
                    "deposit": "rh::k_deposit<float>", "fold": "rh::k_fold<float>",
-                   "sort_count": "rh::k_slot_offsets<false>", "sort_base": "rh::k_slot_offsets<true>"}
+                   "sort_slots": "rh::k_slot_offsets<true>"}
 
 
 def measured_traffic(kernel: str):
@@ -111,7 +111,7 @@ def gpu_bench(args):
     ctx.set_timing(False)
     np_after = sum(ctx.num_particles(s) for s in range(nspec))
     np_mean = 0.5 * (np_before + np_after)
-    names = ["push_fields", "push", "sort_count", "scan", "sort_base", "move", "deposit", "fold", "boundary",
+    names = ["push_fields", "push", "sort_slots", "sort_count", "scan", "sort_base", "move", "deposit", "fold", "boundary",
              "face2cell", "neumann", "imf", "cell2node", "node_J", "node2cell", "cell_Ue", "node_Ue",
              "upwind_B", "node_E", "node_avg", "face_curl", "cell_Ep", "fill", "diag"]
     ktimes = {}
@@ -120,7 +120,7 @@ def gpu_bench(args):
         if n:
             ktimes[nm] = (ms, n)
     total_k = sum(v[0] for v in ktimes.values())
-    pd_names = ("push_fields", "push", "sort_count", "scan", "sort_base", "move", "deposit", "fold")
+    pd_names = ("push_fields", "push", "sort_slots", "sort_count", "scan", "sort_base", "move", "deposit", "fold")
     t_pd = sum(ktimes[k][0] for k in pd_names if k in ktimes) / 1e3
     dom = max(ktimes, key=lambda k: ktimes[k][0])
     per_kernel = {}

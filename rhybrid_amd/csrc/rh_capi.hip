@@ -138,14 +138,16 @@ template <typename Real>
 class Sim final : public Ctx {
 public:
     // Per species: the sorted store (a, sub) and the destination of the next
-    // sort (d, sub_new), swapped after every sort; cnt_tile holds the slot
-    // counts of pass 0 and then the slot bases of pass 1.
+    // sort (d, sub_new), swapped after every sort; cnt_tile holds the 16-bit
+    // slot counts of pass 0 and then, for pass 1, each slot's offset inside
+    // its destination subcell's range (sub_new holds the ranges' starts).
     struct Species {
         rh_species_desc d;
         size_t cap = 0;
         DevBuf<Real> a[6], dst[6];
         DevBuf<uint8_t> a_lc, dst_lc;
-        DevBuf<uint32_t> sub, sub_new, cnt_tile;
+        DevBuf<uint32_t> sub, sub_new;
+        DevBuf<uint16_t> cnt_tile;
         DevBuf<Real> partial;  // deposit: per-tile ghost partials (ntiles x next x 4)
         bool deposited = false;
         bool pushed = false;  // pass 0 done, pass 1 pending
@@ -541,7 +543,8 @@ public:
         prepare_push_fields();
         launch("push", k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, pp,
                (const uint32_t*)pf_tile.p,
-               S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.cnt_tile.p, dcount.p, kbits, 0, (uint32_t)S_.cap);
+               S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.cnt_tile.p, (const uint32_t*)S_.sub_new.p, dcount.p,
+               kbits, 0, (uint32_t)S_.cap);
         S_.pp = pp;
         S_.pushed = true;
     }
@@ -562,12 +565,13 @@ public:
     void wait_particle_sends(int s) override {
         Species& S_ = S(s);
         if (!S_.pushed) throw StateError("wait_particle_sends without a preceding propagate");
-        launch("sort_count", k_slot_offsets<false>, tile_grid(), g.tc * 8, 0, g, S_.cnt_tile.p, S_.sub_new.p);
+        // totals and in-subcell offsets in one pass, then the scan of the totals
+        launch("sort_slots", k_slot_offsets<true>, tile_grid(), g.tc * 8, 0, g, S_.cnt_tile.p, S_.sub_new.p, dcount.p);
         scan(S_.sub_new.p, g.nkeys, S_.sub_new.p, 0);
-        launch("sort_base", k_slot_offsets<true>, tile_grid(), g.tc * 8, 0, g, S_.cnt_tile.p, S_.sub_new.p);
         launch("move", k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, S_.pp,
                (const uint32_t*)pf_tile.p,
-               S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.cnt_tile.p, dcount.p, kbits, 1, (uint32_t)S_.cap);
+               S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.cnt_tile.p, (const uint32_t*)S_.sub_new.p, dcount.p,
+               kbits, 1, (uint32_t)S_.cap);
         S_.swap_stores();
         S_.pushed = false;
     }
@@ -748,7 +752,9 @@ public:
         for (int sl = 0; sl < COUNTER_SLOTS; ++sl)
             for (int q = 0; q < C_N; ++q) h[q] += hs[(size_t)sl * C_N + q];
         for (int q = 0; q < RH_N_COUNTERS; ++q) out[q] = (double)h[q];
-        if (h[C_RANK_OVERFLOW]) throw CapError("particle sort overflow (a subcell slot exceeded 2^20 particles in one tile)");
+        if (h[C_RANK_OVERFLOW])
+            throw CapError("particle sort overflow (a subcell holds more than 65535 particles of one species, "
+                           "or the particle store is full)");
     }
 
     void synchronize() override { RH_HIP(hipStreamSynchronize(stream)); }

@@ -167,7 +167,8 @@ template <typename Real>
 __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp, const uint32_t* __restrict__ pfb,
                                                          PArrays<Real> A, PArrays<Real> D,
                                                          const uint32_t* __restrict__ sub,
-                                                         uint32_t* __restrict__ cnt_tile,
+                                                         uint16_t* __restrict__ cnt_tile,
+                                                         const uint32_t* __restrict__ key_base,
                                                          unsigned long long* __restrict__ counters, int kbits,
                                                          int pass, uint32_t cap) {
     extern __shared__ unsigned char pm_smem[];
@@ -216,6 +217,7 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
             }
         }
         uint32_t key = deadkey;
+        uint32_t gkey = 0;   // destination subcell key of the whole grid (pass 1)
         bool clamped = false, dropped = false;
         uint8_t nlcell = 0;  // destination cell inside the destination tile
         if (act && X >= Real(0)) {
@@ -283,6 +285,16 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
                 // tiles divide the grid, so a periodic wrap keeps the local index
                 nlcell = (uint8_t)(((li + sx) & (g.tx - 1)) + (((lj + sy) & (g.ty - 1)) << g.ltx) +
                                    (((lk + sz) & (g.tz - 1)) << (g.ltx + g.lty)));
+                if (pass == 1) {
+                    // destination tile: the neighbour tile where the local coordinate left the tile
+                    int tX = (int)blockIdx.x + (li + sx < 0 ? -1 : (li + sx >= g.tx ? 1 : 0));
+                    int tY = (int)blockIdx.y + (lj + sy < 0 ? -1 : (lj + sy >= g.ty ? 1 : 0));
+                    int tZ = (int)blockIdx.z + (lk + sz < 0 ? -1 : (lk + sz >= g.tz ? 1 : 0));
+                    tX += tX < 0 ? g.ntx : (tX >= g.ntx ? -g.ntx : 0);
+                    tY += tY < 0 ? g.nty : (tY >= g.nty ? -g.nty : 0);
+                    tZ += tZ < 0 ? g.ntz : (tZ >= g.ntz ? -g.ntz : 0);
+                    gkey = ((uint32_t)(tX + g.ntx * (tY + g.nty * tZ)) * (uint32_t)g.tc + nlcell) * 8u + (uint32_t)o;
+                }
             } else {
                 dropped = true;
             }
@@ -310,7 +322,7 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
         if (key != deadkey) {
             uint32_t rank = cnt[key] + rin;
             for (int v = 0; v < w; ++v) rank += grp[v * g.extk + key];
-            const uint32_t dst = cnt_tile[(size_t)T * g.extk + key] + rank;
+            const uint32_t dst = key_base[gkey] + cnt_tile[(size_t)T * g.extk + key] + rank;
             if (dst < cap) {
                 D.x[dst] = X; D.y[dst] = Y; D.z[dst] = Z;
                 D.vx[dst] = vx; D.vy[dst] = vy; D.vz[dst] = vz;
@@ -328,14 +340,18 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
     }
     __syncthreads();
     if (pass == 0)
-        for (int e = tid; e < g.extk; e += PM_THREADS) cnt_tile[(size_t)T * g.extk + e] = cnt[e];
+        for (int e = tid; e < g.extk; e += PM_THREADS) {
+            const uint32_t c = cnt[e];
+            if (c > 0xFFFFu) atomicAdd(counters + C_RANK_OVERFLOW, 1ull);  // slot 0
+            cnt_tile[(size_t)T * g.extk + e] = (uint16_t)c;
+        }
 }
 
 // Number of particles arriving in each subcell: the slot counts of every tile
-// whose extended region holds the cell, in a fixed order.  With WRITE_BASE,
-// the slot counts are replaced by the first destination index of each slot
-// (out must already hold the exclusive scan of the totals).  One block per
-// tile (3D grid of tiles), one thread per subcell of the tile.
+// whose extended region holds the cell, in a fixed order (out = totals, to be
+// scanned into the subcells' first destination index).  With WRITE_BASE the
+// slot counts are also replaced by each slot's offset inside its subcell's
+// range.  One block per tile (3D grid of tiles), one thread per subcell.
 // axis_candidates for an axis with at least three tiles or without wrap: the
 // previous tile (cell on the tile's low face), the own tile, the next tile
 // (cell on the high face), in that order.
@@ -349,8 +365,9 @@ __device__ __forceinline__ int axis_candidates_fast(int i, int lt, int nt, int p
 }
 
 template <bool WRITE_BASE>
-__global__ __launch_bounds__(512) void k_slot_offsets(Geom g, uint32_t* __restrict__ cnt_tile,
-                                                     uint32_t* __restrict__ out) {
+__global__ __launch_bounds__(512) void k_slot_offsets(Geom g, uint16_t* __restrict__ cnt_tile,
+                                                     uint32_t* __restrict__ out,
+                                                     unsigned long long* __restrict__ counters) {
     const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
     const int tid = threadIdx.x;
     const uint32_t key = (uint32_t)T * (uint32_t)(g.tc * 8) + tid;
@@ -360,23 +377,57 @@ __global__ __launch_bounds__(512) void k_slot_offsets(Geom g, uint32_t* __restri
     const int i = (blockIdx.x << g.ltx) + li, j = (blockIdx.y << g.lty) + lj, k = (blockIdx.z << g.ltz) + lk;
     int tx[6], ex[6], ty[6], ey[6], tz[6], ez[6];
     const bool fx = !g.per[0] || g.ntx >= 3, fy = !g.per[1] || g.nty >= 3, fz = !g.per[2] || g.ntz >= 3;
-    const int mx = fx ? axis_candidates_fast(i, g.ltx, g.ntx, g.per[0], tx, ex)
-                      : axis_candidates(i, g.nx, g.ltx, g.ntx, g.per[0], tx, ex);
-    const int my = fy ? axis_candidates_fast(j, g.lty, g.nty, g.per[1], ty, ey)
-                      : axis_candidates(j, g.ny, g.lty, g.nty, g.per[1], ty, ey);
-    const int mz = fz ? axis_candidates_fast(k, g.ltz, g.ntz, g.per[2], tz, ez)
-                      : axis_candidates(k, g.nz, g.ltz, g.ntz, g.per[2], tz, ez);
-    uint32_t run = WRITE_BASE ? out[key] : 0u;
-    for (int c = 0; c < mz; ++c)
-        for (int b = 0; b < my; ++b)
-            for (int a = 0; a < mx; ++a) {
-                const int TT = tx[a] + g.ntx * (ty[b] + g.nty * tz[c]);
-                const size_t slot = (size_t)TT * g.extk + (size_t)((ex[a] + g.ex * (ey[b] + g.ey * ez[c])) * 8 + o);
-                const uint32_t v = cnt_tile[slot];
-                if (WRITE_BASE) cnt_tile[slot] = run;
-                run += v;
-            }
-    if (!WRITE_BASE) out[key] = run;
+    uint32_t run = 0;
+    if (fx && fy && fz) {
+        // at most three candidates per axis: all loads issued before the
+        // ordered accumulation
+        const int mx = axis_candidates_fast(i, g.ltx, g.ntx, g.per[0], tx, ex);
+        const int my = axis_candidates_fast(j, g.lty, g.nty, g.per[1], ty, ey);
+        const int mz = axis_candidates_fast(k, g.ltz, g.ntz, g.per[2], tz, ez);
+        size_t slot[27];
+        uint32_t v[27];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const int q = a + 3 * b + 9 * c;
+                    const bool ok = a < mx && b < my && c < mz;
+                    const int TT = ok ? tx[a] + g.ntx * (ty[b] + g.nty * tz[c]) : T;
+                    slot[q] = (size_t)TT * g.extk + (size_t)((ok ? ex[a] + g.ex * (ey[b] + g.ey * ez[c]) : 0) * 8 + o);
+                    v[q] = ok ? (uint32_t)cnt_tile[slot[q]] : 0u;
+                }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const int q = a + 3 * b + 9 * c;
+                    if (!(a < mx && b < my && c < mz)) continue;
+                    if (WRITE_BASE) cnt_tile[slot[q]] = (uint16_t)run;  // offset inside the subcell's range
+                    run += v[q];
+                }
+    } else {
+        const int mx = fx ? axis_candidates_fast(i, g.ltx, g.ntx, g.per[0], tx, ex)
+                          : axis_candidates(i, g.nx, g.ltx, g.ntx, g.per[0], tx, ex);
+        const int my = fy ? axis_candidates_fast(j, g.lty, g.nty, g.per[1], ty, ey)
+                          : axis_candidates(j, g.ny, g.lty, g.nty, g.per[1], ty, ey);
+        const int mz = fz ? axis_candidates_fast(k, g.ltz, g.ntz, g.per[2], tz, ez)
+                          : axis_candidates(k, g.nz, g.ltz, g.ntz, g.per[2], tz, ez);
+        for (int c = 0; c < mz; ++c)
+            for (int b = 0; b < my; ++b)
+                for (int a = 0; a < mx; ++a) {
+                    const int TT = tx[a] + g.ntx * (ty[b] + g.nty * tz[c]);
+                    const size_t slot = (size_t)TT * g.extk + (size_t)((ex[a] + g.ex * (ey[b] + g.ey * ez[c])) * 8 + o);
+                    const uint32_t v = cnt_tile[slot];
+                    if (WRITE_BASE) cnt_tile[slot] = (uint16_t)run;
+                    run += v;
+                }
+    }
+    out[key] = run;
+    if (run > 0xFFFFu) atomicAdd(counters + C_RANK_OVERFLOW, 1ull);  // offsets are 16-bit
 }
 
 // Exclusive scan of uint32 counts (three passes: block sums, scan of the block
