@@ -228,11 +228,13 @@ def cpu_baseline(threads: int):
     """The FP64 oracle (the reference algorithm in the reference's data
     layout): timed on a 64x48x48-cell box of the same workload (1/8 of the
     particles) over 2 steps after 1 warm-up step, on `threads` cores and on
-    one core over 1 step; and, as the checker, the matched-error table of the
-    FP32 GPU path against it on the 32^3 periodic box (64 ppc, 2.1 M
-    particles) after 10 full steps from the same upload."""
+    one core over 1 step.  As the checker, it also gives the matched-error
+    tables of the FP32 GPU path: on that obstacle box after its 3 steps, and
+    on the 32^3 periodic box (64 ppc, 2.1 M particles) after 100 steps, both
+    from the same upload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+    from rhybrid_amd.gpu import RH_FP32, Context
     O.build()
     pr = workload(scale=2)
     o = O.Oracle(pr, threads=threads)
@@ -242,6 +244,12 @@ def cpu_baseline(threads: int):
     steps = 2
     o.step(steps)
     dt = time.perf_counter() - t0
+    ctx = Context(pr, precision=RH_FP32)
+    ctx.step(1 + steps)
+    err_obstacle = {"workload": "64x48x48 obstacle box of the same workload, 3 steps"}
+    for key, nm, dim in (("n", "cellRhoQi", 1), ("J", "cellJi", 3), ("E", "nodeE", 3), ("B", "faceB", 3)):
+        err_obstacle[key] = _rel_errors(ctx.field(nm), o.field(nm).copy(), dim)
+    ctx.close()
     del o
     o1 = O.Oracle(pr, threads=1)
     n1 = sum(o1.num_particles(s) for s in range(o1.nspecies))
@@ -253,10 +261,9 @@ def cpu_baseline(threads: int):
            "sample": "FP64 oracle, 64x48x48-cell box of the same workload (%d particles), %d steps, "
                      "full step incl. field update" % (n, steps),
            "value_1core": n1 / dt1}
-    # matched error of the FP32 GPU path
+    # matched error of the FP32 GPU path on the 32^3 periodic box
     from rhybrid_amd import problem as P
-    from rhybrid_amd.gpu import RH_FP32, Context
-    err_steps = 10
+    err_steps = 100
     pr = P.config_periodic(n=32, ppc=64)
     ctx = Context(pr, precision=RH_FP32)
     ctx.step(err_steps)
@@ -268,6 +275,7 @@ def cpu_baseline(threads: int):
                        "|X_gpu - X_oracle|_2 / |X_oracle|_2, max_rel over cells with |X| > 1e-3 max|X|" % err_steps}
     for key, nm, dim in (("n", "cellRhoQi", 1), ("J", "cellJi", 3), ("E", "nodeE", 3), ("B", "faceB", 3)):
         err[key] = _rel_errors(g[nm], o.field(nm).copy(), dim)
+    err["obstacle_box"] = err_obstacle
     return cpu, err
 
 

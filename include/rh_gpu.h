@@ -160,9 +160,10 @@ int rh_step(rh_ctx* ctx, int32_t nsteps);
  * (cellRhoQi, cellJi) by the denoiser's output, with cellB as the magnetic
  * input channels, for callers that run at low ppc.  Call between
  * rh_add_remote_updates and rh_propagate_B.  Fallback bound: a cell whose
- * denoised charge density differs from the deposited one by more than
- * max_rel_change * |rho| keeps its deposited moments and is counted in
- * RH_CNT_DENOISE_FALLBACK; the rho floor is applied again afterwards.
+ * denoised charge density differs from the binomially filtered deposit
+ * (weights 1/4, 1/2, 1/4 per axis) by more than max_rel_change * |filtered|
+ * keeps its deposited moments and is counted in RH_CNT_DENOISE_FALLBACK; the
+ * rho floor is applied again afterwards.
  * Needs nx % 16 == 0, ny % 4 == 0, nz % 4 == 0 and a denoiser created for at
  * least nx*ny*nz voxels. */
 typedef struct rh_denoiser rh_denoiser;

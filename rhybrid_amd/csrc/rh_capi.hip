@@ -105,27 +105,46 @@ public:
 };
 
 // Moments <-> denoiser volume [Z][Y][X][7] (n, Ji, cellB) and back; the
-// cell index n = i + nx*(j + ny*k) is the voxel index of the volume.
+// cell index n = i + nx*(j + ny*k) is the voxel index of the volume.  The
+// gather also writes the binomially filtered charge density (weights 1/4,
+// 1/2, 1/4 along each axis; a missing neighbour's weight goes to the cell
+// itself), the reference the fallback bound is measured against.
 template <typename Real>
-__global__ void k_dn_gather(int nc, const Real* __restrict__ rho, V3<Real> ji, V3<Real> b, float* __restrict__ out) {
+__global__ void k_dn_gather(Geom g, const Real* __restrict__ rho, V3<Real> ji, V3<Real> b, float* __restrict__ out,
+                            float* __restrict__ filt) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nc) return;
+    if (n >= g.nc) return;
     float* o = out + (size_t)n * 7;
     o[0] = (float)rho[n];
     o[1] = (float)ji.x[n]; o[2] = (float)ji.y[n]; o[3] = (float)ji.z[n];
     o[4] = (float)b.x[n]; o[5] = (float)b.y[n]; o[6] = (float)b.z[n];
+    int i, j, k;
+    cell_ijk(g, n, i, j, k);
+    const float w1[3] = {0.25f, 0.5f, 0.25f};
+    float acc = 0.0f;
+    for (int c = -1; c <= 1; ++c)
+        for (int bb = -1; bb <= 1; ++bb)
+            for (int a = -1; a <= 1; ++a) {
+                const int m = nbr(g, i, j, k, a, bb, c);
+                acc += w1[a + 1] * w1[bb + 1] * w1[c + 1] * (float)rho[m >= 0 ? m : n];
+            }
+    filt[n] = acc;
 }
 
+// Fallback bound: a cell whose denoised charge density differs from the
+// binomially filtered deposit by more than max_rel * |filtered| keeps its
+// deposited moments (also for a non-finite output) and is counted.
 template <typename Real>
-__global__ void k_dn_scatter(int nc, const float* __restrict__ y, double max_rel, double min_rho, Real* __restrict__ rho,
-                             V3<Real> ji, unsigned long long* __restrict__ counters) {
+__global__ void k_dn_scatter(int nc, const float* __restrict__ y, const float* __restrict__ filt, double max_rel,
+                             double min_rho, Real* __restrict__ rho, V3<Real> ji,
+                             unsigned long long* __restrict__ counters) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     bool fallback = false;
     if (n < nc) {
         const float* q = y + (size_t)n * 4;
-        const double r0 = (double)rho[n], r1 = (double)q[0];
-        if (!(fabs(r1 - r0) <= max_rel * fabs(r0))) {
-            fallback = true;  // keep the deposited moments (also for a non-finite output)
+        const double f = (double)filt[n], r1 = (double)q[0];
+        if (!(fabs(r1 - f) <= max_rel * fabs(f))) {
+            fallback = true;
         } else {
             rho[n] = (Real)(r1 < min_rho ? min_rho : r1);
             ji.x[n] = (Real)q[1]; ji.y[n] = (Real)q[2]; ji.z[n] = (Real)q[3];
@@ -604,7 +623,7 @@ public:
                fld[RH_CELL_RHOQ].p, ji.x, ji.y, ji.z, dcount.p);
     }
 
-    DevBuf<float> dn_in, dn_out;  // denoiser volumes (allocated on first use)
+    DevBuf<float> dn_in, dn_out, dn_filt;  // denoiser volumes, filtered density (allocated on first use)
     DevBuf<uint32_t> pf_tile;     // push fields, one block per tile (k_push_fields)
     bool pf_dirty = true;         // fields changed since the blocks were written
 
@@ -633,16 +652,17 @@ public:
         if (!dn_in.p) {
             dn_in.alloc((size_t)g.nc * 7);
             dn_out.alloc((size_t)g.nc * 4);
+            dn_filt.alloc((size_t)g.nc);
         }
         const V3<Real> ji = v3(RH_CELL_JI), b = v3(RH_CELL_B);
-        launch("dn_gather", k_dn_gather<Real>, cdiv(g.nc, 256), 256, 0, g.nc, (const Real*)fld[RH_CELL_RHOQ].p, ji, b,
-               dn_in.p);
+        launch("dn_gather", k_dn_gather<Real>, cdiv(g.nc, 256), 256, 0, g, (const Real*)fld[RH_CELL_RHOQ].p, ji, b,
+               dn_in.p, dn_filt.p);
         RH_HIP(hipStreamSynchronize(stream));  // the denoiser runs on its own stream
         if (rh_dn_forward(dn, dn_in.p, dn_out.p, 1, g.nx, g.ny, g.nz) != RH_DN_OK ||
             rh_dn_synchronize(dn) != RH_DN_OK)
             throw ArgError(std::string("denoiser: ") + rh_dn_last_error(dn));
-        launch("dn_scatter", k_dn_scatter<Real>, cdiv(g.nc, 256), 256, 0, g.nc, (const float*)dn_out.p, max_rel,
-               P.min_rho, fld[RH_CELL_RHOQ].p, ji, dcount.p);
+        launch("dn_scatter", k_dn_scatter<Real>, cdiv(g.nc, 256), 256, 0, g.nc, (const float*)dn_out.p,
+               (const float*)dn_filt.p, max_rel, P.min_rho, fld[RH_CELL_RHOQ].p, ji, dcount.p);
     }
 
     void apply_bcs(int s) override {

@@ -68,15 +68,15 @@ template <> __device__ __forceinline__ double below_one<double>() { return 0x1.f
 //           64-194, fields by face2r/cell2r, src/hybrid_propagator.cpp:
 //           1740-1891), the move in FP64, and the destination slot in the
 //           tile's extended region; counts particles per slot;
-//   pass 1: the same, then writes each particle to (slot base + rank), the
-//           slot bases being the scan of all tiles' counts.
+//   pass 1: the same, then writes each particle to (destination subcell's
+//           start + the slot's offset inside it + rank).
 // The tile's particles are taken 256 at a time.  Pass 0 counts slots with
 // integer LDS atomics.  In pass 1 the rank of a particle is its position among
 // the tile's particles with the same slot in memory order (the migration is a
 // stable counting sort): each wave ranks its lanes with ballots and publishes
 // its per-slot group sizes; a lane's rank adds the slot's running count and
 // the group sizes of the earlier waves.
-// The tile's fields are staged in LDS; removed particles carry x < 0.
+// The tile's field block is copied to LDS; removed particles carry x < 0.
 constexpr int PM_THREADS = 256;
 constexpr int PM_WAVES = PM_THREADS / 64;
 
@@ -532,12 +532,17 @@ __device__ __forceinline__ void cic_accumulate(Real (&acc)[4][8], bool hx, bool 
     }
 }
 
+// Octant block of the deposit's LDS staging: the tile's cells padded by 8
+// words, so that the 8 octants x 8 cells a wave writes fall in distinct banks
+// (octant stride = 8 banks).
+RH_HD int deposit_octant_block(const Geom& g) { return g.tc + 8; }
+
 // LDS of the deposit: half of the subcell sums (2 quantities x 8 corners) as
-// [quantity*8+corner][octant][cell], each octant block padded by one word so
-// that neighbouring cells fall in different banks; the first half's results.
+// [quantity*8+corner][octant][cell] (octant blocks of deposit_octant_block
+// words); the first half's results.
 template <typename Real>
 RH_HD size_t deposit_lds_bytes(const Geom& g) {
-    return sizeof(Real) * (16 * 8 * (size_t)(g.tc + 1) + 2 * (size_t)g.next);
+    return sizeof(Real) * (16 * 8 * (size_t)deposit_octant_block(g) + 2 * (size_t)g.next);
 }
 
 // For every cell of the extended region, the LDS offsets (corner plane,
@@ -545,7 +550,7 @@ RH_HD size_t deposit_lds_bytes(const Geom& g) {
 // corner = target along each axis.  The same for every tile of a shape;
 // built once on the host, at most 64 entries per cell.
 inline int deposit_gather_lists(const Geom& g, uint16_t* list, uint8_t* count) {
-    const int OB = g.tc + 1, QS = 8 * OB;
+    const int OB = deposit_octant_block(g), QS = 8 * OB;
     for (int e = 0; e < g.next; ++e) {
         const int lx = e % g.ex - 1, ly = (e / g.ex) % g.ey - 1, lz = e / (g.ex * g.ey) - 1;
         int m = 0;
@@ -588,7 +593,7 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
                                                  const uint8_t* __restrict__ gcount, Real* __restrict__ partial) {
     extern __shared__ unsigned char smem_raw[];
     const int NS = g.tc * 8;
-    const int OB = g.tc + 1;          // octant block (padded)
+    const int OB = deposit_octant_block(g);  // octant block (padded)
     const int QS = 8 * OB;            // one (quantity, corner) plane
     Real* S = reinterpret_cast<Real*>(smem_raw);
     Real* R = S + 16 * QS;  // first-half results, next x 2

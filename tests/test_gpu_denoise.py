@@ -159,11 +159,20 @@ def test_weights_roundtrip_and_errors(dn_mod):
     d.close()
 
 
+def binomial_filter_periodic(rho, shape):
+    """[1,2,1]/4 along each axis with periodic wrap (the 16^3 box below)."""
+    r = rho.reshape(shape)
+    for ax in range(3):
+        r = 0.25 * np.roll(r, 1, axis=ax) + 0.5 * r + 0.25 * np.roll(r, -1, axis=ax)
+    return r.reshape(-1)
+
+
 def test_denoise_moments_in_context(dn_mod, gpu_mod):
-    """rh_denoise_moments: zero weights leave the moments as deposited; with
-    a loose bound the moments become the denoiser's output on the volume
-    (n, Ji, cellB); with a zero bound every cell whose rho the network
-    changes keeps its moments and is counted."""
+    """rh_denoise_moments: with zero weights the output is the deposit, and
+    exactly the cells whose deposit departs from its binomial filter by more
+    than the bound are counted as fallbacks (moments unchanged either way);
+    with a loose bound the moments become the denoiser's output on the volume
+    (n, Ji, cellB); with a zero bound every cell keeps its moments."""
     from rhybrid_amd import problem as P
     pr = P.config_periodic(n=16, ppc=8)
     ctx = gpu_mod.Context(pr, precision=gpu_mod.RH_FP32)
@@ -173,10 +182,16 @@ def test_denoise_moments_in_context(dn_mod, gpu_mod):
     rho0, ji0, b0 = ctx.field("cellRhoQi"), ctx.field("cellJi"), ctx.field("cellB")
     scales = [3e6 * P.QE, 3e6 * P.QE * 4e5, 3e6 * P.QE * 4e5, 3e6 * P.QE * 4e5, 3e-9, 3e-9, 3e-9]
     d = dn_mod.Denoiser(nc, scales=scales, weights=np.zeros(dn_mod.N_WEIGHTS, np.float32))
-    ctx.denoise_moments(d, 0.5)
+    bound = 0.2
+    f = binomial_filter_periodic(rho0, (16, 16, 16))
+    dev = np.abs(rho0 - f) / np.abs(f)
+    clear = np.abs(dev - bound) > 1e-4
+    assert 0 < (dev > bound).sum() < nc
+    ctx.denoise_moments(d, bound)
     np.testing.assert_array_equal(ctx.field("cellRhoQi"), rho0)
     np.testing.assert_array_equal(ctx.field("cellJi"), ji0)
-    assert ctx.counters()["denoise_fallback"] == 0
+    nfb = ctx.counters()["denoise_fallback"]
+    assert (dev[clear] > bound).sum() <= nfb <= (dev[clear] > bound).sum() + (~clear).sum()
     # random weights, loose bound: the denoiser's own output
     d.set_weights(dn_mod.init_weights(seed=4, gain=0.05))
     vol = np.concatenate([rho0[:, None], ji0.reshape(-1, 3), b0.reshape(-1, 3)], 1).astype(np.float32)
@@ -186,18 +201,35 @@ def test_denoise_moments_in_context(dn_mod, gpu_mod):
     np.testing.assert_allclose(ctx.field("cellRhoQi"), np.maximum(want[:, 0], pr.params.min_rho), rtol=1e-6)
     np.testing.assert_allclose(ctx.field("cellJi").reshape(-1, 3), want[:, 1:], rtol=1e-5,
                                atol=1e-6 * np.abs(want[:, 1:]).max())
-    assert ctx.counters()["denoise_fallback"] == 0
-    # zero bound: every cell whose denoised rho differs at all falls back
+    assert ctx.counters()["denoise_fallback"] == nfb
+    # zero bound: every cell keeps its moments and is counted
     rho1, ji1 = ctx.field("cellRhoQi"), ctx.field("cellJi")
-    vol = np.concatenate([rho1[:, None], ji1.reshape(-1, 3), b0.reshape(-1, 3)], 1).astype(np.float32)
-    want = d.forward(torch.from_numpy(vol).reshape(1, 16, 16, 16, 7).cuda()).cpu().numpy().reshape(-1, 4)
-    changed = want[:, 0] != rho1.astype(np.float32)
-    assert changed.sum() > nc // 2
     ctx.denoise_moments(d, 0.0)
     np.testing.assert_array_equal(ctx.field("cellRhoQi"), rho1)
-    ji2 = ctx.field("cellJi").reshape(-1, 3)
-    np.testing.assert_array_equal(ji2[changed], ji1.reshape(-1, 3)[changed])
-    np.testing.assert_allclose(ji2[~changed], want[~changed, 1:], rtol=1e-5, atol=1e-6 * np.abs(want[:, 1:]).max())
-    assert ctx.counters()["denoise_fallback"] == changed.sum()
+    np.testing.assert_array_equal(ctx.field("cellJi"), ji1)
+    assert ctx.counters()["denoise_fallback"] == nfb + nc
     d.close()
     ctx.close()
+
+
+def test_training_on_moment_pairs_beats_raw_low_ppc(dn_mod):
+    """Pairs from the 32^3 periodic box at 64 ppc (target) and 1/8 subsamples
+    (8 ppc input): after training on four snapshots, the denoised moments of
+    a held-out snapshot are closer to the 64-ppc deposit than the raw 8-ppc
+    deposit is, for n and for J."""
+    from rhybrid_amd import denoise_data as DD
+    from rhybrid_amd import problem as P
+    pr = P.config_periodic(n=32, ppc=64)
+    X, T = DD.make_pairs(pr, subsample=8, snapshots=5, steps_between=2, seed=3)
+    scales = DD.channel_scales(pr)
+    raw = DD.rel_error(X[4:, ..., :4], T[4:], scales)
+    d = dn_mod.Denoiser(4 * 32 ** 3, scales=scales, weights=dn_mod.init_weights(seed=2, gain=0.1))
+    xt, tt = torch.from_numpy(X[:4]).cuda(), torch.from_numpy(T[:4]).cuda()
+    losses = [d.train_step(xt, tt, lr=2e-3) for _ in range(150)]
+    y = d.forward(torch.from_numpy(X[4:]).cuda()).cpu().numpy()
+    d.close()
+    den = DD.rel_error(y, T[4:], scales)
+    print("loss %.4f -> %.4f; held-out n error raw %.4f denoised %.4f; J raw %.4f denoised %.4f"
+          % (losses[0], losses[-1], raw[0], den[0], raw[1], den[1]))
+    assert den[0] < 0.8 * raw[0]
+    assert den[1] < 0.8 * raw[1]
