@@ -293,6 +293,9 @@ def main():
         sys.exit(2)
     res = gpu_bench(args)
     dn = None if args.no_denoiser else denoiser_bench()
+    if dn is not None:
+        from rhybrid_amd import denoise_loop
+        dn["in_loop"] = denoise_loop.compare()
     cpu, err = None, None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)

@@ -63,6 +63,11 @@ int rh_dn_gradient(rh_denoiser* d, const float* x, const float* target, int P, i
 
 int rh_dn_synchronize(rh_denoiser* d);
 
+/* Order the denoiser's later work on `hip_stream` (a hipStream_t; NULL goes
+ * back to the denoiser's own stream), e.g. the stream of a simulation
+ * context that applies it every step. */
+int rh_dn_set_stream(rh_denoiser* d, void* hip_stream);
+
 /* per-kernel device time of the denoiser's launches since the last reset */
 int rh_dn_set_timing(rh_denoiser* d, int on);
 int rh_dn_kernel_time(rh_denoiser* d, const char* name, double* ms, int64_t* launches);

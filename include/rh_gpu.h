@@ -168,6 +168,10 @@ int rh_step(rh_ctx* ctx, int32_t nsteps);
  * least nx*ny*nz voxels. */
 typedef struct rh_denoiser rh_denoiser;
 int rh_denoise_moments(rh_ctx* ctx, rh_denoiser* dn, double max_rel_change);
+/* Attach a denoiser that rh_step applies before every field update (with the
+ * same fallback bound); NULL detaches.  The denoiser's work is then ordered
+ * on the context's stream. */
+int rh_set_denoiser(rh_ctx* ctx, rh_denoiser* dn, double max_rel_change);
 
 /* ---- diagnostics ------------------------------------------------------ */
 

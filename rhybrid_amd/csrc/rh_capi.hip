@@ -97,6 +97,7 @@ public:
     virtual void diagnostics(int s, double* out) = 0;
     virtual void counters(double* out) = 0;
     virtual void denoise(rh_denoiser* dn, double max_rel) = 0;
+    virtual void set_denoiser(rh_denoiser* dn, double max_rel) = 0;
     virtual void synchronize() = 0;
     virtual void set_timing(bool on) = 0;
     virtual void kernel_time(const char* name, double* ms, int64_t* launches) = 0;
@@ -265,6 +266,7 @@ public:
 
     ~Sim() override {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (dn_attached) (void)rh_dn_set_stream(dn_attached, nullptr);  // before the stream goes
         for (auto& t : timers)
             for (auto& pe : t.second.pending) { (void)hipEventDestroy(pe.first); (void)hipEventDestroy(pe.second); }
         for (auto e : event_pool) (void)hipEventDestroy(e);
@@ -646,6 +648,13 @@ public:
     }
 
     void denoise(rh_denoiser* dn, double max_rel) override {
+        denoise_on_stream(dn, max_rel);
+        // a standalone call hands the denoiser its own stream back
+        if (dn != dn_attached && rh_dn_set_stream(dn, nullptr) != RH_DN_OK)
+            throw ArgError(std::string("denoiser: ") + rh_dn_last_error(dn));
+    }
+
+    void denoise_on_stream(rh_denoiser* dn, double max_rel) {
         if (!dn) throw ArgError("null denoiser");
         if (!(max_rel >= 0.0)) throw ArgError("max_rel_change must be >= 0");
         if (g.nx % 16 || g.ny % 4 || g.nz % 4) throw ArgError("denoiser needs nx % 16 == 0, ny % 4 == 0, nz % 4 == 0");
@@ -657,12 +666,21 @@ public:
         const V3<Real> ji = v3(RH_CELL_JI), b = v3(RH_CELL_B);
         launch("dn_gather", k_dn_gather<Real>, cdiv(g.nc, 256), 256, 0, g, (const Real*)fld[RH_CELL_RHOQ].p, ji, b,
                dn_in.p, dn_filt.p);
-        RH_HIP(hipStreamSynchronize(stream));  // the denoiser runs on its own stream
-        if (rh_dn_forward(dn, dn_in.p, dn_out.p, 1, g.nx, g.ny, g.nz) != RH_DN_OK ||
-            rh_dn_synchronize(dn) != RH_DN_OK)
+        // the denoiser's kernels are ordered on this context's stream
+        if (rh_dn_set_stream(dn, (void*)stream) != RH_DN_OK ||
+            rh_dn_forward(dn, dn_in.p, dn_out.p, 1, g.nx, g.ny, g.nz) != RH_DN_OK)
             throw ArgError(std::string("denoiser: ") + rh_dn_last_error(dn));
         launch("dn_scatter", k_dn_scatter<Real>, cdiv(g.nc, 256), 256, 0, g.nc, (const float*)dn_out.p,
                (const float*)dn_filt.p, max_rel, P.min_rho, fld[RH_CELL_RHOQ].p, ji, dcount.p);
+    }
+
+    rh_denoiser* dn_attached = nullptr;
+    double dn_bound = 0.0;
+    void set_denoiser(rh_denoiser* dn, double max_rel) override {
+        if (dn && !(max_rel >= 0.0)) throw ArgError("max_rel_change must be >= 0");
+        if (dn_attached && dn_attached != dn) (void)rh_dn_set_stream(dn_attached, nullptr);
+        dn_attached = dn;
+        dn_bound = max_rel;
     }
 
     void apply_bcs(int s) override {
@@ -742,6 +760,7 @@ public:
             for (size_t s = 0; s < sp.size(); ++s) accumulate((int)s);
             add_remote_updates();
             for (size_t s = 0; s < sp.size(); ++s) apply_bcs((int)s);
+            if (dn_attached) denoise_on_stream(dn_attached, dn_bound);
             propagate_B();
         }
     }
@@ -935,6 +954,9 @@ int rh_kernel_times(rh_ctx* ctx, const char* kernel, double* ms, int64_t* launch
 }
 int rh_reset_timing(rh_ctx* ctx) {
     return guarded(ctx, [&](rh::Ctx& c) { c.reset_timing(); });
+}
+int rh_set_denoiser(rh_ctx* ctx, rh_denoiser* dn, double max_rel_change) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.set_denoiser(dn, max_rel_change); });
 }
 int rh_denoise_moments(rh_ctx* ctx, rh_denoiser* dn, double max_rel_change) {
     return guarded(ctx, [&](rh::Ctx& c) { c.denoise(dn, max_rel_change); });

@@ -443,7 +443,8 @@ unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 }  // namespace
 
 struct rh_denoiser {
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // where the work is ordered
+    hipStream_t own_stream = nullptr;  // the denoiser's own stream
     std::string err;
     int64_t max_voxels = 0;
     float scale[7] = {1, 1, 1, 1, 1, 1, 1};
@@ -651,7 +652,8 @@ int rh_dn_create(int64_t max_voxels, rh_denoiser** out) {
     *out = nullptr;
     rh_denoiser* d = new rh_denoiser();
     const int rc = guarded(d, [&] {
-        DN_HIP(hipStreamCreate(&d->stream));
+        DN_HIP(hipStreamCreate(&d->own_stream));
+        d->stream = d->own_stream;
         d->alloc(max_voxels);
         DN_HIP(hipStreamSynchronize(d->stream));
     });
@@ -667,11 +669,12 @@ int rh_dn_create(int64_t max_voxels, rh_denoiser** out) {
 void rh_dn_destroy(rh_denoiser* d) {
     if (!d) return;
     if (d->stream) (void)hipStreamSynchronize(d->stream);
+    if (d->own_stream) (void)hipStreamSynchronize(d->own_stream);
     for (auto& pe : d->pending) {
         (void)hipEventDestroy(pe.second.first);
         (void)hipEventDestroy(pe.second.second);
     }
-    hipStream_t s = d->stream;
+    hipStream_t s = d->own_stream;
     delete d;
     if (s) (void)hipStreamDestroy(s);
 }
@@ -754,6 +757,17 @@ int rh_dn_synchronize(rh_denoiser* d) {
     return guarded(d, [&] {
         DN_HIP(hipStreamSynchronize(d->stream));
         if (d->timing) d->collect();
+    });
+}
+
+int rh_dn_set_stream(rh_denoiser* d, void* hip_stream) {
+    if (!d) return RH_DN_ERR_ARG;
+    return guarded(d, [&] {
+        hipStream_t ns = hip_stream ? (hipStream_t)hip_stream : d->own_stream;
+        if (ns == d->stream) return;
+        // the work already queued finishes before later work on the new stream starts
+        DN_HIP(hipStreamSynchronize(d->stream));
+        d->stream = ns;
     });
 }
 

@@ -46,6 +46,7 @@ def _lib():
         L.rh_dn_train_step.argtypes = [vp, vp, vp, i32, i32, i32, i32, f32, C.POINTER(f32)]
         L.rh_dn_gradient.argtypes = [vp, vp, vp, i32, i32, i32, i32, vp, C.POINTER(f32)]
         L.rh_dn_synchronize.argtypes = [vp]
+        L.rh_dn_set_stream.argtypes = [vp, vp]
         L.rh_dn_set_timing.argtypes = [vp, i32]
         L.rh_dn_kernel_time.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(i64)]
         L.rh_dn_reset_timing.argtypes = [vp]

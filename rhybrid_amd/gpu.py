@@ -35,7 +35,7 @@ EXPORTS = ["rh_create", "rh_destroy", "rh_last_error", "rh_add_species", "rh_upl
            "rh_clear_accumulation_arrays", "rh_accumulate", "rh_add_remote_updates",
            "rh_apply_boundary_conditions", "rh_propagate_B", "rh_step", "rh_diagnostics",
            "rh_counters", "rh_synchronize", "rh_set_timing", "rh_kernel_times", "rh_reset_timing",
-           "rh_layout_info", "rh_denoise_moments"]
+           "rh_layout_info", "rh_denoise_moments", "rh_set_denoiser"]
 
 
 class GridDesc(C.Structure):
@@ -110,6 +110,7 @@ def load_library(path: str = LIB_PATH):
     L.rh_diagnostics.argtypes = [vp, i32, vp]
     L.rh_counters.argtypes = [vp, vp]
     L.rh_denoise_moments.argtypes = [vp, vp, C.c_double]
+    L.rh_set_denoiser.argtypes = [vp, vp, C.c_double]
     L.rh_kernel_times.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(i64)]
     L.rh_layout_info.argtypes = [vp, vp]
     for name in EXPORTS:
@@ -235,6 +236,12 @@ class Context:
         """Learned denoiser on the deposited moments (no counterpart in the
         reference); call between add_remote_updates and propagate_B."""
         self._check(load_library().rh_denoise_moments(self._h, denoiser._h, float(max_rel_change)))
+
+    def set_denoiser(self, denoiser, max_rel_change: float = 0.5) -> None:
+        """Apply `denoiser` in every step before the field update (None detaches)."""
+        self._denoiser = denoiser  # kept alive while attached
+        h = denoiser._h if denoiser is not None else None
+        self._check(load_library().rh_set_denoiser(self._h, h, float(max_rel_change)))
 
     def counters(self) -> Dict[str, float]:
         out = np.zeros(len(COUNTER_NAMES))

@@ -233,3 +233,20 @@ def test_training_on_moment_pairs_beats_raw_low_ppc(dn_mod):
           % (losses[0], losses[-1], raw[0], den[0], raw[1], den[1]))
     assert den[0] < 0.8 * raw[0]
     assert den[1] < 0.8 * raw[1]
+
+
+def test_denoised_8ppc_in_the_loop_beats_plain_8ppc(dn_mod):
+    """The trained denoiser attached to an 8-ppc FP32 run (applied before
+    every field update): after 20 steps its n and J are closer to the FP64
+    256-ppc run than the plain 8-ppc run's, and within 1.5x of the plain
+    64-ppc run's error; the FP32 256-ppc run from the same particles matches
+    the FP64 one to 1e-5."""
+    from rhybrid_amd import denoise_loop as DL
+    r = DL.compare(nx=64, ny=32, nz=32, steps=20, train_iters=200)
+    runs = r["runs"]
+    print({k: v for k, v in runs.items()})
+    d8, p8, p64 = runs["fp32_8_denoised"], runs["fp32_8"], runs["fp32_64"]
+    assert runs["fp32_256"]["n_error"] < 1e-5 and runs["fp32_256"]["J_error"] < 1e-5
+    assert d8["n_error"] < p8["n_error"] and d8["J_error"] < p8["J_error"]
+    assert d8["n_error"] < 1.5 * p64["n_error"] and d8["J_error"] < 1.5 * p64["J_error"]
+    assert 0.0 <= d8["fallback_fraction"] < 0.5
