@@ -11,7 +11,8 @@
 //    voxel tile (one wave per z plane, four M tiles of one x row each); the
 //    18x6x6 input halo sits in LDS in channels-last order, so a lane's A
 //    fragment (8 channels of one voxel) is one 16-byte LDS read and a wave
-//    reads 1 KB contiguously.  B fragments (weights) are pre-packed in the
+//    reads 1 KB contiguously; each input row's fragment serves the (up to)
+//    three output rows it reaches through the dy taps.  B fragments (weights) are pre-packed in the
 //    lane order of the instruction and read from L1/L2.  The epilogue applies
 //    ReLU (forward), the ReLU mask of the forward activation (backward), or
 //    the FP32 residual (last layer).
@@ -75,20 +76,37 @@ template <int CIN, int NT, int EP>
 __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
     constexpr int CH = CIN / 8;  // 16-byte chunks per voxel
     __shared__ bf16x8 tile[HV * CH];
+    // the first weight group of the 32-channel path, loaded while the halo is staged
+    bf16x8 bnx[3][NT];
+    if constexpr (CIN == 32) {
+        const int l = threadIdx.x & 63;
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) bnx[dy][nt] = a.wf[((dy * 3) * NT + nt) * 64 + l];
+    }
     const int bx = blockIdx.x * TX, by = blockIdx.y * TY;
     const int p = blockIdx.z / a.nzb, bz = (blockIdx.z % a.nzb) * TZ;
     const size_t vol = (size_t)a.X * a.Y * a.Z;
     const bf16* inp = a.in + (size_t)p * vol * CIN;
-    for (int idx = threadIdx.x; idx < HV * CH; idx += blockDim.x) {
+    // halo staging: all of a thread's loads are issued before its LDS stores
+    constexpr int NST = (HV * CH + 255) / 256;
+    bf16x8 stage[NST];
+#pragma unroll
+    for (int q = 0; q < NST; ++q) {
+        const int idx = (int)threadIdx.x + 256 * q;
         const int v = idx / CH, c8 = idx - v * CH;
         const int hx = v % HX, hy = (v / HX) % HY, hz = v / (HX * HY);
         const int gx = bx + hx - 1, gy = by + hy - 1, gz = bz + hz - 1;
-        bf16x8 val;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) val[j] = (bf16)0.0f;
-        if (gx >= 0 && gx < a.X && gy >= 0 && gy < a.Y && gz >= 0 && gz < a.Z)
-            val = *reinterpret_cast<const bf16x8*>(inp + (((size_t)gz * a.Y + gy) * a.X + gx) * CIN + c8 * 8);
-        tile[idx] = val;
+        for (int j = 0; j < 8; ++j) stage[q][j] = (bf16)0.0f;
+        if (idx < HV * CH && gx >= 0 && gx < a.X && gy >= 0 && gy < a.Y && gz >= 0 && gz < a.Z)
+            stage[q] = *reinterpret_cast<const bf16x8*>(inp + (((size_t)gz * a.Y + gy) * a.X + gx) * CIN + c8 * 8);
+    }
+#pragma unroll
+    for (int q = 0; q < NST; ++q) {
+        const int idx = (int)threadIdx.x + 256 * q;
+        if (idx < HV * CH) tile[idx] = stage[q];
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -99,19 +117,37 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[my][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
     if constexpr (CIN == 32) {
-        // fully unrolled, so that the weight loads of later taps issue early
+        // Input row r of the halo (for taps dz, dx) feeds output rows my with
+        // dy = r - my in 0..2: each A fragment is read from LDS once and used
+        // with the weights of its (up to) three dy taps.  The weights of the
+        // next (dz, dx) group are loaded while the current group computes.
 #pragma unroll
-        for (int tap = 0; tap < NTAP; ++tap) {
-            const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
-            bf16x8 b[NT];
+        for (int grp = 0; grp < 9; ++grp) {
+            const int dz = grp / 3, dx = grp % 3;
+            bf16x8 b[3][NT];
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) b[nt] = a.wf[(tap * NT + nt) * 64 + lane];
+            for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-            for (int my = 0; my < TY; ++my) {
-                const int v = ((w + dz) * HY + (my + dy)) * HX + (m + dx);
+                for (int nt = 0; nt < NT; ++nt) b[dy][nt] = bnx[dy][nt];
+            if (grp + 1 < 9) {
+                const int dz1 = (grp + 1) / 3, dx1 = (grp + 1) % 3;
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt)
+                        bnx[dy][nt] = a.wf[((dz1 * 9 + dy * 3 + dx1) * NT + nt) * 64 + lane];
+            }
+#pragma unroll
+            for (int r = 0; r < TY + 2; ++r) {
+                const int v = ((w + dz) * HY + r) * HX + (m + dx);
                 const bf16x8 av = tile[v * CH + kg];
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt) acc[my][nt] = mfma(av, b[nt], acc[my][nt]);
+                for (int my = 0; my < TY; ++my) {
+                    const int dy = r - my;
+                    if (dy < 0 || dy > 2) continue;
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt) acc[my][nt] = mfma(av, b[dy][nt], acc[my][nt]);
+                }
             }
         }
     } else {
