@@ -111,7 +111,8 @@ def gpu_bench(args):
     ctx.set_timing(False)
     np_after = sum(ctx.num_particles(s) for s in range(nspec))
     np_mean = 0.5 * (np_before + np_after)
-    names = ["push_fields", "push", "sort_slots", "sort_count", "scan", "sort_base", "move", "deposit", "fold", "boundary",
+    names = ["push_fields", "push", "sort_slots", "sort_count", "scan", "sort_base", "move", "deposit",
+             "deposit_dense", "fold", "boundary",
              "face2cell", "neumann", "imf", "cell2node", "node_J", "node2cell", "cell_Ue", "node_Ue",
              "upwind_B", "node_E", "node_avg", "face_curl", "cell_Ep", "fill", "diag"]
     ktimes = {}
@@ -120,7 +121,8 @@ def gpu_bench(args):
         if n:
             ktimes[nm] = (ms, n)
     total_k = sum(v[0] for v in ktimes.values())
-    pd_names = ("push_fields", "push", "sort_slots", "sort_count", "scan", "sort_base", "move", "deposit", "fold")
+    pd_names = ("push_fields", "push", "sort_slots", "sort_count", "scan", "sort_base", "move", "deposit",
+                "deposit_dense", "fold")
     t_pd = sum(ktimes[k][0] for k in pd_names if k in ktimes) / 1e3
     dom = max(ktimes, key=lambda k: ktimes[k][0])
     per_kernel = {}

@@ -606,8 +606,15 @@ public:
         if (!S_.d.accumulate) return;
         if (S_.pushed) throw StateError("accumulate between propagate and wait_particle_sends");
         const int NS = g.tc * 8;
+        // tiles with at least 6 particles per subcell on average take the
+        // coalesced eight-lanes-per-subcell variant
+        const uint32_t dense_min = 6u * (uint32_t)NS;
         launch("deposit", k_deposit<Real>, tile_grid(), NS, deposit_lds_bytes<Real>(g), g, S_.A(),
-               (const uint32_t*)S_.sub.p, (const uint16_t*)dep_list.p, (const uint8_t*)dep_count.p, S_.partial.p);
+               (const uint32_t*)S_.sub.p, (const uint16_t*)dep_list.p, (const uint8_t*)dep_count.p, S_.partial.p,
+               dense_min);
+        launch("deposit_dense", k_deposit_dense<Real>, tile_grid(), 512, deposit_dense_lds_bytes<Real>(g), g,
+               S_.A(), (const uint32_t*)S_.sub.p, (const uint16_t*)dep_list.p, (const uint8_t*)dep_count.p,
+               S_.partial.p, dense_min);
         S_.deposited = true;
     }
 

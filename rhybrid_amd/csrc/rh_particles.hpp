@@ -590,7 +590,8 @@ struct alignas(4 * sizeof(Real)) Real4 { Real v[4]; };
 template <typename Real>
 __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const uint32_t* __restrict__ sub,
                                                  const uint16_t* __restrict__ glist,
-                                                 const uint8_t* __restrict__ gcount, Real* __restrict__ partial) {
+                                                 const uint8_t* __restrict__ gcount, Real* __restrict__ partial,
+                                                 uint32_t dense_min) {
     extern __shared__ unsigned char smem_raw[];
     const int NS = g.tc * 8;
     const int OB = deposit_octant_block(g);  // octant block (padded)
@@ -598,6 +599,7 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
     Real* S = reinterpret_cast<Real*>(smem_raw);
     Real* R = S + 16 * QS;  // first-half results, next x 2
     const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
+    if (sub[(size_t)(T + 1) * NS] - sub[(size_t)T * NS] >= dense_min) return;  // k_deposit_dense's tile
     const int ls = threadIdx.x;
     const uint32_t key = (uint32_t)(T * NS + ls);
     const uint32_t p0 = sub[key], p1 = sub[key + 1];
@@ -637,6 +639,72 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
                 out[e] = r;
             }
         }
+    }
+}
+
+// The same deposit for crowded tiles (at least dense_min particles): eight
+// lanes share a subcell and read eight consecutive particles per array (one
+// cache line per group instead of one per lane), then add their sums in a
+// fixed butterfly order; 64 subcells per round, all four quantities staged at
+// once (32 LDS planes).
+template <typename Real>
+RH_HD size_t deposit_dense_lds_bytes(const Geom& g) {
+    return sizeof(Real) * 32 * 8 * (size_t)deposit_octant_block(g);
+}
+
+template <typename Real>
+__global__ __launch_bounds__(512) void k_deposit_dense(Geom g, PArrays<Real> A, const uint32_t* __restrict__ sub,
+                                                       const uint16_t* __restrict__ glist,
+                                                       const uint8_t* __restrict__ gcount,
+                                                       Real* __restrict__ partial, uint32_t dense_min) {
+    extern __shared__ unsigned char smem_raw[];
+    const int NS = g.tc * 8;
+    const int OB = deposit_octant_block(g);
+    const int QS = 8 * OB;
+    Real* S = reinterpret_cast<Real*>(smem_raw);
+    const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
+    if (sub[(size_t)(T + 1) * NS] - sub[(size_t)T * NS] < dense_min) return;  // k_deposit's tile
+    const int tid = threadIdx.x, grp = tid >> 3, gl = tid & 7;
+    for (int base = 0; base < NS; base += 64) {
+        const int ls = base + grp;
+        const bool valid = ls < NS;
+        const uint32_t key = (uint32_t)(T * NS + (valid ? ls : 0));
+        const uint32_t p0 = valid ? sub[key] : 0u, p1 = valid ? sub[key + 1] : 0u;
+        const int o = ls & 7, cell = ls >> 3;
+        const bool hx = o & 1, hy = (o >> 1) & 1, hz = (o >> 2) & 1;
+        Real acc[4][8];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) acc[q][c] = Real(0);
+        for (uint32_t p = p0 + gl; p < p1; p += 8)
+            cic_accumulate(acc, hx, hy, hz, A.x[p], A.y[p], A.z[p], A.vx[p], A.vy[p], A.vz[p]);
+        // butterfly over the group's eight lanes; lane 0's association is fixed
+#pragma unroll
+        for (int off = 1; off < 8; off <<= 1)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int c = 0; c < 8; ++c) acc[q][c] += __shfl_xor(acc[q][c], off);
+        if (valid && gl == 0)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int c = 0; c < 8; ++c) S[(q * 8 + c) * QS + o * OB + cell] = acc[q][c];
+    }
+    __syncthreads();
+    Real4<Real>* out = reinterpret_cast<Real4<Real>*>(partial) + (size_t)T * g.next;
+    for (int e = tid; e < g.next; e += blockDim.x) {
+        const int cnt_e = gcount[e];
+        const uint16_t* lst = glist + (size_t)e * 64;
+        Real4<Real> r;
+        r.v[0] = r.v[1] = r.v[2] = r.v[3] = Real(0);
+        for (int t = 0; t < cnt_e; ++t) {
+            const int off = lst[t];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) r.v[q] += S[q * 8 * QS + off];
+        }
+        out[e] = r;
     }
 }
 

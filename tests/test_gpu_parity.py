@@ -62,8 +62,11 @@ def test_push_parity(gpu_mod, oracle_mod, prec, tol):
 
 
 @pytest.mark.parametrize("prec,tol", [(1, 1e-12), (0, 1e-6)])
-def test_deposit_parity(gpu_mod, oracle_mod, prec, tol):
-    pr = P.config_obstacle(nx=12, ny=8, nz=8, ppc=6, R_cells=2, seed=4)
+@pytest.mark.parametrize("ppc", [6, 96])
+def test_deposit_parity(gpu_mod, oracle_mod, prec, tol, ppc):
+    """6 ppc: one lane per subcell; 96 ppc (12 per subcell): the eight-lanes-
+    per-subcell variant for crowded tiles."""
+    pr = P.config_obstacle(nx=12, ny=8, nz=8, ppc=ppc, R_cells=2, seed=4)
     ctx = make_ctx(gpu_mod, pr, prec)
     ora = oracle_mod.Oracle(pr)
     ctx.deposit()
