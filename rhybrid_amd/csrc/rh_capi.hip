@@ -606,15 +606,13 @@ public:
         if (!S_.d.accumulate) return;
         if (S_.pushed) throw StateError("accumulate between propagate and wait_particle_sends");
         const int NS = g.tc * 8;
-        // tiles with at least 6 particles per subcell on average take the
-        // coalesced eight-lanes-per-subcell variant
-        const uint32_t dense_min = 6u * (uint32_t)NS;
+        // tiles with at least 10 particles per subcell on average take the
+        // coalesced eight-lanes-per-subcell path (measured crossover: at 8 per
+        // subcell the one-lane path is faster, at 16 three times slower)
+        const uint32_t dense_min = 10u * (uint32_t)NS;
         launch("deposit", k_deposit<Real>, tile_grid(), NS, deposit_lds_bytes<Real>(g), g, S_.A(),
                (const uint32_t*)S_.sub.p, (const uint16_t*)dep_list.p, (const uint8_t*)dep_count.p, S_.partial.p,
                dense_min);
-        launch("deposit_dense", k_deposit_dense<Real>, tile_grid(), 512, deposit_dense_lds_bytes<Real>(g), g,
-               S_.A(), (const uint32_t*)S_.sub.p, (const uint16_t*)dep_list.p, (const uint8_t*)dep_count.p,
-               S_.partial.p, dense_min);
         S_.deposited = true;
     }
 
@@ -693,7 +691,7 @@ public:
     void apply_bcs(int s) override {
         Species& S_ = S(s);
         if (S_.pushed) throw StateError("boundary conditions between propagate and wait_particle_sends");
-        launch("boundary", k_boundary<Real>, cdiv(n_bc, 256), 256, 0, g, (const int*)bc_cells.p, n_bc,
+        launch("boundary", k_boundary<Real>, cdiv(n_bc, BC_CELLS_PER_BLOCK), 256, 0, g, (const int*)bc_cells.p, n_bc,
                (const uint32_t*)S_.sub.p, S_.A(), S_.d.R2_obstacle);
     }
 

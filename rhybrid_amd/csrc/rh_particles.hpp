@@ -537,6 +537,24 @@ __device__ __forceinline__ void cic_accumulate(Real (&acc)[4][8], bool hx, bool 
 // (octant stride = 8 banks).
 RH_HD int deposit_octant_block(const Geom& g) { return g.tc + 8; }
 
+// Two of the four quantities (the charge and vx, or vy and vz): a and b
+// are the particle's factors for them (1 and vx, or vy and vz).
+template <typename Real>
+__device__ __forceinline__ void cic_accumulate2(Real (&acc)[2][8], bool hx, bool hy, bool hz, Real X, Real Y,
+                                                Real Z, Real a, Real b) {
+    const Real wx1 = hx ? X - Real(0.5) : X + Real(0.5);
+    const Real wy1 = hy ? Y - Real(0.5) : Y + Real(0.5);
+    const Real wz1 = hz ? Z - Real(0.5) : Z + Real(0.5);
+    const Real live = X >= Real(0) ? Real(1) : Real(0);
+    const Real wx[2] = {live - live * wx1, live * wx1}, wy[2] = {Real(1) - wy1, wy1}, wz[2] = {Real(1) - wz1, wz1};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const Real w = wx[c & 1] * wy[(c >> 1) & 1] * wz[c >> 2];
+        acc[0][c] += w * a;
+        acc[1][c] += w * b;
+    }
+}
+
 // LDS of the deposit: half of the subcell sums (2 quantities x 8 corners) as
 // [quantity*8+corner][octant][cell] (octant blocks of deposit_octant_block
 // words); the first half's results.
@@ -587,6 +605,73 @@ inline int deposit_gather_lists(const Geom& g, uint16_t* list, uint8_t* count) {
 template <typename Real>
 struct alignas(4 * sizeof(Real)) Real4 { Real v[4]; };
 
+// The deposit of a crowded tile (k_deposit sends tiles with at least
+// dense_min particles here): eight lanes share a subcell and read eight
+// consecutive particles per array (one cache line per group instead of one
+// per lane), then add their sums in a fixed butterfly order; blockDim/8
+// subcells per round.  Within the same LDS as the sparse path, the two
+// halves (charge and vx, then vy and vz) each read the particles once.
+template <typename Real>
+__device__ void deposit_dense_tile(const Geom& g, const PArrays<Real>& A, const uint32_t* __restrict__ sub,
+                                   const uint16_t* __restrict__ glist, const uint8_t* __restrict__ gcount,
+                                   Real* __restrict__ partial, int T, Real* S, Real* R) {
+    const int NS = g.tc * 8;
+    const int OB = deposit_octant_block(g);
+    const int QS = 8 * OB;
+    const int tid = threadIdx.x, grp = tid >> 3, gl = tid & 7, ngrp = (int)blockDim.x >> 3;
+    Real4<Real>* out = reinterpret_cast<Real4<Real>*>(partial) + (size_t)T * g.next;
+    for (int half = 0; half < 2; ++half) {
+        if (half) __syncthreads();  // the first half's reads are done
+        for (int base = 0; base < NS; base += ngrp) {
+            const int ls = base + grp;
+            const bool valid = ls < NS;
+            const uint32_t key = (uint32_t)(T * NS + (valid ? ls : 0));
+            const uint32_t p0 = valid ? sub[key] : 0u, p1 = valid ? sub[key + 1] : 0u;
+            const int o = ls & 7, cell = ls >> 3;
+            const bool hx = o & 1, hy = (o >> 1) & 1, hz = (o >> 2) & 1;
+            Real acc[2][8];
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int c = 0; c < 8; ++c) acc[q][c] = Real(0);
+            for (uint32_t p = p0 + gl; p < p1; p += 8) {
+                const Real a = half ? A.vy[p] : Real(1), b = half ? A.vz[p] : A.vx[p];
+                cic_accumulate2(acc, hx, hy, hz, A.x[p], A.y[p], A.z[p], a, b);
+            }
+#pragma unroll
+            for (int off = 1; off < 8; off <<= 1)
+#pragma unroll
+                for (int q = 0; q < 2; ++q)
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) acc[q][c] += __shfl_xor(acc[q][c], off);
+            if (valid && gl == 0)
+#pragma unroll
+                for (int q = 0; q < 2; ++q)
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) S[(q * 8 + c) * QS + o * OB + cell] = acc[q][c];
+        }
+        __syncthreads();
+        for (int e = tid; e < g.next; e += blockDim.x) {
+            const int cnt_e = gcount[e];
+            const uint16_t* lst = glist + (size_t)e * 64;
+            Real res[2] = {Real(0), Real(0)};
+            for (int t = 0; t < cnt_e; ++t) {
+                const int off = lst[t];
+                res[0] += S[off];
+                res[1] += S[8 * QS + off];
+            }
+            if (half == 0) {
+                R[2 * e] = res[0];
+                R[2 * e + 1] = res[1];
+            } else {
+                Real4<Real> r;
+                r.v[0] = R[2 * e]; r.v[1] = R[2 * e + 1]; r.v[2] = res[0]; r.v[3] = res[1];
+                out[e] = r;
+            }
+        }
+    }
+}
+
 template <typename Real>
 __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const uint32_t* __restrict__ sub,
                                                  const uint16_t* __restrict__ glist,
@@ -599,7 +684,10 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
     Real* S = reinterpret_cast<Real*>(smem_raw);
     Real* R = S + 16 * QS;  // first-half results, next x 2
     const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
-    if (sub[(size_t)(T + 1) * NS] - sub[(size_t)T * NS] >= dense_min) return;  // k_deposit_dense's tile
+    if (sub[(size_t)(T + 1) * NS] - sub[(size_t)T * NS] >= dense_min) {
+        deposit_dense_tile(g, A, sub, glist, gcount, partial, T, S, R);
+        return;
+    }
     const int ls = threadIdx.x;
     const uint32_t key = (uint32_t)(T * NS + ls);
     const uint32_t p0 = sub[key], p1 = sub[key + 1];
@@ -639,72 +727,6 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
                 out[e] = r;
             }
         }
-    }
-}
-
-// The same deposit for crowded tiles (at least dense_min particles): eight
-// lanes share a subcell and read eight consecutive particles per array (one
-// cache line per group instead of one per lane), then add their sums in a
-// fixed butterfly order; 64 subcells per round, all four quantities staged at
-// once (32 LDS planes).
-template <typename Real>
-RH_HD size_t deposit_dense_lds_bytes(const Geom& g) {
-    return sizeof(Real) * 32 * 8 * (size_t)deposit_octant_block(g);
-}
-
-template <typename Real>
-__global__ __launch_bounds__(512) void k_deposit_dense(Geom g, PArrays<Real> A, const uint32_t* __restrict__ sub,
-                                                       const uint16_t* __restrict__ glist,
-                                                       const uint8_t* __restrict__ gcount,
-                                                       Real* __restrict__ partial, uint32_t dense_min) {
-    extern __shared__ unsigned char smem_raw[];
-    const int NS = g.tc * 8;
-    const int OB = deposit_octant_block(g);
-    const int QS = 8 * OB;
-    Real* S = reinterpret_cast<Real*>(smem_raw);
-    const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
-    if (sub[(size_t)(T + 1) * NS] - sub[(size_t)T * NS] < dense_min) return;  // k_deposit's tile
-    const int tid = threadIdx.x, grp = tid >> 3, gl = tid & 7;
-    for (int base = 0; base < NS; base += 64) {
-        const int ls = base + grp;
-        const bool valid = ls < NS;
-        const uint32_t key = (uint32_t)(T * NS + (valid ? ls : 0));
-        const uint32_t p0 = valid ? sub[key] : 0u, p1 = valid ? sub[key + 1] : 0u;
-        const int o = ls & 7, cell = ls >> 3;
-        const bool hx = o & 1, hy = (o >> 1) & 1, hz = (o >> 2) & 1;
-        Real acc[4][8];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int c = 0; c < 8; ++c) acc[q][c] = Real(0);
-        for (uint32_t p = p0 + gl; p < p1; p += 8)
-            cic_accumulate(acc, hx, hy, hz, A.x[p], A.y[p], A.z[p], A.vx[p], A.vy[p], A.vz[p]);
-        // butterfly over the group's eight lanes; lane 0's association is fixed
-#pragma unroll
-        for (int off = 1; off < 8; off <<= 1)
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int c = 0; c < 8; ++c) acc[q][c] += __shfl_xor(acc[q][c], off);
-        if (valid && gl == 0)
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int c = 0; c < 8; ++c) S[(q * 8 + c) * QS + o * OB + cell] = acc[q][c];
-    }
-    __syncthreads();
-    Real4<Real>* out = reinterpret_cast<Real4<Real>*>(partial) + (size_t)T * g.next;
-    for (int e = tid; e < g.next; e += blockDim.x) {
-        const int cnt_e = gcount[e];
-        const uint16_t* lst = glist + (size_t)e * 64;
-        Real4<Real> r;
-        r.v[0] = r.v[1] = r.v[2] = r.v[3] = Real(0);
-        for (int t = 0; t < cnt_e; ++t) {
-            const int off = lst[t];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) r.v[q] += S[q * 8 * QS + off];
-        }
-        out[e] = r;
     }
 }
 
@@ -758,12 +780,15 @@ __global__ __launch_bounds__(256) void k_fold(Geom g, FoldSources<Real> src, dou
 // ParticleBoundaryCondHybrid::apply (src/particle_boundary_cond_hybrid.h:
 // 112-185): empty exterior cells, remove particles inside the obstacle in
 // innerFlagParticle cells.  Removed particles get x = -1; the next sort drops
-// them.  One thread per listed cell.
+// them.  One wave per listed cell, its lanes striding through the cell's
+// particles (coalesced at any ppc).
+constexpr int BC_CELLS_PER_BLOCK = 4;
 template <typename Real>
 __global__ __launch_bounds__(256) void k_boundary(Geom g, const int* __restrict__ cells, int ncells,
                                                  const uint32_t* __restrict__ sub, PArrays<Real> A,
                                                  double R2) {
-    const int q = blockIdx.x * 256 + threadIdx.x;
+    const int q = blockIdx.x * BC_CELLS_PER_BLOCK + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (q >= ncells) return;
     const int n = cells[q];
     int i, j, k;
@@ -772,7 +797,7 @@ __global__ __launch_bounds__(256) void k_boundary(Geom g, const int* __restrict_
     const uint32_t p0 = sub[(size_t)kc * 8], p1 = sub[(size_t)kc * 8 + 8];
     const bool exterior = !all_nbrs(g, i, j, k);
     const double xb = g.x0 + i * g.dx, yb = g.y0 + j * g.dx, zb = g.z0 + k * g.dx;
-    for (uint32_t p = p0; p < p1; ++p) {
+    for (uint32_t p = p0 + lane; p < p1; p += 64) {
         if (exterior) { A.x[p] = Real(-1); continue; }
         if (A.x[p] < Real(0)) continue;
         const double x = xb + (double)A.x[p] * g.dx, y = yb + (double)A.y[p] * g.dx, z = zb + (double)A.z[p] * g.dx;
