@@ -46,6 +46,15 @@ class OrcParams(C.Structure):
                 ("use_gravity", C.c_int32), ("GMdt", C.c_double)]
 
 
+class OrcInjector(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("seed", C.c_uint64), ("temperature", C.c_double),
+                ("velocity", C.c_double * 3), ("density", C.c_double), ("ppc", C.c_double),
+                ("cell_rate", C.c_void_p), ("exobase_radius", C.c_double)]
+
+
+INJ_NONE, INJ_SOLAR_WIND, INJ_EXOSPHERE = 0, 1, 2
+
+
 class OrcSpecies(C.Structure):
     _fields_ = [("q", C.c_double), ("m", C.c_double), ("w", C.c_double),
                 ("R2_obstacle", C.c_double), ("accumulate", C.c_int32), ("accelerate", C.c_int32)]
@@ -89,6 +98,15 @@ def lib():
                      "orc_step", "orc_reset_counters"]:
             getattr(L, name).argtypes = [vp]
         L.orc_diagnostics.argtypes = [vp, C.c_int, vp]
+        L.orc_philox4x32_10.argtypes = [vp, vp, vp]
+        L.orc_rng_sample.argtypes = [C.c_uint64, C.c_int, C.c_double, C.c_int64, vp]
+        L.orc_set_injector.argtypes = [vp, C.c_int, C.POINTER(OrcInjector)]
+        L.orc_inject.argtypes = [vp, C.c_int]
+        L.orc_set_timestep.argtypes = [vp, C.c_int64]
+        L.orc_timestep.restype = C.c_int64
+        L.orc_timestep.argtypes = [vp]
+        L.orc_injected.restype = C.c_int64
+        L.orc_injected.argtypes = [vp, C.c_int]
         L.orc_counter.restype = C.c_double
         L.orc_counter.argtypes = [vp, C.c_int]
         L.orc_get_fields.argtypes = [vp, C.c_int64, vp, vp, vp, vp]
@@ -111,6 +129,43 @@ def lib():
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def philox4x32_10(ctr, key) -> np.ndarray:
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox4x32_10(_ptr(c), _ptr(k), _ptr(out))
+    return out
+
+
+RNG_UNIFORM, RNG_GAUSS, RNG_DERIV_GAUSS, RNG_PROB_ROUND = 0, 1, 2, 3
+
+
+def rng_sample(seed: int, which: int, n: int, x0: float = 0.0) -> np.ndarray:
+    """n draws of one generator stream: uniform, sampleGaussian,
+    sampleDerivGaussian(x0) or probRound(x0)."""
+    out = np.empty(n, dtype=np.float64)
+    lib().orc_rng_sample(int(seed), int(which), float(x0), int(n), _ptr(out))
+    return out
+
+
+def injector_struct(inj) -> OrcInjector:
+    """An OrcInjector from a problem.Injector (keeps the rate array alive on
+    the returned object)."""
+    ci = OrcInjector()
+    ci.kind = inj.kind
+    ci.seed = inj.seed
+    ci.temperature = inj.temperature
+    for l in range(3):
+        ci.velocity[l] = inj.velocity[l]
+    ci.density = inj.density
+    ci.ppc = inj.ppc
+    if inj.cell_rate is not None:
+        ci._rate = np.ascontiguousarray(inj.cell_rate, dtype=np.float64)
+        ci.cell_rate = ci._rate.ctypes.data
+    ci.exobase_radius = inj.exobase_radius
+    return ci
 
 
 class Oracle:
@@ -146,6 +201,9 @@ class Oracle:
             self.flag(name)[:] = arr
         for s, parts in enumerate(problem.particles):
             self.set_particles(s, parts.cell, parts.state)
+        for s, inj in enumerate(getattr(problem, "injectors", []) or []):
+            if inj is not None:
+                self.set_injector(s, inj)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -212,6 +270,16 @@ class Oracle:
         B, Ue, Ep = np.zeros(3), np.zeros(3), np.zeros(3)
         lib().orc_get_fields(self._h, int(cell), _ptr(r), _ptr(B), _ptr(Ue), _ptr(Ep))
         return B, Ue, Ep
+
+    def set_injector(self, s: int, inj) -> None:
+        rc = lib().orc_set_injector(self._h, s, C.byref(injector_struct(inj)))
+        if rc != 0:
+            raise ValueError("orc_set_injector failed: %d" % rc)
+
+    def inject(self, s: int) -> None: lib().orc_inject(self._h, s)
+    def set_timestep(self, t: int) -> None: lib().orc_set_timestep(self._h, int(t))
+    def timestep(self) -> int: return int(lib().orc_timestep(self._h))
+    def injected(self, s: int) -> int: return int(lib().orc_injected(self._h, s))
 
     def set_exterior_order_reversed(self, r: bool) -> None:
         lib().orc_set_exterior_order_reversed(self._h, int(r))

@@ -29,6 +29,7 @@
 /* CODATA values; Corsair's constants:: table is not in the reference tree
  * (tools/trace_test_particles.py:47-48 carries the same e and m_p). */
 #define ORC_E    1.60217653e-19
+#define ORC_KB   1.3806505e-23
 #define ORC_MU0  (4.0e-7 * M_PI)
 
 #define MAX_SPECIES 8
@@ -52,6 +53,18 @@ struct orc_sim {
     double counters[ORC_N_COUNTERS];
     double* accbuf;               /* per cell 27 x (rho, Jx, Jy, Jz) */
     int nthreads;
+    int64_t timestep;             /* Simulation::timestep */
+    struct orc_inj {
+        int kind;
+        uint32_t key[2];
+        double vth;
+        int axis, pos_wall;       /* solar wind: normal axis; 1 = the +axis wall */
+        double rate;              /* solar wind: macroparticles per cell per dt */
+        double x0, xm, c, d, cxm; /* sampleDerivGaussian constants for x0 = U/vth */
+        double* cell_rate;        /* exosphere */
+        double R2_exobase;
+        int64_t injected;
+    } inj[MAX_SPECIES];
 };
 
 static const int FIELD_DIM[ORC_N_FIELDS] = {3, 3, 3, 3, 3, 3, 1, 1, 3, 3, 3, 3, 3, 1};
@@ -131,7 +144,7 @@ void orc_destroy(orc_sim* s) {
     if (!s) return;
     for (int f = 0; f < ORC_N_FIELDS; ++f) free(s->f[f]);
     for (int f = 0; f < ORC_N_FLAGS; ++f) free(s->flag[f]);
-    for (int i = 0; i < s->nspecies; ++i) { free(s->pst[i]); free(s->cstart[i]); }
+    for (int i = 0; i < s->nspecies; ++i) { free(s->pst[i]); free(s->cstart[i]); free(s->inj[i].cell_rate); }
     free(s->nflags); free(s->exterior); free(s->accbuf);
     free(s);
 }
@@ -994,16 +1007,263 @@ void orc_propagate_B(orc_sim* s) {
     if (s->p.use_e_pressure) orc_calc_cell_Ep(s);
 }
 
-/* propagate() (src/user.cpp:118-134) without logging, detectors and
- * injection: push all, clear, migrate, accumulate all, finalise, particle
- * boundary conditions, field update. */
+/* ------------------------------------------------------------ injection */
+
+/* Philox4x32-10: ten rounds of two 32x32->64 multiplications with the key
+ * bumped by the Weyl constants between rounds (Salmon et al., SC'11). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0; c1 = (uint32_t)p1; c2 = n2; c3 = (uint32_t)p0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* 53 random bits -> [0,1) */
+static inline double u53(uint32_t a, uint32_t b) {
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+typedef struct {
+    uint32_t key[2], ctr[4];
+    double su, sg;
+    int hu, hg;
+} orc_stream;
+
+static void stream_init(orc_stream* st, const uint32_t key[2], uint32_t c0, uint32_t c1, uint32_t c2) {
+    st->key[0] = key[0]; st->key[1] = key[1];
+    st->ctr[0] = c0; st->ctr[1] = c1; st->ctr[2] = c2; st->ctr[3] = 0;
+    st->hu = st->hg = 0;
+}
+
+static double stream_uniform(orc_stream* st) {
+    if (st->hu) { st->hu = 0; return st->su; }
+    uint32_t o[4];
+    orc_philox4x32_10(st->ctr, st->key, o);
+    st->ctr[3]++;
+    st->su = u53(o[2], o[3]);
+    st->hu = 1;
+    return u53(o[0], o[1]);
+}
+
+/* sampleGaussian (src/particle_injector.cpp:59-80): polar method, the second
+ * deviate kept for the next call (per stream here).  r2 == 0 is also
+ * rejected (log(0)). */
+static double stream_gauss(orc_stream* st) {
+    if (st->hg) { st->hg = 0; return st->sg; }
+    double x, y, r2;
+    do {
+        x = 2 * stream_uniform(st) - 1;
+        y = 2 * stream_uniform(st) - 1;
+        r2 = x * x + y * y;
+    } while (r2 >= 1.0 || r2 == 0.0);
+    const double fac = sqrt(-2.0 * log(r2) / r2);
+    st->sg = y * fac;
+    st->hg = 1;
+    return x * fac;
+}
+
+/* constants of sampleDerivGaussian (src/particle_injector.cpp:97-106) */
+static void deriv_gauss_consts(double x0, double* xm, double* c, double* d, double* cxm) {
+    *xm = 0.5 * (x0 + sqrt(x0 * x0 + 4.0));
+    *c = 1.0 / (exp(-0.5 * x0 * x0) + x0 * sqrt(0.5 * M_PI) * erfc(-x0 * (1.0 / sqrt(2.0))));
+    *d = sqr(x0 - *xm);
+    *cxm = *c * *xm;
+}
+
+/* sampleDerivGaussian (src/particle_injector.cpp:97-114): f(x) ~ max(0,x)
+ * exp(-(x-x0)^2/2) by rejection from the unit Gaussian at its mode xm. */
+static double stream_deriv_gauss(orc_stream* st, double x0, double xm, double c, double d, double cxm) {
+    for (;;) {
+        const double x = xm + stream_gauss(st);
+        if (x < 0) continue;
+        const double majorant = cxm * exp(-0.5 * (sqr(x - xm) + d));
+        const double pdf = c * x * exp(-0.5 * sqr(x - x0));
+        if (stream_uniform(st) * majorant > pdf) continue;
+        return x;
+    }
+}
+
+/* probRound (src/particle_injector.cpp:41-47) */
+static int prob_round(double x, double r) {
+    if (x <= 0) return 0;
+    const int f = (int)floor(x);
+    return (r < x - f) ? f + 1 : f;
+}
+
+void orc_rng_sample(uint64_t seed, int which, double x0, int64_t n, double* out) {
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    orc_stream st;
+    stream_init(&st, key, 0, 0, 0);
+    double xm = 0, c = 0, d = 0, cxm = 0;
+    if (which == 2) deriv_gauss_consts(x0, &xm, &c, &d, &cxm);
+    for (int64_t q = 0; q < n; ++q) {
+        switch (which) {
+        case 0: out[q] = stream_uniform(&st); break;
+        case 1: out[q] = stream_gauss(&st); break;
+        case 2: out[q] = stream_deriv_gauss(&st, x0, xm, c, d, cxm); break;
+        default: out[q] = prob_round(x0, stream_uniform(&st)); break;
+        }
+    }
+}
+
+int orc_set_injector(orc_sim* s, int sp, const orc_injector* in) {
+    if (sp < 0 || sp >= s->nspecies) return -1;
+    struct orc_inj* I = &s->inj[sp];
+    free(I->cell_rate);
+    memset(I, 0, sizeof(*I));
+    if (!in || in->kind == ORC_INJ_NONE) return 0;
+    I->key[0] = (uint32_t)in->seed;
+    I->key[1] = (uint32_t)(in->seed >> 32);
+    if (!(in->temperature > 0)) return -2;
+    I->vth = sqrt(ORC_KB * in->temperature / s->sp[sp].m);
+    if (in->kind == ORC_INJ_SOLAR_WIND) {
+        /* InjectorSolarWind::initialize (src/particle_injector.cpp:588-682) */
+        int nz = 0, axis = -1;
+        for (int l = 0; l < 3; ++l)
+            if (in->velocity[l] != 0.0) { ++nz; axis = l; }
+        if (nz != 1 || !(in->ppc > 0) || !(in->density > 0)) return -2;
+        const double U = fabs(in->velocity[axis]);
+        I->axis = axis;
+        I->pos_wall = in->velocity[axis] < 0;
+        I->rate = in->ppc * U * s->p.dt / s->g.dx;
+        I->x0 = U / I->vth;
+        deriv_gauss_consts(I->x0, &I->xm, &I->c, &I->d, &I->cxm);
+    } else if (in->kind == ORC_INJ_EXOSPHERE) {
+        if (!in->cell_rate) return -2;
+        I->cell_rate = (double*)malloc(sizeof(double) * s->nc);
+        memcpy(I->cell_rate, in->cell_rate, sizeof(double) * s->nc);
+        I->R2_exobase = in->exobase_radius > 0 ? sqr(in->exobase_radius) : 0.0;
+    } else {
+        return -2;
+    }
+    I->kind = in->kind;
+    return 0;
+}
+
+/* checkIfInjectionCell{X,Y,Z}{Pos,Neg} (src/particle_injector.cpp:392-454):
+ * the wall neighbour is missing, the four transverse neighbours exist. */
+static int sw_injection_cell(const orc_sim* s, const struct orc_inj* I, int n) {
+    static const uint32_t pos[3] = {X_POS, Y_POS, Z_POS}, neg[3] = {X_NEG, Y_NEG, Z_NEG};
+    const uint32_t nf = s->nflags[n];
+    if (nf & (I->pos_wall ? pos[I->axis] : neg[I->axis])) return 0;
+    for (int l = 0; l < 3; ++l) {
+        if (l == I->axis) continue;
+        if (!(nf & pos[l]) || !(nf & neg[l])) return 0;
+    }
+    return 1;
+}
+
+/* One injection cell's new particles (cell-local metres, state[7]) written
+ * to out; returns their number.  Stream of the cell's count: particle index
+ * 0xFFFFFFFF; stream of trial particle m: index m. */
+static int inject_cell(const orc_sim* s, int sp, int n, uint32_t step, double* out, int max_out) {
+    const struct orc_inj* I = &s->inj[sp];
+    const double dx = s->g.dx;
+    orc_stream st;
+    stream_init(&st, I->key, (uint32_t)n, step, 0xFFFFFFFFu);
+    const double rate = I->kind == ORC_INJ_SOLAR_WIND ? I->rate : I->cell_rate[n];
+    const int N = prob_round(rate, stream_uniform(&st));
+    int q = 0;
+    for (int m = 0; m < N; ++m) {
+        stream_init(&st, I->key, (uint32_t)n, step, (uint32_t)m);
+        double r[3], v[3];
+        if (I->kind == ORC_INJ_SOLAR_WIND) {
+            /* initParticleCrdVel{X,Y,Z}{Pos,Neg} (src/particle_injector.cpp:464-535):
+             * on the wall face, uniform across it; flux-weighted normal speed */
+            const int a = I->axis, t1 = (a + 1) % 3, t2 = (a + 2) % 3;
+            const int l1 = t1 < t2 ? t1 : t2, l2 = t1 < t2 ? t2 : t1;
+            r[a] = I->pos_wall ? 0.0 : dx;
+            r[l1] = stream_uniform(&st) * dx;
+            r[l2] = stream_uniform(&st) * dx;
+            const double vn = I->vth * stream_deriv_gauss(&st, I->x0, I->xm, I->c, I->d, I->cxm);
+            v[a] = I->pos_wall ? -vn : vn;
+            v[l1] = I->vth * stream_gauss(&st);
+            v[l2] = I->vth * stream_gauss(&st);
+        } else {
+            /* InjectorExosphere::injectParticles (src/particle_injector.cpp:1522-1591):
+             * uniform in the cell shrunk by epsCell, discarded below the exobase */
+            int i, j, k;
+            decompose(&s->g, n, &i, &j, &k);
+            const double eps = 1.0e-4;
+            for (int l = 0; l < 3; ++l) r[l] = 0.5 * dx + (1.0 - eps) * dx * (stream_uniform(&st) - 0.5);
+            const double gx = (s->g.x0 + i * dx) + r[0], gy = (s->g.y0 + j * dx) + r[1],
+                         gz = (s->g.z0 + k * dx) + r[2];
+            if (gx * gx + gy * gy + gz * gz < I->R2_exobase) continue;
+            for (int l = 0; l < 3; ++l) v[l] = I->vth * stream_gauss(&st);
+        }
+        if (q < max_out) {
+            double* o = out + 7 * q;
+            o[0] = r[0]; o[1] = r[1]; o[2] = r[2];
+            o[3] = v[0]; o[4] = v[1]; o[5] = v[2];
+            o[6] = s->sp[sp].w;
+        }
+        ++q;
+    }
+    return q;
+}
+
+/* ParticleInjectorBase::inject: every listed cell, in cell order; the new
+ * particles go to the end of their cell's list. */
+void orc_inject(orc_sim* s, int sp) {
+    if (sp < 0 || sp >= s->nspecies) return;
+    struct orc_inj* I = &s->inj[sp];
+    if (I->kind == ORC_INJ_NONE || s->timestep <= 0) return;
+    const uint32_t step = (uint32_t)s->timestep;
+    const int nc = s->nc;
+    int* cnt = (int*)calloc((size_t)nc, sizeof(int));
+    int64_t total = 0, cap = 0;
+    for (int n = 0; n < nc; ++n) {
+        if (I->kind == ORC_INJ_SOLAR_WIND ? !sw_injection_cell(s, I, n) : !(I->cell_rate[n] > 0)) continue;
+        cnt[n] = inject_cell(s, sp, n, step, NULL, 0);
+        total += cnt[n];
+        if (cnt[n] > cap) cap = cnt[n];
+    }
+    if (total == 0) { free(cnt); return; }
+    const int64_t np = s->np[sp];
+    double* nst = (double*)malloc(sizeof(double) * 7 * (size_t)(np + total));
+    double* tmp = (double*)malloc(sizeof(double) * 7 * (size_t)cap);
+    int64_t* cs = s->cstart[sp];
+    int64_t out = 0;
+    for (int n = 0; n < nc; ++n) {
+        const int64_t b = cs[n], e = cs[n + 1];
+        cs[n] = out;
+        if (e > b) memcpy(nst + 7 * out, s->pst[sp] + 7 * b, sizeof(double) * 7 * (size_t)(e - b));
+        out += e - b;
+        if (cnt[n]) {
+            inject_cell(s, sp, n, step, tmp, cnt[n]);
+            memcpy(nst + 7 * out, tmp, sizeof(double) * 7 * (size_t)cnt[n]);
+            out += cnt[n];
+        }
+    }
+    cs[nc] = out;
+    free(s->pst[sp]);
+    s->pst[sp] = nst;
+    s->np[sp] = out;
+    I->injected += total;
+    free(tmp);
+    free(cnt);
+}
+
+void orc_set_timestep(orc_sim* s, int64_t t) { s->timestep = t; }
+int64_t orc_timestep(const orc_sim* s) { return s->timestep; }
+int64_t orc_injected(const orc_sim* s, int sp) { return (sp >= 0 && sp < s->nspecies) ? s->inj[sp].injected : -1; }
+
+/* propagate() (src/user.cpp:118-134) without logging and detectors: the
+ * timestep advances, then push all, clear, migrate, accumulate all,
+ * finalise, particle boundary conditions, injection, field update. */
 void orc_step(orc_sim* s) {
+    s->timestep++;
     for (int sp = 0; sp < s->nspecies; ++sp) orc_push(s, sp);
     orc_clear_accumulation(s);
     for (int sp = 0; sp < s->nspecies; ++sp) orc_migrate(s, sp);
     for (int sp = 0; sp < s->nspecies; ++sp) orc_accumulate(s, sp);
     orc_add_remote_updates(s);
     for (int sp = 0; sp < s->nspecies; ++sp) orc_apply_boundary_conditions(s, sp);
+    for (int sp = 0; sp < s->nspecies; ++sp) orc_inject(s, sp);
     orc_propagate_B(s);
 }
 

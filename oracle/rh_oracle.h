@@ -15,6 +15,8 @@
  *   particle BCs             src/particle_boundary_cond_hybrid.h:112-185
  *   step order               src/user.cpp:118-134
  *   conservation sums        src/diagnostics.h:209-220
+ *   solar-wind injection     src/particle_injector.cpp:41-114,380-700
+ *   exosphere injection      src/particle_injector.cpp:1489-1840
  * Cell-to-cell particle migration lives in the un-vendored Corsair
  * ParticleListSkeleton; it is re-specified here (see orc_migrate).
  */
@@ -110,7 +112,8 @@ void orc_accumulate(orc_sim* s, int species);           /* Accumulator::accumula
 void orc_add_remote_updates(orc_sim* s);                /* Accumulator::addRemoteUpdates */
 void orc_apply_boundary_conditions(orc_sim* s, int species); /* ParticleBoundaryCondHybrid::apply */
 void orc_propagate_B(orc_sim* s);                       /* propagateB */
-void orc_step(orc_sim* s);                              /* propagate(), src/user.cpp:118-134 */
+void orc_step(orc_sim* s);                              /* propagate(), src/user.cpp:118-134;
+                                                           advances the timestep first */
 
 /* Sub-steps of propagateB exposed for unit checks. */
 void orc_face2cell(orc_sim* s, const double* face, double* cell);
@@ -129,6 +132,38 @@ void orc_upwind_node_B(orc_sim* s);
 void orc_calc_node_E(orc_sim* s);
 void orc_node_avg(orc_sim* s, const double* oldE, double* E);
 void orc_calc_cell_Ep(orc_sim* s);
+
+/* ---- injection (src/particle_injector.cpp) ----------------------------
+ *
+ * The reference draws from Corsair's sequential generator, which is not in
+ * the reference tree; the restatement draws from Philox4x32-10 (Salmon,
+ * Moraes, Dror and Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+ * SC'11) with key = seed and counter = (cell, timestep, particle, block), so
+ * that any order of evaluation gives the same particles.  Two doubles of 53
+ * random bits per generator call. */
+enum { ORC_INJ_NONE = 0, ORC_INJ_SOLAR_WIND = 1, ORC_INJ_EXOSPHERE = 2 };
+typedef struct orc_injector {
+    int32_t kind;
+    uint64_t seed;
+    double temperature;        /* vth = sqrt(kB T / m) */
+    double velocity[3];        /* solar wind: bulk velocity, one non-zero component */
+    double density;            /* solar wind: n [m^-3] */
+    double ppc;                /* solar wind: macroparticles per cell */
+    const double* cell_rate;   /* exosphere: nc values, macroparticles per cell per dt (copied) */
+    double exobase_radius;     /* exosphere: trial positions below it are discarded */
+} orc_injector;
+
+void    orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* n draws from one stream (counter (0,0,0,block)): which = 0 uniform [0,1),
+ * 1 sampleGaussian, 2 sampleDerivGaussian(x0), 3 probRound(x0). */
+void    orc_rng_sample(uint64_t seed, int which, double x0, int64_t n, double* out);
+int     orc_set_injector(orc_sim* s, int species, const orc_injector* inj);
+/* ParticleInjectorBase::inject for the current timestep (nothing at <= 0). */
+void    orc_inject(orc_sim* s, int species);
+void    orc_set_timestep(orc_sim* s, int64_t t);
+int64_t orc_timestep(const orc_sim* s);
+/* macroparticles injected so far per species */
+int64_t orc_injected(const orc_sim* s, int species);
 
 /* Field gather for one point (cell-local metres) as the push sees it. */
 void orc_get_fields(orc_sim* s, int64_t cell, const double r[3], double B[3], double Ue[3], double Ep[3]);

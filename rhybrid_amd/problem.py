@@ -118,6 +118,25 @@ class Particles:
         return int(self.cell.shape[0])
 
 
+INJ_NONE, INJ_SOLAR_WIND, INJ_EXOSPHERE = 0, 1, 2
+
+
+@dataclass
+class Injector:
+    """A species' injector (src/particle_injector.cpp), as the C ABI's
+    rh_injector_desc: the solar-wind inflow through one outer wall
+    (InjectorSolarWind, :380-700) or photoions from a per-cell rate table
+    (InjectorExosphere, :1489-1840)."""
+    kind: int
+    seed: int
+    temperature: float
+    velocity: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    density: float = 0.0
+    ppc: float = 0.0
+    cell_rate: Optional[np.ndarray] = None   # exosphere: macroparticles per cell per dt
+    exobase_radius: float = 0.0
+
+
 @dataclass
 class Problem:
     name: str
@@ -129,6 +148,7 @@ class Problem:
     particles: List[Particles]
     steps: int = 1
     meta: Dict[str, object] = field(default_factory=dict)
+    injectors: List[Optional[Injector]] = field(default_factory=list)
 
     def copy(self) -> "Problem":
         return dataclasses.replace(
@@ -136,7 +156,7 @@ class Problem:
             fields={k: v.copy() for k, v in self.fields.items()},
             flags={k: v.copy() for k, v in self.flags.items()},
             particles=[Particles(p.cell.copy(), p.state.copy()) for p in self.particles],
-            meta=dict(self.meta))
+            meta=dict(self.meta), injectors=list(self.injectors))
 
     @property
     def n_particles(self) -> int:
@@ -249,14 +269,16 @@ def config_periodic(n: int = 32, ppc: int = 64, steps: int = 100, seed: int = 64
 def config_obstacle(nx: int = 128, ny: int = 96, nz: int = 96, ppc: int = 32,
                     steps: int = 500, seed: int = 6461674, dx: float = 300e3,
                     dt: float = 0.04, R_cells: float = 10.0, he_fraction: float = 0.04,
-                    n_sw: float = 3e6) -> Problem:
+                    n_sw: float = 3e6, inflow: bool = False) -> Problem:
     """Inflow/outflow box with an absorbing spherical obstacle of radius
     R_cells cells at the centre; H+ and He++ (he_fraction by number) solar
     wind, ppc each, drifting Maxwellians at 450 km/s, T=1e5 K, IMF
     B=(-1.5,2.5,0) nT, IMF on the +x wall (as examples/mars.cfg [IMF]).
     Choices the workload description leaves open: n_sw=3 cm^-3 total,
     dt=0.04 s, isothermal electrons at Te=1e5 K, eta=0, uniform preload of
-    the interior (the reference's InjectorUniform) and no further inflow."""
+    the interior (the reference's InjectorUniform); with inflow, both
+    species also enter through the +x wall every step (InjectorSolarWind,
+    as examples/anomaly.cfg pairs a uniform and a solar-wind population)."""
     grid = Grid(nx, ny, nz, (0, 0, 0), dx, -0.5 * nx * dx, -0.5 * ny * dx, -0.5 * nz * dx)
     T, Te = 1e5, 1e5
     U = (-450e3, 0.0, 0.0)
@@ -274,9 +296,57 @@ def config_obstacle(nx: int = 128, ny: int = 96, nz: int = 96, ppc: int = 32,
     flags = make_flags(grid, R_field=R, R_particle=R)
     pH = uniform_particles(grid, spH, ppc, U, T, rng, R_exclude=R)
     pHe = uniform_particles(grid, spHe, ppc, U, T, rng, R_exclude=R)
-    return Problem("obstacle%dx%dx%d_ppc%d" % (nx, ny, nz, ppc), grid, p, [spH, spHe],
-                   make_fields(grid, B0), flags, [pH, pHe], steps=steps,
-                   meta={"ppc": ppc, "density": n_sw, "U": U, "T": T, "R_cells": R_cells})
+    injectors = []
+    if inflow:
+        injectors = [solar_wind_injector(U, nH, T, ppc, seed=seed + 1),
+                     solar_wind_injector(U, nHe, T, ppc, seed=seed + 2)]
+    return Problem("obstacle%dx%dx%d_ppc%d%s" % (nx, ny, nz, ppc, "_inflow" if inflow else ""), grid, p,
+                   [spH, spHe], make_fields(grid, B0), flags, [pH, pHe], steps=steps,
+                   meta={"ppc": ppc, "density": n_sw, "U": U, "T": T, "R_cells": R_cells, "inflow": inflow},
+                   injectors=injectors)
+
+
+def solar_wind_injector(velocity, density: float, T: float, ppc: float, seed: int) -> Injector:
+    """InjectorSolarWind: inflow through the wall the bulk velocity points
+    away from; the species weight must be density*dV/ppc."""
+    return Injector(INJ_SOLAR_WIND, seed, T, tuple(float(v) for v in velocity), density, float(ppc))
+
+
+def exosphere_neutrals(grid: Grid, n0: float, r0: float, H: float, R_exobase: float,
+                       sub: int = 4) -> np.ndarray:
+    """Neutrals per cell (cell volume times the mean density) of the
+    Exponential profile n0*exp(-(r-r0)/H) (src/neutral_profiles.h:57-66), zero
+    below the exobase.  The reference averages 1000 random points per cell
+    (src/particle_injector.cpp:1690-1712); this uses the midpoint rule on
+    sub^3 points of the same shrunk cell, a deterministic estimate of the
+    same integral."""
+    i, j, k = grid.ijk()
+    dx = grid.dx
+    xc, yc, zc = grid.x0 + (i + 0.5) * dx, grid.y0 + (j + 0.5) * dx, grid.z0 + (k + 0.5) * dx
+    offs = ((np.arange(sub) + 0.5) / sub - 0.5) * (1.0 - EPS_CELL) * dx
+    acc = np.zeros(grid.nc)
+    for ox in offs:
+        for oy in offs:
+            for oz in offs:
+                r = np.sqrt((xc + ox) ** 2 + (yc + oy) ** 2 + (zc + oz) ** 2)
+                acc += np.where(r * r < R_exobase * R_exobase, 0.0, n0 * np.exp(-(r - r0) / H))
+    return acc / sub ** 3 * grid.dV
+
+
+def exosphere_injector(grid: Grid, neutrals: np.ndarray, T: float, macro_per_dt: float,
+                       R_exobase: float, seed: int, R_shadow: float = 0.0,
+                       shadow_factor: float = 0.0) -> Injector:
+    """InjectorExosphere's rate table (src/particle_injector.cpp:1726-1792):
+    macro_per_dt macroparticles per step spread over the cells in proportion
+    to their neutrals times the ionisation factor (shadow_factor in the
+    shadow cylinder behind the object, x < 0).  The species weight is then
+    total_rate*dt/macro_per_dt with total_rate = ionisation rate * sum(neutrals)."""
+    i, j, k = grid.ijk()
+    dx = grid.dx
+    xc, yc, zc = grid.x0 + (i + 0.5) * dx, grid.y0 + (j + 0.5) * dx, grid.z0 + (k + 0.5) * dx
+    factor = np.where((xc < 0) & (yc * yc + zc * zc < R_shadow * R_shadow), shadow_factor, 1.0)
+    rate = macro_per_dt * neutrals * factor / neutrals.sum()
+    return Injector(INJ_EXOSPHERE, seed, T, cell_rate=rate, exobase_radius=R_exobase)
 
 
 def perturb(problem: Problem, seed: int = 1, scale: float = 0.2) -> Problem:
