@@ -151,8 +151,57 @@ int rh_add_remote_updates(rh_ctx* ctx);
 int rh_apply_boundary_conditions(rh_ctx* ctx, int32_t species);
 /* propagateB (src/hybrid_propagator.cpp:42-440). */
 int rh_propagate_B(rh_ctx* ctx);
-/* propagate() (src/user.cpp:118-134) without logging or injection, nsteps times. */
+/* propagate() (src/user.cpp:118-134) without logging, nsteps times: each
+ * step advances the timestep by one, then push, migration, deposit, particle
+ * boundary conditions, injection and field update.  Synchronises at the end;
+ * returns RH_ERR_CAPACITY when a particle store overflowed in these steps. */
 int rh_step(rh_ctx* ctx, int32_t nsteps);
+
+/* ---- injection (src/particle_injector.cpp) ----------------------------- */
+
+/* Injector kinds: the solar-wind inflow through one outer wall
+ * (InjectorSolarWind, src/particle_injector.cpp:380-700) and photoions from a
+ * per-cell rate table (InjectorExosphere, :1489-1840). */
+typedef enum rh_injector_kind { RH_INJ_NONE = 0, RH_INJ_SOLAR_WIND = 1, RH_INJ_EXOSPHERE = 2 } rh_injector_kind;
+
+/* Random numbers: Philox4x32-10 with key = seed and counter = (cell,
+ * timestep, particle, block), so a step's new particles do not depend on
+ * the order of evaluation (Corsair's sequential generator, which the
+ * reference uses, is not reproduced).  Temperatures give vth = sqrt(kB T/m)
+ * with kB = 1.3806505e-23. */
+typedef struct rh_injector_desc {
+    int32_t kind;              /* rh_injector_kind */
+    uint64_t seed;
+    double temperature;        /* [K] */
+    /* RH_INJ_SOLAR_WIND: inflow through the wall the bulk velocity points
+     * away from (velocity (U,0,0), U < 0: the +x wall), into the wall's
+     * non-edge exterior cells, probRound(ppc*|U|*dt/dx) particles per cell per
+     * step on the wall face with a flux-weighted normal speed
+     * (src/particle_injector.cpp:392-577,681).  The species weight must be
+     * density*dx^3/ppc (:682). */
+    double velocity[3];        /* [m/s], exactly one non-zero component */
+    double density;            /* [m^-3] */
+    double ppc;                /* macroparticles per cell of the undisturbed wind */
+    /* RH_INJ_EXOSPHERE: probRound(cell_rate[n]) trial particles per cell per
+     * step, uniform in the cell shrunk by epsCell = 1e-4, discarded below the
+     * exobase, Maxwellian velocities (:1522-1591).  cell_rate (nc values,
+     * copied) is cellExosphere after its normalisation (:1772-1781). */
+    const double* cell_rate;   /* macroparticles per cell per dt */
+    double exobase_radius;     /* [m] */
+} rh_injector_desc;
+
+/* Attach (or with NULL / RH_INJ_NONE detach) the injector of a species
+ * (ParticleInjectorBase::initialize). */
+int rh_set_injector(rh_ctx* ctx, int32_t species, const rh_injector_desc* desc);
+/* ParticleInjectorBase::inject (src/particle_injector.cpp:540-551) for the
+ * current timestep; nothing at timestep <= 0.  The new particles are pushed
+ * and deposited from the next step on, as in the reference's step order. */
+int rh_inject_particles(rh_ctx* ctx, int32_t species);
+/* Simulation::timestep, the injectors' counter (0 after rh_create). */
+int rh_set_timestep(rh_ctx* ctx, int64_t timestep);
+int rh_get_timestep(rh_ctx* ctx, int64_t* timestep);
+/* macroparticles injected so far (Hybrid::logCounterParticleInjectMacroparticles) */
+int rh_injected(rh_ctx* ctx, int32_t species, int64_t* n);
 
 /* ---- learned moment denoiser (include/rh_denoise.h) -------------------- */
 

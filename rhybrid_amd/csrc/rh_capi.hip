@@ -17,6 +17,7 @@
 #include "../../include/rh_denoise.h"
 #include "rh_fields.hpp"
 #include "rh_geom.hpp"
+#include "rh_inject.hpp"
 #include "rh_particles.hpp"
 
 namespace rh {
@@ -103,7 +104,25 @@ public:
     virtual void kernel_time(const char* name, double* ms, int64_t* launches) = 0;
     virtual void reset_timing() = 0;
     virtual void layout(int64_t* out) = 0;
+    virtual void set_injector(int s, const rh_injector_desc* d) = 0;
+    virtual void inject(int s) = 0;
+    virtual void set_timestep(int64_t t) = 0;
+    virtual int64_t timestep() = 0;
+    virtual int64_t injected(int s) = 0;
 };
+
+// Boltzmann constant of vth = sqrt(kB T / m) (CODATA 2002, the same table as
+// the e and m_p of tools/trace_test_particles.py:47-48).
+constexpr double KB = 1.3806505e-23;
+
+// Constants of sampleDerivGaussian (src/particle_injector.cpp:97-106) for x0.
+static void deriv_gauss_consts(double x0, double& xm, double& c, double& d, double& cxm) {
+#pragma clang fp contract(off)
+    xm = 0.5 * (x0 + std::sqrt(x0 * x0 + 4.0));
+    c = 1.0 / (std::exp(-0.5 * x0 * x0) + x0 * std::sqrt(0.5 * M_PI) * std::erfc(-x0 * (1.0 / std::sqrt(2.0))));
+    d = (x0 - xm) * (x0 - xm);
+    cxm = c * xm;
+}
 
 // Moments <-> denoiser volume [Z][Y][X][7] (n, Ji, cellB) and back; the
 // cell index n = i + nx*(j + ny*k) is the voxel index of the volume.  The
@@ -172,6 +191,19 @@ public:
         bool deposited = false;
         bool pushed = false;  // pass 0 done, pass 1 pending
         PushParams pp{};      // parameters of the pending push
+        // injector: listed cells in subcell-key order, their rates
+        // (exosphere), per tile the first listed cell, per listed cell the
+        // offset of its new particles past the store's end
+        InjParams inj{};
+        int inj_ncell = 0;
+        DevBuf<int> inj_cells;
+        DevBuf<double> inj_rate;
+        DevBuf<uint32_t> inj_cbeg, inj_off;
+        DevBuf<unsigned long long> inj_total;
+        bool inj_pending = false;  // injected particles wait past the store's end
+        InjAppendix appendix() const {
+            return inj_pending ? InjAppendix{inj_cbeg.p, inj_off.p} : InjAppendix{nullptr, nullptr};
+        }
         PArrays<Real> A() { return {a[0].p, a[1].p, a[2].p, a[3].p, a[4].p, a[5].p, a_lc.p}; }
         PArrays<Real> Dst() { return {dst[0].p, dst[1].p, dst[2].p, dst[3].p, dst[4].p, dst[5].p, dst_lc.p}; }
         void swap_stores() {
@@ -499,10 +531,12 @@ public:
         RH_HIP(hipMemcpyAsync(S_.sub.p, sub.data(), sizeof(uint32_t) * sub.size(), hipMemcpyHostToDevice, stream));
         RH_HIP(hipStreamSynchronize(stream));
         S_.pushed = false;
+        S_.inj_pending = false;
     }
 
     // live particles (those removed by the boundary conditions stay stored,
-    // marked, until the next sort)
+    // marked, until the next sort; injected ones waiting for their first
+    // push are included)
     int64_t num_particles(int s) override {
         double t[5];
         diag_raw(s, t);
@@ -512,6 +546,7 @@ public:
     void download_particles(int s, int64_t capn, int64_t* cell, double* st) override {
         Species& S_ = S(s);
         if (S_.pushed) throw StateError("download between propagate and wait_particle_sends");
+        flush_injected(S_);
         std::vector<uint32_t> sub((size_t)g.nkeys + 1);
         RH_HIP(hipMemcpyAsync(sub.data(), S_.sub.p, sizeof(uint32_t) * sub.size(), hipMemcpyDeviceToHost, stream));
         RH_HIP(hipStreamSynchronize(stream));
@@ -561,13 +596,30 @@ public:
         pp.use_e_pressure = P.use_e_pressure;
         pp.use_gravity = P.use_gravity;
         pp.initial_flow_through = P.initial_flow_through;
+        pp.still = 0;
         prepare_push_fields();
         launch("push", k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, pp,
                (const uint32_t*)pf_tile.p,
-               S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.cnt_tile.p, (const uint32_t*)S_.sub_new.p, dcount.p,
-               kbits, 0, (uint32_t)S_.cap);
+               S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.appendix(), S_.cnt_tile.p,
+               (const uint32_t*)S_.sub_new.p, dcount.p, kbits, 0, (uint32_t)S_.cap);
         S_.pp = pp;
         S_.pushed = true;
+    }
+
+    // Merge injected particles that wait past the store's end into the
+    // sorted store without pushing anything (the same two passes, still).
+    void flush_injected(Species& S_) {
+        if (!S_.inj_pending) return;
+        if (S_.pushed) throw StateError("injected particles pending between propagate and wait_particle_sends");
+        PushParams pp{};
+        pp.still = 1;
+        prepare_push_fields();
+        launch("sort_still", k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, pp,
+               (const uint32_t*)pf_tile.p, S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.appendix(), S_.cnt_tile.p,
+               (const uint32_t*)S_.sub_new.p, dcount.p, kbits, 0, (uint32_t)S_.cap);
+        S_.pp = pp;
+        S_.pushed = true;
+        finish_sort(S_, "sort_still");
     }
 
     void scan(const uint32_t* in, uint32_t n, uint32_t* out, size_t level) {
@@ -586,15 +638,20 @@ public:
     void wait_particle_sends(int s) override {
         Species& S_ = S(s);
         if (!S_.pushed) throw StateError("wait_particle_sends without a preceding propagate");
+        finish_sort(S_, "move");
+    }
+
+    void finish_sort(Species& S_, const char* move_name) {
         // totals and in-subcell offsets in one pass, then the scan of the totals
         launch("sort_slots", k_slot_offsets<true>, tile_grid(), g.tc * 8, 0, g, S_.cnt_tile.p, S_.sub_new.p, dcount.p);
         scan(S_.sub_new.p, g.nkeys, S_.sub_new.p, 0);
-        launch("move", k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, S_.pp,
+        launch(move_name, k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, S_.pp,
                (const uint32_t*)pf_tile.p,
-               S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.cnt_tile.p, (const uint32_t*)S_.sub_new.p, dcount.p,
-               kbits, 1, (uint32_t)S_.cap);
+               S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.appendix(), S_.cnt_tile.p,
+               (const uint32_t*)S_.sub_new.p, dcount.p, kbits, 1, (uint32_t)S_.cap);
         S_.swap_stores();
         S_.pushed = false;
+        S_.inj_pending = false;
     }
 
     void clear_accumulation() override {
@@ -605,6 +662,7 @@ public:
         Species& S_ = S(s);
         if (!S_.d.accumulate) return;
         if (S_.pushed) throw StateError("accumulate between propagate and wait_particle_sends");
+        flush_injected(S_);
         const int NS = g.tc * 8;
         // tiles with at least 10 particles per subcell on average take the
         // coalesced eight-lanes-per-subcell path (measured crossover: at 8 per
@@ -691,6 +749,7 @@ public:
     void apply_bcs(int s) override {
         Species& S_ = S(s);
         if (S_.pushed) throw StateError("boundary conditions between propagate and wait_particle_sends");
+        flush_injected(S_);
         launch("boundary", k_boundary<Real>, cdiv(n_bc, BC_CELLS_PER_BLOCK), 256, 0, g, (const int*)bc_cells.p, n_bc,
                (const uint32_t*)S_.sub.p, S_.A(), S_.d.R2_obstacle);
     }
@@ -759,21 +818,167 @@ public:
 
     void step(int n) override {
         for (int it = 0; it < n; ++it) {
+            ++tstep;
             for (size_t s = 0; s < sp.size(); ++s) propagate((int)s);
             clear_accumulation();
             for (size_t s = 0; s < sp.size(); ++s) wait_particle_sends((int)s);
             for (size_t s = 0; s < sp.size(); ++s) accumulate((int)s);
             add_remote_updates();
             for (size_t s = 0; s < sp.size(); ++s) apply_bcs((int)s);
+            for (size_t s = 0; s < sp.size(); ++s) inject((int)s);
             if (dn_attached) denoise_on_stream(dn_attached, dn_bound);
             propagate_B();
         }
+        check_overflow();
+    }
+
+    // The sort and the injection count overflows of a particle store in
+    // slot 0 of their counter; read it so that the steps that lost particles
+    // report it.
+    void check_overflow() {
+        unsigned long long h = 0;
+        RH_HIP(hipMemcpyAsync(&h, dcount.p + C_RANK_OVERFLOW, sizeof(h), hipMemcpyDeviceToHost, stream));
+        RH_HIP(hipStreamSynchronize(stream));
+        if (h)
+            throw CapError("particle sort overflow (a subcell holds more than 65535 particles of one species, "
+                           "or the particle store is full)");
+    }
+
+    // ---- injection -------------------------------------------------------
+
+    int64_t tstep = 0;  // Simulation::timestep: advanced by one at the start of every step
+    void set_timestep(int64_t t) override { tstep = t; }
+    int64_t timestep() override { return tstep; }
+
+    void set_injector(int s, const rh_injector_desc* d) override {
+        Species& S_ = S(s);
+        if (S_.pushed) throw StateError("set_injector between propagate and wait_particle_sends");
+        flush_injected(S_);
+        S_.inj = InjParams{};
+        S_.inj_ncell = 0;
+        if (!d || d->kind == RH_INJ_NONE) return;
+        InjParams ip{};
+        ip.k0 = (uint32_t)d->seed;
+        ip.k1 = (uint32_t)(d->seed >> 32);
+        if (!(d->temperature > 0)) throw ArgError("injector temperature must be positive");
+        ip.vth = std::sqrt(KB * d->temperature / S_.d.m);
+        std::vector<int> cells;
+        std::vector<double> rate;
+        if (d->kind == RH_INJ_SOLAR_WIND) {
+            // InjectorSolarWind::initialize (src/particle_injector.cpp:588-682)
+            int nzc = 0, axis = -1;
+            for (int l = 0; l < 3; ++l)
+                if (d->velocity[l] != 0.0) { ++nzc; axis = l; }
+            if (nzc != 1) throw ArgError("solar wind velocity must have exactly one non-zero component");
+            if (!(d->ppc > 0) || !(d->density > 0)) throw ArgError("solar wind density and ppc must be positive");
+            const double w = d->density * g.dx * g.dx * g.dx / d->ppc;
+            if (!(std::fabs(w - S_.d.w) <= 1e-9 * std::fabs(w)))
+                throw ArgError("species weight differs from density*dV/ppc of its solar wind injector");
+            const double U = std::fabs(d->velocity[axis]);
+            ip.kind = INJ_SOLAR_WIND;
+            ip.axis = axis;
+            ip.pos_wall = d->velocity[axis] < 0;
+            ip.rate = d->ppc * U * P.dt / g.dx;
+            ip.x0 = U / ip.vth;
+            deriv_gauss_consts(ip.x0, ip.xm, ip.c, ip.d, ip.cxm);
+            // checkIfInjectionCell{X,Y,Z}{Pos,Neg} (:392-454): the wall
+            // neighbour is missing, the four transverse neighbours exist
+            const int dims[3] = {g.nx, g.ny, g.nz};
+            for (int n = 0; n < g.nc; ++n) {
+                int c3[3];
+                cell_ijk(g, n, c3[0], c3[1], c3[2]);
+                bool ok = ip.pos_wall ? !has_pos(c3[axis], dims[axis], g.per[axis]) : !has_neg(c3[axis], g.per[axis]);
+                for (int l = 0; l < 3 && ok; ++l)
+                    if (l != axis) ok = has_pos(c3[l], dims[l], g.per[l]) && has_neg(c3[l], g.per[l]);
+                if (ok) cells.push_back(n);
+            }
+        } else if (d->kind == RH_INJ_EXOSPHERE) {
+            if (!d->cell_rate) throw ArgError("exosphere injector needs cell_rate");
+            ip.kind = INJ_EXOSPHERE;
+            ip.eps = 1.0e-4;  // epsCell (src/particle_injector.cpp:31)
+            ip.R2_exobase = d->exobase_radius > 0 ? d->exobase_radius * d->exobase_radius : 0.0;
+            for (int n = 0; n < g.nc; ++n) {
+                const double r = d->cell_rate[n];
+                if (!(r >= 0) || !std::isfinite(r)) throw ArgError("cell_rate must be finite and non-negative");
+                if (r > 0) cells.push_back(n);
+            }
+        } else {
+            throw ArgError("unknown injector kind");
+        }
+        // listed cells in subcell-key order: each tile's cells are one range
+        std::vector<uint32_t> key(cells.size());
+        for (size_t q = 0; q < cells.size(); ++q) {
+            int i, j, k;
+            cell_ijk(g, cells[q], i, j, k);
+            key[q] = cell_key(g, i, j, k);
+        }
+        std::vector<size_t> order(cells.size());
+        for (size_t q = 0; q < order.size(); ++q) order[q] = q;
+        std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return key[a] < key[b]; });
+        std::vector<int> hc(cells.size());
+        std::vector<double> hr(cells.size());
+        std::vector<uint32_t> cbeg((size_t)g.ntiles + 1, 0);
+        for (size_t q = 0; q < order.size(); ++q) {
+            hc[q] = cells[order[q]];
+            if (ip.kind == INJ_EXOSPHERE) hr[q] = d->cell_rate[cells[order[q]]];
+            cbeg[key[order[q]] / (uint32_t)g.tc + 1]++;
+        }
+        for (int T = 0; T < g.ntiles; ++T) cbeg[T + 1] += cbeg[T];
+        const size_t nl = std::max<size_t>(hc.size(), 1);
+        S_.inj_cells.alloc(nl);
+        S_.inj_rate.alloc(ip.kind == INJ_EXOSPHERE ? nl : 1);
+        S_.inj_cbeg.alloc(cbeg.size());
+        S_.inj_off.alloc(nl + 1);
+        if (!S_.inj_total.p) {
+            S_.inj_total.alloc(1);
+            RH_HIP(hipMemsetAsync(S_.inj_total.p, 0, sizeof(unsigned long long), stream));
+        }
+        if (!hc.empty()) {
+            RH_HIP(hipMemcpyAsync(S_.inj_cells.p, hc.data(), hc.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+            if (ip.kind == INJ_EXOSPHERE)
+                RH_HIP(hipMemcpyAsync(S_.inj_rate.p, hr.data(), hr.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+        }
+        RH_HIP(hipMemcpyAsync(S_.inj_cbeg.p, cbeg.data(), cbeg.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        RH_HIP(hipStreamSynchronize(stream));
+        S_.inj = ip;
+        S_.inj_ncell = (int)hc.size();
+    }
+
+    // ParticleInjectorBase::inject for the current timestep (nothing at
+    // timestep <= 0, src/particle_injector.cpp:544): count, scan, fill.
+    void inject(int s) override {
+        Species& S_ = S(s);
+        if (S_.pushed) throw StateError("inject between propagate and wait_particle_sends");
+        if (S_.inj.kind == INJ_NONE || S_.inj_ncell == 0 || tstep <= 0) return;
+        flush_injected(S_);  // a second injection before the next push
+        InjParams ip = S_.inj;
+        ip.step = (uint32_t)tstep;
+        const unsigned nb = cdiv(S_.inj_ncell, 256);
+        const double* rate = ip.kind == INJ_EXOSPHERE ? S_.inj_rate.p : nullptr;
+        launch("inject_count", k_inject_count<Real>, nb, 256, 0, g, ip, S_.inj_ncell, (const int*)S_.inj_cells.p,
+               rate, S_.inj_off.p);
+        scan(S_.inj_off.p, (uint32_t)S_.inj_ncell, S_.inj_off.p, 0);
+        launch("inject_fill", k_inject_fill<Real>, nb, 256, 0, g, ip, S_.inj_ncell, (const int*)S_.inj_cells.p, rate,
+               (const uint32_t*)S_.inj_off.p, (const uint32_t*)(S_.sub.p + g.nkeys), S_.A(), (uint32_t)S_.cap,
+               S_.inj_total.p, dcount.p);
+        S_.inj_pending = true;
+    }
+
+    int64_t injected(int s) override {
+        Species& S_ = S(s);
+        if (!S_.inj_total.p) return 0;
+        unsigned long long h = 0;
+        RH_HIP(hipMemcpyAsync(&h, S_.inj_total.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+        RH_HIP(hipStreamSynchronize(stream));
+        return (int64_t)h;
     }
 
     void diag_raw(int s, double* t) {
         Species& S_ = S(s);
         if (S_.pushed) throw StateError("diagnostics between propagate and wait_particle_sends");
-        launch("diag", k_diag<Real>, DIAG_BLOCKS, 256, 0, S_.A(), (const uint32_t*)(S_.sub.p + g.nkeys), diag_partial.p);
+        const uint32_t* extra = S_.inj_pending ? (const uint32_t*)(S_.inj_off.p + S_.inj_ncell) : nullptr;
+        launch("diag", k_diag<Real>, DIAG_BLOCKS, 256, 0, S_.A(), (const uint32_t*)(S_.sub.p + g.nkeys), extra,
+               (uint32_t)S_.cap, diag_partial.p);
         std::vector<double> h(DIAG_BLOCKS * 5);
         RH_HIP(hipMemcpyAsync(h.data(), diag_partial.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
         RH_HIP(hipStreamSynchronize(stream));
@@ -965,6 +1170,23 @@ int rh_set_denoiser(rh_ctx* ctx, rh_denoiser* dn, double max_rel_change) {
 }
 int rh_denoise_moments(rh_ctx* ctx, rh_denoiser* dn, double max_rel_change) {
     return guarded(ctx, [&](rh::Ctx& c) { c.denoise(dn, max_rel_change); });
+}
+int rh_set_injector(rh_ctx* ctx, int32_t s, const rh_injector_desc* desc) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.set_injector(s, desc); });
+}
+int rh_inject_particles(rh_ctx* ctx, int32_t s) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.inject(s); });
+}
+int rh_set_timestep(rh_ctx* ctx, int64_t t) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.set_timestep(t); });
+}
+int rh_get_timestep(rh_ctx* ctx, int64_t* t) {
+    if (!t) return RH_ERR_ARG;
+    return guarded(ctx, [&](rh::Ctx& c) { *t = c.timestep(); });
+}
+int rh_injected(rh_ctx* ctx, int32_t s, int64_t* n) {
+    if (!n) return RH_ERR_ARG;
+    return guarded(ctx, [&](rh::Ctx& c) { *n = c.injected(s); });
 }
 int rh_layout_info(rh_ctx* ctx, int64_t out[6]) {
     if (!out) return RH_ERR_ARG;

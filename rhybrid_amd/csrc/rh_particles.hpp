@@ -38,6 +38,7 @@ struct PushParams {
     double max_vi2;
     double GMdt;
     int accelerate, use_e_pressure, use_gravity, initial_flow_through;
+    int still;           // sort only: no acceleration, no move
 };
 
 enum DevCounter { C_MAX_VI = 0, C_MIN_CELL_RHO, C_MAX_CELL_UE, C_MIN_NODE_RHO, C_MAX_NODE_UE,
@@ -163,10 +164,18 @@ __global__ __launch_bounds__(256) void k_push_fields(Geom g, PushFields<Real> f,
             }
 }
 
+// Injected particles waiting for their first push: per tile, the range of
+// listed injection cells (cbeg, ntiles + 1 entries, null when none wait) and
+// per listed cell the offset of its particles past the sorted store's end.
+struct InjAppendix {
+    const uint32_t* cbeg;
+    const uint32_t* off;
+};
+
 template <typename Real>
 __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp, const uint32_t* __restrict__ pfb,
                                                          PArrays<Real> A, PArrays<Real> D,
-                                                         const uint32_t* __restrict__ sub,
+                                                         const uint32_t* __restrict__ sub, InjAppendix app,
                                                          uint16_t* __restrict__ cnt_tile,
                                                          const uint32_t* __restrict__ key_base,
                                                          unsigned long long* __restrict__ counters, int kbits,
@@ -196,24 +205,41 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
     }
     __syncthreads();
     const uint32_t p0 = sub[(size_t)T * NS], p1 = sub[(size_t)(T + 1) * NS];
+    // the tile's injected particles (past the store's end) follow its own
+    uint32_t a0 = 0, na = 0;
+    if (app.cbeg) {
+        const uint32_t c0 = app.cbeg[T], c1 = app.cbeg[T + 1];
+        if (c1 > c0) { a0 = sub[g.nkeys] + app.off[c0]; na = app.off[c1] - app.off[c0]; }
+    }
+    const uint32_t n0 = p1 - p0, ntot = n0 + na;
     const uint32_t deadkey = (uint32_t)MAX_EXTK;
-    // prefetch of the first chunk
+    // prefetch of the first chunk; an injected particle beyond the store's
+    // capacity was never written (the injection counted the overflow)
     Real nX = 0, nY = 0, nZ = 0, nvx = 0, nvy = 0, nvz = 0;
     int nlc = 0;
     {
-        const uint32_t p = p0 + tid;
-        if (p < p1) { nX = A.x[p]; nY = A.y[p]; nZ = A.z[p]; nvx = A.vx[p]; nvy = A.vy[p]; nvz = A.vz[p]; nlc = A.lc[p]; }
+        const uint32_t u = tid;
+        const uint32_t p = u < n0 ? p0 + u : a0 + (u - n0);
+        if (u < ntot) {
+            if (p < cap) { nX = A.x[p]; nY = A.y[p]; nZ = A.z[p]; nvx = A.vx[p]; nvy = A.vy[p]; nvz = A.vz[p]; nlc = A.lc[p]; }
+            else nX = Real(-1);
+        }
     }
-    for (uint32_t base = p0; base < p1; base += PM_THREADS) {
-        const uint32_t p = base + tid;
-        const bool act = p < p1;
+    for (uint32_t ub = 0; ub < ntot; ub += PM_THREADS) {
+        const uint32_t u = ub + tid;
+        const bool act = u < ntot;
         Real X = nX, Y = nY, Z = nZ, vx = nvx, vy = nvy, vz = nvz;
         const int lc = nlc;
         {
-            const uint32_t pn = p + PM_THREADS;
-            if (pn < p1) {
-                nX = A.x[pn]; nY = A.y[pn]; nZ = A.z[pn]; nvx = A.vx[pn]; nvy = A.vy[pn]; nvz = A.vz[pn];
-                nlc = A.lc[pn];
+            const uint32_t un = u + PM_THREADS;
+            const uint32_t pn = un < n0 ? p0 + un : a0 + (un - n0);
+            if (un < ntot) {
+                if (pn < cap) {
+                    nX = A.x[pn]; nY = A.y[pn]; nZ = A.z[pn]; nvx = A.vx[pn]; nvy = A.vy[pn]; nvz = A.vz[pn];
+                    nlc = A.lc[pn];
+                } else {
+                    nX = Real(-1);
+                }
             }
         }
         uint32_t key = deadkey;
@@ -268,18 +294,23 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
                     clamped = true;
                 }
             }
-            // move in FP64; the integer part is the cell crossing
-            const double Xd = (double)X + (double)vx * pp.dt_dx;
-            const double Yd = (double)Y + (double)vy * pp.dt_dx;
-            const double Zd = (double)Z + (double)vz * pp.dt_dx;
-            const double fx = floor(Xd), fy = floor(Yd), fz = floor(Zd);
-            const bool ok = (fx >= -1.0 && fx <= 1.0 && fy >= -1.0 && fy <= 1.0 && fz >= -1.0 && fz <= 1.0);
-            const int sx = (int)fx, sy = (int)fy, sz = (int)fz;
-            if (ok && ((nm >> ((sx + 1) + 3 * (sy + 1) + 9 * (sz + 1))) & 1u)) {
+            // move in FP64; the integer part is the cell crossing (a still
+            // pass keeps every particle where it is)
+            int sx = 0, sy = 0, sz = 0;
+            bool ok = true;
+            if (!pp.still) {
+                const double Xd = (double)X + (double)vx * pp.dt_dx;
+                const double Yd = (double)Y + (double)vy * pp.dt_dx;
+                const double Zd = (double)Z + (double)vz * pp.dt_dx;
+                const double fx = floor(Xd), fy = floor(Yd), fz = floor(Zd);
+                ok = (fx >= -1.0 && fx <= 1.0 && fy >= -1.0 && fy <= 1.0 && fz >= -1.0 && fz <= 1.0);
+                sx = (int)fx; sy = (int)fy; sz = (int)fz;
                 X = (Real)(Xd - fx); Y = (Real)(Yd - fy); Z = (Real)(Zd - fz);
                 if (X >= Real(1)) X = below_one<Real>();
                 if (Y >= Real(1)) Y = below_one<Real>();
                 if (Z >= Real(1)) Z = below_one<Real>();
+            }
+            if (ok && ((nm >> ((sx + 1) + 3 * (sy + 1) + 9 * (sz + 1))) & 1u)) {
                 const int o = (X >= Real(0.5)) + 2 * (Y >= Real(0.5)) + 4 * (Z >= Real(0.5));
                 key = (uint32_t)(((li + sx + 1) + g.ex * ((lj + sy + 1) + g.ey * (lk + sz + 1))) * 8 + o);
                 // tiles divide the grid, so a periodic wrap keeps the local index
@@ -808,11 +839,15 @@ __global__ __launch_bounds__(256) void k_boundary(Geom g, const int* __restrict_
 // FP64 conservation sums (logCalcParticle, src/diagnostics.h:209-220), one
 // partial per block, summed on the host in block order.
 constexpr int DIAG_BLOCKS = 1024;
+// Injected particles waiting past the store's end (*extra of them, extra may
+// be null) are included.
 template <typename Real>
 __global__ __launch_bounds__(256) void k_diag(PArrays<Real> A, const uint32_t* __restrict__ np_ptr,
+                                             const uint32_t* __restrict__ extra, uint32_t cap,
                                              double* __restrict__ partial) {
     __shared__ double red[5][256];
-    const uint32_t np = *np_ptr;
+    uint32_t np = *np_ptr;
+    if (extra) np = (uint64_t)np + *extra > cap ? cap : np + *extra;
     double s[5] = {0, 0, 0, 0, 0};
     for (uint32_t p = blockIdx.x * 256u + threadIdx.x; p < np; p += 256u * gridDim.x) {
         if (A.x[p] < Real(0)) continue;

@@ -35,7 +35,8 @@ EXPORTS = ["rh_create", "rh_destroy", "rh_last_error", "rh_add_species", "rh_upl
            "rh_clear_accumulation_arrays", "rh_accumulate", "rh_add_remote_updates",
            "rh_apply_boundary_conditions", "rh_propagate_B", "rh_step", "rh_diagnostics",
            "rh_counters", "rh_synchronize", "rh_set_timing", "rh_kernel_times", "rh_reset_timing",
-           "rh_layout_info", "rh_denoise_moments", "rh_set_denoiser"]
+           "rh_layout_info", "rh_denoise_moments", "rh_set_denoiser", "rh_set_injector",
+           "rh_inject_particles", "rh_set_timestep", "rh_get_timestep", "rh_injected"]
 
 
 class GridDesc(C.Structure):
@@ -61,6 +62,12 @@ class SpeciesDesc(C.Structure):
     _fields_ = [("q", C.c_double), ("m", C.c_double), ("w", C.c_double),
                 ("R2_obstacle", C.c_double), ("accumulate", C.c_int32),
                 ("accelerate", C.c_int32), ("capacity", C.c_int64)]
+
+
+class InjectorDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("seed", C.c_uint64), ("temperature", C.c_double),
+                ("velocity", C.c_double * 3), ("density", C.c_double), ("ppc", C.c_double),
+                ("cell_rate", C.c_void_p), ("exobase_radius", C.c_double)]
 
 
 class RhError(RuntimeError):
@@ -113,6 +120,11 @@ def load_library(path: str = LIB_PATH):
     L.rh_set_denoiser.argtypes = [vp, vp, C.c_double]
     L.rh_kernel_times.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(i64)]
     L.rh_layout_info.argtypes = [vp, vp]
+    L.rh_set_injector.argtypes = [vp, i32, C.POINTER(InjectorDesc)]
+    L.rh_inject_particles.argtypes = [vp, i32]
+    L.rh_set_timestep.argtypes = [vp, i64]
+    L.rh_get_timestep.argtypes = [vp, C.POINTER(i64)]
+    L.rh_injected.argtypes = [vp, i32, C.POINTER(i64)]
     for name in EXPORTS:
         if name not in ("rh_last_error",):
             getattr(L, name).restype = C.c_int
@@ -152,7 +164,8 @@ class Context:
         self.species: List[int] = []
         for sp in problem.species:
             sid = C.c_int32()
-            d = SpeciesDesc(sp.q, sp.m, sp.w, sp.R2_obstacle, sp.accumulate, sp.accelerate, 0)
+            d = SpeciesDesc(sp.q, sp.m, sp.w, sp.R2_obstacle, sp.accumulate, sp.accelerate,
+                            int(getattr(sp, "capacity", 0)))
             self._check(L.rh_add_species(self._h, C.byref(d), C.byref(sid)))
             self.species.append(sid.value)
         for name, arr in problem.fields.items():
@@ -162,6 +175,9 @@ class Context:
                                           _ptr(np.ascontiguousarray(arr, dtype=np.uint8))))
         for s, parts in enumerate(problem.particles):
             self.upload_particles(s, parts.cell, parts.state)
+        for s, inj in enumerate(problem.injectors or []):
+            if inj is not None:
+                self.set_injector(s, inj)
 
     def _check(self, rc: int) -> None:
         if rc != RH_OK:
@@ -243,6 +259,36 @@ class Context:
         h = denoiser._h if denoiser is not None else None
         self._check(load_library().rh_set_denoiser(self._h, h, float(max_rel_change)))
 
+    # -- injection -------------------------------------------------------------
+    def set_injector(self, s: int, inj) -> None:
+        """Attach a problem.Injector to species s (None detaches)."""
+        d = InjectorDesc()
+        if inj is not None:
+            d.kind, d.seed, d.temperature = inj.kind, inj.seed, inj.temperature
+            for l in range(3):
+                d.velocity[l] = inj.velocity[l]
+            d.density, d.ppc, d.exobase_radius = inj.density, inj.ppc, inj.exobase_radius
+            rate = None
+            if inj.cell_rate is not None:
+                rate = np.ascontiguousarray(inj.cell_rate, dtype=np.float64)
+                if rate.shape != (self.grid.nc,):
+                    raise ValueError("cell_rate needs one value per cell")
+                d.cell_rate = rate.ctypes.data
+        self._check(load_library().rh_set_injector(self._h, s, C.byref(d)))
+
+    def inject(self, s: int) -> None: self._check(load_library().rh_inject_particles(self._h, s))
+    def set_timestep(self, t: int) -> None: self._check(load_library().rh_set_timestep(self._h, int(t)))
+
+    def timestep(self) -> int:
+        t = C.c_int64()
+        self._check(load_library().rh_get_timestep(self._h, C.byref(t)))
+        return t.value
+
+    def injected(self, s: int) -> int:
+        n = C.c_int64()
+        self._check(load_library().rh_injected(self._h, s, C.byref(n)))
+        return n.value
+
     def counters(self) -> Dict[str, float]:
         out = np.zeros(len(COUNTER_NAMES))
         self._check(load_library().rh_counters(self._h, _ptr(out)))
@@ -315,6 +361,20 @@ class Accumulator:
         return True
 
 
+class ParticleInjector:
+    """A species' injector (ParticleInjectorBase, src/particle_injector.h):
+    inject() adds the new particles of the context's current timestep."""
+
+    def __init__(self, ctx: Context, species: int, injector=None):
+        self.ctx, self.species = ctx, species
+        if injector is not None:
+            ctx.set_injector(species, injector)
+
+    def inject(self) -> bool:
+        self.ctx.inject(self.species)
+        return True
+
+
 def setupGetFields(ctx: Context) -> None:
     """src/hybrid_propagator.cpp:1861-1876."""
     ctx.setup_get_fields()
@@ -326,9 +386,11 @@ def propagateB(ctx: Context) -> bool:
     return True
 
 
-def propagate(ctx: Context, propagators: List[BorisBuneman], accumulators: List[Accumulator]) -> bool:
+def propagate(ctx: Context, propagators: List[BorisBuneman], accumulators: List[Accumulator],
+              injectors: Optional[List[ParticleInjector]] = None) -> bool:
     """One time step in the order of propagate() (src/user.cpp:118-134),
-    without logging and injection."""
+    without logging.  The caller advances the timestep first (Corsair's
+    main loop does, ctx.set_timestep)."""
     setupGetFields(ctx)
     for p in propagators:
         p.propagateCell()
@@ -345,4 +407,6 @@ def propagate(ctx: Context, propagators: List[BorisBuneman], accumulators: List[
         a.addRemoteUpdates()
     for s in range(len(propagators)):
         ctx.apply_boundary_conditions(s)
+    for inj in injectors or []:
+        inj.inject()
     return propagateB(ctx)

@@ -104,6 +104,7 @@ class Species:
     R2_obstacle: float = -1.0
     accumulate: int = 1
     accelerate: int = 1
+    capacity: int = 0       # particle store size to reserve on the GPU (0: 1.5x the first upload)
 
 
 @dataclass

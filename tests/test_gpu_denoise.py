@@ -242,7 +242,7 @@ def test_denoised_8ppc_in_the_loop_beats_plain_8ppc(dn_mod):
     64-ppc run's error; the FP32 256-ppc run from the same particles matches
     the FP64 one to 1e-5."""
     from rhybrid_amd import denoise_loop as DL
-    r = DL.compare(nx=64, ny=32, nz=32, steps=20, train_iters=200)
+    r = DL.compare(nx=64, ny=32, nz=32, steps=20, train_iters=400)
     runs = r["runs"]
     print({k: v for k, v in runs.items()})
     d8, p8, p64 = runs["fp32_8_denoised"], runs["fp32_8"], runs["fp32_64"]

@@ -1,0 +1,185 @@
+"""Parity of the device injection (src/particle_injector.cpp) with the FP64
+oracle, through the C ABI.
+
+Both draw from the same counter-based generator, so a timestep's new
+particles are the same set: counts per cell and the uniform positions agree
+exactly, velocities (which go through exp and log of the device math library
+and of the host's) to 1e-12 relative in FP64; FP32 rounds the stored state.
+Runs with inflow then follow the oracle step for step."""
+import math
+
+import numpy as np
+import pytest
+
+from rhybrid_amd import problem as P
+
+pytestmark = pytest.mark.gpu
+
+
+def canon(cell, st):
+    o = np.lexsort((st[:, 5], st[:, 4], st[:, 3], st[:, 2], st[:, 1], st[:, 0], cell))
+    return cell[o], st[o]
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def wall_box(velocity, nx=16, ny=8, nz=8, periodic=(0, 0, 0), ppc=24.0, dx=1e5, dt=0.1, seed=21, capacity=200000):
+    g = P.Grid(nx, ny, nz, periodic, dx, -0.5 * nx * dx, -0.5 * ny * dx, -0.5 * nz * dx)
+    p = P.Params(dt=dt, use_hall=0, max_ion_speed2=1e30)
+    n = 2e6
+    sp = P.Species("H+", P.QE, P.MP, n * g.dV / ppc, accelerate=0, capacity=capacity)
+    empty = P.Particles(np.zeros(0, dtype=np.int64), np.zeros((0, 7)))
+    inj = P.solar_wind_injector(velocity, n, 2e5, ppc, seed=seed)
+    return P.Problem("wall", g, p, [sp], P.make_fields(g), P.make_flags(g), [empty], injectors=[inj])
+
+
+def exo_box(n=16, dx=1e5, macro=3000.0, R_exobase_cells=3.0, seed=31, capacity=200000):
+    g = P.Grid(n, n, n, (0, 0, 0), dx, -0.5 * n * dx, -0.5 * n * dx, -0.5 * n * dx)
+    p = P.Params(dt=0.05)
+    sp = P.Species("O+", P.QE, 16 * P.MP, 1.0, capacity=capacity)
+    neut = P.exosphere_neutrals(g, 1e12, 3 * dx, 1.5 * dx, R_exobase=3 * dx, sub=2)
+    inj = P.exosphere_injector(g, neut, 2e4, macro, R_exobase=R_exobase_cells * dx, seed=seed)
+    empty = P.Particles(np.zeros(0, dtype=np.int64), np.zeros((0, 7)))
+    return P.Problem("exo", g, p, [sp], P.make_fields(g), P.make_flags(g), [empty], injectors=[inj])
+
+
+def injected_pair(gpu_mod, oracle_mod, pr, prec, steps, tile=(0, 0, 0)):
+    print("case %s precision=%s steps=%s tile=%s" % (pr.name, prec, steps, tile), flush=True)
+    ctx = gpu_mod.Context(pr, precision=prec, tile=tile)
+    ora = oracle_mod.Oracle(pr, threads=1)
+    for t in steps:
+        ctx.set_timestep(t)
+        ctx.inject(0)
+        ora.set_timestep(t)
+        ora.inject(0)
+    return ctx, ora
+
+
+def compare_sets(ctx, ora, prec, dx):
+    cg, sg = canon(*ctx.particles(0))
+    co, so = canon(*ora.particles(0))
+    assert cg.size > 0
+    assert np.array_equal(cg, co)
+    if prec == 1:
+        assert np.array_equal(sg[:, 0:3], so[:, 0:3]) or np.abs(sg[:, 0:3] - so[:, 0:3]).max() <= 4e-16 * dx
+        vs = np.abs(so[:, 3:6]).max()
+        assert np.abs(sg[:, 3:6] - so[:, 3:6]).max() <= 1e-12 * vs
+    else:
+        assert np.abs(sg[:, 0:3] - so[:, 0:3]).max() <= 1.2e-7 * dx
+        assert np.all(np.abs(sg[:, 3:6] - so[:, 3:6]) <= 1.2e-7 * np.abs(so[:, 3:6]) + 1e-30)
+    assert np.all(sg[:, 6] == so[:, 6])
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+@pytest.mark.parametrize("velocity", [(-3e5, 0, 0), (3e5, 0, 0), (0, -3e5, 0), (0, 0, 3e5)])
+def test_solar_wind_matches_oracle(gpu_mod, oracle_mod, prec, velocity):
+    pr = wall_box(velocity)
+    ctx, ora = injected_pair(gpu_mod, oracle_mod, pr, prec, steps=[0, 1, 2, 7])
+    compare_sets(ctx, ora, prec, pr.grid.dx)
+    assert ctx.injected(0) == ora.injected(0)
+
+
+def test_solar_wind_periodic_transverse_and_tiles(gpu_mod, oracle_mod):
+    # periodic y, z: every cell of the wall is an injection cell; odd tile shape
+    pr = wall_box((-3e5, 0, 0), nx=16, ny=8, nz=4, periodic=(0, 1, 1))
+    ctx, ora = injected_pair(gpu_mod, oracle_mod, pr, 1, steps=[3], tile=(4, 2, 4))
+    compare_sets(ctx, ora, 1, pr.grid.dx)
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_exosphere_matches_oracle(gpu_mod, oracle_mod, prec):
+    pr = exo_box()
+    ctx, ora = injected_pair(gpu_mod, oracle_mod, pr, prec, steps=[1, 2])
+    compare_sets(ctx, ora, prec, pr.grid.dx)
+
+
+def test_pending_particles_in_diagnostics(gpu_mod, oracle_mod):
+    pr = wall_box((-3e5, 0, 0))
+    ctx, ora = injected_pair(gpu_mod, oracle_mod, pr, 1, steps=[4])
+    # before any push the new particles wait past the sorted store
+    assert ctx.num_particles(0) == ora.num_particles(0)
+    assert np.allclose(ctx.diagnostics(0), ora.diagnostics(0), rtol=1e-12)
+    # a second injection before the next push keeps both sets
+    ctx.set_timestep(5); ctx.inject(0)
+    ora.set_timestep(5); ora.inject(0)
+    assert ctx.num_particles(0) == ora.num_particles(0)
+    compare_sets(ctx, ora, 1, pr.grid.dx)
+
+
+def test_neg_wall_particles_stay_on_the_face(gpu_mod, oracle_mod):
+    # inflow through the -x wall starts at local x = dx of the exterior cell
+    pr = wall_box((3e5, 0, 0))
+    ctx, _ = injected_pair(gpu_mod, oracle_mod, pr, 1, steps=[1])
+    cell, st = ctx.particles(0)
+    assert np.all(cell % pr.grid.nx == 0)
+    assert np.all(st[:, 0] == pr.grid.dx)
+    assert np.all(st[:, 3] > 0)
+
+
+def test_inflow_steps_match_oracle(gpu_mod, oracle_mod):
+    """Inflow/outflow box with the obstacle, preload plus solar-wind
+    injection of both species, FP64: particle counts every step and the
+    fields after 10 steps."""
+    pr = P.config_obstacle(nx=16, ny=12, nz=12, ppc=8, R_cells=3, seed=14, inflow=True)
+    for sp in pr.species:
+        sp.capacity = 3 * pr.particles[0].n
+    print("case inflow obstacle np=%d" % pr.n_particles, flush=True)
+    ctx = gpu_mod.Context(pr, precision=gpu_mod.RH_FP64)
+    ora = oracle_mod.Oracle(pr)
+    for step in range(10):
+        ctx.step(1)
+        ora.step(1)
+        for s in range(2):
+            assert ctx.num_particles(s) == ora.num_particles(s), (step, s)
+    assert ctx.timestep() == ora.timestep() == 10
+    for s in range(2):
+        assert ctx.injected(s) == ora.injected(s) > 0
+    for name in ("faceB", "cellRhoQi", "cellJi", "nodeE", "cellUe"):
+        b = ora.field(name)
+        assert rel(ctx.field(name), b) < 1e-9, (name, rel(ctx.field(name), b))
+
+
+def test_inflow_fp32_follows_fp64(gpu_mod, oracle_mod):
+    pr = P.config_obstacle(nx=16, ny=12, nz=12, ppc=16, R_cells=3, seed=15, inflow=True)
+    for sp in pr.species:
+        sp.capacity = 3 * pr.particles[0].n
+    print("case inflow obstacle fp32 np=%d" % pr.n_particles, flush=True)
+    ctx = gpu_mod.Context(pr, precision=gpu_mod.RH_FP32)
+    ora = oracle_mod.Oracle(pr)
+    ctx.step(20)
+    ora.step(20)
+    for s in range(2):
+        assert abs(ctx.num_particles(s) - ora.num_particles(s)) <= 2
+    for name, tol in (("cellRhoQi", 1e-5), ("cellJi", 1e-5), ("faceB", 1e-5)):
+        r = rel(ctx.field(name), ora.field(name))
+        print("inflow fp32 vs fp64 after 20 steps: %s rel L2 = %.3e" % (name, r), flush=True)
+        assert r < tol, (name, r)
+
+
+def test_store_overflow_is_reported(gpu_mod):
+    pr = wall_box((-3e5, 0, 0), capacity=100)
+    print("case overflow", flush=True)
+    ctx = gpu_mod.Context(pr, precision=gpu_mod.RH_FP32)
+    with pytest.raises(gpu_mod.RhError) as e:
+        ctx.step(3)
+    assert e.value.code == gpu_mod.RH_ERR_CAPACITY
+
+
+def test_injector_argument_errors(gpu_mod):
+    pr = wall_box((-3e5, 0, 0))
+    ctx = gpu_mod.Context(pr, precision=gpu_mod.RH_FP32)
+    bad = P.solar_wind_injector((-3e5, 1e5, 0), 2e6, 2e5, 24.0, seed=1)
+    with pytest.raises(gpu_mod.RhError) as e:
+        ctx.set_injector(0, bad)
+    assert e.value.code == gpu_mod.RH_ERR_ARG
+    wrong_w = P.solar_wind_injector((-3e5, 0, 0), 1e6, 2e5, 24.0, seed=1)
+    with pytest.raises(gpu_mod.RhError) as e:
+        ctx.set_injector(0, wrong_w)
+    assert e.value.code == gpu_mod.RH_ERR_ARG
+    ctx.set_injector(0, None)
+    ctx.set_timestep(1)
+    ctx.inject(0)
+    assert ctx.num_particles(0) == 0
+    assert math.isfinite(ctx.diagnostics(0)[0])
