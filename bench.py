@@ -3,9 +3,10 @@
 Workload: 128x96x96-cell inflow/outflow box (dx=300 km) with an absorbing
 sphere of radius 10 cells, H+ and He++ (4% by number) at 32 particles per
 cell each (~71 M macroparticles), drifting Maxwellians at 450 km/s, IMF
-(-1.5, 2.5, 0) nT, FP32 cell-local particles.  One step = push + cell sort +
-CIC deposit + FP64 ghost fold + particle boundary conditions + field update
-(propagate() of the reference, src/user.cpp:118-134, without injection).
+(-1.5, 2.5, 0) nT, FP32 cell-local particles; uniform preload plus
+solar-wind inflow of both species through the +x wall.  One step = push +
+cell sort + CIC deposit + FP64 ghost fold + particle boundary conditions +
+injection + field update (propagate() of the reference, src/user.cpp:118-134).
 
 value = particle-steps per second over the whole timed region (all of the
 step, not only push+deposit); roofline = the kernel with the largest total
@@ -33,7 +34,7 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
 def workload(scale: int = 1, seed: int = 6461674):
     from rhybrid_amd import problem as P
     return P.config_obstacle(nx=128 // scale, ny=96 // scale, nz=96 // scale, ppc=32,
-                             R_cells=10.0 / scale, seed=seed)
+                             R_cells=10.0 / scale, seed=seed, inflow=True)
 
 
 def kernel_bytes(name: str, ctx, np_total: float, nspecies: int) -> float:
@@ -112,7 +113,7 @@ def gpu_bench(args):
     np_after = sum(ctx.num_particles(s) for s in range(nspec))
     np_mean = 0.5 * (np_before + np_after)
     names = ["push_fields", "push", "sort_slots", "sort_count", "scan", "sort_base", "move", "deposit",
-             "deposit_dense", "fold", "boundary",
+             "deposit_dense", "fold", "boundary", "inject_slots", "inject_count", "inject_fill", "sort_still",
              "face2cell", "neumann", "imf", "cell2node", "node_J", "node2cell", "cell_Ue", "node_Ue",
              "upwind_B", "node_E", "node_avg", "face_curl", "cell_Ep", "fill", "diag"]
     ktimes = {}
@@ -318,7 +319,7 @@ def main():
         "config": {"workload": "128x96x96-cell inflow/outflow box, dx=300 km, absorbing sphere R=10 cells, "
                                "H+ and He++ (4%) at 32 ppc each, 450 km/s, T=1e5 K, IMF (-1.5,2.5,0) nT; "
                                "full step (push, cell sort, CIC deposit, FP64 ghost fold, particle BCs, "
-                               "field update), no injection",
+                               "solar-wind injection, field update); uniform preload plus +x-wall inflow",
                    "particles": res["np"], "cells": 128 * 96 * 96, "tile": res["tiles"],
                    "parallelism": "single GPU"},
         "roofline": res["roofline"],

@@ -196,6 +196,7 @@ public:
         // offset of its new particles past the store's end
         InjParams inj{};
         int inj_ncell = 0;
+        int inj_slots = 0;  // solar wind: fixed places per listed cell (ceil of the rate)
         DevBuf<int> inj_cells;
         DevBuf<double> inj_rate;
         DevBuf<uint32_t> inj_cbeg, inj_off;
@@ -856,6 +857,7 @@ public:
         flush_injected(S_);
         S_.inj = InjParams{};
         S_.inj_ncell = 0;
+        S_.inj_slots = 0;
         if (!d || d->kind == RH_INJ_NONE) return;
         InjParams ip{};
         ip.k0 = (uint32_t)d->seed;
@@ -939,9 +941,22 @@ public:
                 RH_HIP(hipMemcpyAsync(S_.inj_rate.p, hr.data(), hr.size() * sizeof(double), hipMemcpyHostToDevice, stream));
         }
         RH_HIP(hipMemcpyAsync(S_.inj_cbeg.p, cbeg.data(), cbeg.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        int slots = 0;
+        std::vector<uint32_t> off;
+        if (ip.kind == INJ_SOLAR_WIND) {
+            // fixed places: probRound never exceeds ceil(rate)
+            const double cr = std::ceil(ip.rate);
+            if (cr * (double)std::max<size_t>(hc.size(), 1) >= 2147483647.0)
+                throw ArgError("solar wind injection rate too large for the fixed places");
+            slots = (int)cr;
+            off.resize(hc.size() + 1);
+            for (size_t q = 0; q <= hc.size(); ++q) off[q] = (uint32_t)(q * (size_t)slots);
+            RH_HIP(hipMemcpyAsync(S_.inj_off.p, off.data(), off.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        }
         RH_HIP(hipStreamSynchronize(stream));
         S_.inj = ip;
         S_.inj_ncell = (int)hc.size();
+        S_.inj_slots = slots;
     }
 
     // ParticleInjectorBase::inject for the current timestep (nothing at
@@ -953,8 +968,16 @@ public:
         flush_injected(S_);  // a second injection before the next push
         InjParams ip = S_.inj;
         ip.step = (uint32_t)tstep;
+        if (ip.kind == INJ_SOLAR_WIND) {
+            if (S_.inj_slots > 0)
+                launch("inject_slots", k_inject_slots<Real>, cdiv((size_t)S_.inj_ncell * S_.inj_slots, 256), 256, 0, g,
+                       ip, S_.inj_ncell, S_.inj_slots, (const int*)S_.inj_cells.p,
+                       (const uint32_t*)(S_.sub.p + g.nkeys), S_.A(), (uint32_t)S_.cap, S_.inj_total.p, dcount.p);
+            S_.inj_pending = S_.inj_slots > 0;
+            return;
+        }
         const unsigned nb = cdiv(S_.inj_ncell, 256);
-        const double* rate = ip.kind == INJ_EXOSPHERE ? S_.inj_rate.p : nullptr;
+        const double* rate = S_.inj_rate.p;
         launch("inject_count", k_inject_count<Real>, nb, 256, 0, g, ip, S_.inj_ncell, (const int*)S_.inj_cells.p,
                rate, S_.inj_off.p);
         scan(S_.inj_off.p, (uint32_t)S_.inj_ncell, S_.inj_off.p, 0);

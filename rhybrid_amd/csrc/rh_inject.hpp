@@ -11,11 +11,13 @@
 //   sampleGaussian       :59-80 (polar method, second deviate kept per stream)
 //   sampleDerivGaussian  :97-114 (rejection from the unit Gaussian at the mode)
 //
-// A step's injection runs as count (new particles per listed cell), scan
-// (offsets), fill (the particles, written past the end of the sorted store
-// in the order of the listed cells).  The list is sorted by subcell key, so
-// each tile's injected particles are one range; the next push/sort pass of
-// the tile takes them after the tile's own particles.
+// New particles are written past the end of the sorted store in the order of
+// the listed cells.  The list is sorted by subcell key, so each tile's new
+// particles are one range; the next push/sort pass of the tile takes them
+// after the tile's own particles.  The solar wind gives every listed cell a
+// fixed number of places (at most ceil(rate) particles per step), filled in
+// one launch; the exosphere's per-cell rates differ by orders of magnitude,
+// so its step is count (new particles per listed cell), scan (offsets), fill.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -105,6 +107,50 @@ RH_HD double deriv_gauss(RngStream& st, const InjParams& ip) {
     }
 }
 
+// Trial particle m of listed cell n (its own stream): position in cell units,
+// velocity in m/s; false when the exosphere discards it below the exobase.
+__device__ __forceinline__ bool gen_particle(const Geom& g, const InjParams& ip, int n, int i, int j, int k, uint32_t m,
+                                             bool need_velocity, double (&r)[3], double (&v)[3]) {
+#pragma clang fp contract(off)
+    RngStream st(ip.k0, ip.k1, (uint32_t)n, ip.step, m);
+    if (ip.kind == INJ_SOLAR_WIND) {
+        const int a = ip.axis;
+        const int l1 = a == 0 ? 1 : 0, l2 = a == 2 ? 1 : 2;
+        r[a] = ip.pos_wall ? 0.0 : 1.0;
+        r[l1] = st.uniform();
+        r[l2] = st.uniform();
+        if (!need_velocity) return true;
+        const double vn = ip.vth * deriv_gauss(st, ip);
+        v[a] = ip.pos_wall ? -vn : vn;
+        v[l1] = ip.vth * st.gauss();
+        v[l2] = ip.vth * st.gauss();
+        return true;
+    }
+    const double dx = g.dx;
+    double rm[3];
+    for (int l = 0; l < 3; ++l) rm[l] = 0.5 * dx + (1.0 - ip.eps) * dx * (st.uniform() - 0.5);
+    const double gx = (g.x0 + i * dx) + rm[0], gy = (g.y0 + j * dx) + rm[1], gz = (g.z0 + k * dx) + rm[2];
+    if (gx * gx + gy * gy + gz * gz < ip.R2_exobase) return false;
+    if (!need_velocity) return true;
+    for (int l = 0; l < 3; ++l) r[l] = rm[l] / dx;
+    for (int l = 0; l < 3; ++l) v[l] = ip.vth * st.gauss();
+    return true;
+}
+
+template <typename Real>
+__device__ __forceinline__ void store_particle(const InjParams& ip, PArrays<Real> A, uint32_t p, const double (&r)[3],
+                                               const double (&v)[3], uint8_t lc) {
+    Real X = (Real)r[0], Y = (Real)r[1], Z = (Real)r[2];
+    // a uniform draw just below 1 can round up to 1 in FP32 (the -axis
+    // wall's coordinate is exactly 1 and stays)
+    if (ip.kind == INJ_EXOSPHERE || ip.axis != 0 || ip.pos_wall) X = X >= Real(1) ? below_one<Real>() : X;
+    if (ip.kind == INJ_EXOSPHERE || ip.axis != 1 || ip.pos_wall) Y = Y >= Real(1) ? below_one<Real>() : Y;
+    if (ip.kind == INJ_EXOSPHERE || ip.axis != 2 || ip.pos_wall) Z = Z >= Real(1) ? below_one<Real>() : Z;
+    A.x[p] = X; A.y[p] = Y; A.z[p] = Z;
+    A.vx[p] = (Real)v[0]; A.vy[p] = (Real)v[1]; A.vz[p] = (Real)v[2];
+    A.lc[p] = lc;
+}
+
 // The new particles of listed cell n: their number (WRITE = false), or
 // their state written to A[at..] (WRITE = true; indices >= cap are not
 // written and set `overflow`).  Positions in cell units, velocities in m/s.
@@ -126,46 +172,56 @@ __device__ uint32_t inject_cell(const Geom& g, const InjParams& ip, int n, doubl
     cell_ijk(g, n, i, j, k);
     uint32_t q = 0;
     for (int m = 0; m < N; ++m) {
-        RngStream st(ip.k0, ip.k1, (uint32_t)n, ip.step, (uint32_t)m);
         double r[3], v[3];
-        if (ip.kind == INJ_SOLAR_WIND) {
-            const int a = ip.axis;
-            const int l1 = a == 0 ? 1 : 0, l2 = a == 2 ? 1 : 2;
-            r[a] = ip.pos_wall ? 0.0 : 1.0;
-            r[l1] = st.uniform();
-            r[l2] = st.uniform();
-            const double vn = ip.vth * deriv_gauss(st, ip);
-            v[a] = ip.pos_wall ? -vn : vn;
-            v[l1] = ip.vth * st.gauss();
-            v[l2] = ip.vth * st.gauss();
-        } else {
-            const double dx = g.dx;
-            double rm[3];
-            for (int l = 0; l < 3; ++l) rm[l] = 0.5 * dx + (1.0 - ip.eps) * dx * (st.uniform() - 0.5);
-            const double gx = (g.x0 + i * dx) + rm[0], gy = (g.y0 + j * dx) + rm[1], gz = (g.z0 + k * dx) + rm[2];
-            if (gx * gx + gy * gy + gz * gz < ip.R2_exobase) continue;
-            if (!WRITE) { ++q; continue; }
-            for (int l = 0; l < 3; ++l) r[l] = rm[l] / dx;
-            for (int l = 0; l < 3; ++l) v[l] = ip.vth * st.gauss();
-        }
+        if (!gen_particle(g, ip, n, i, j, k, (uint32_t)m, WRITE, r, v)) continue;
         if (WRITE) {
             const uint32_t p = at + q;
-            if (p < cap) {
-                Real X = (Real)r[0], Y = (Real)r[1], Z = (Real)r[2];
-                // a uniform draw just below 1 can round up to 1 in FP32
-                if (ip.kind == INJ_EXOSPHERE || ip.axis != 0 || ip.pos_wall) X = X >= Real(1) ? below_one<Real>() : X;
-                if (ip.kind == INJ_EXOSPHERE || ip.axis != 1 || ip.pos_wall) Y = Y >= Real(1) ? below_one<Real>() : Y;
-                if (ip.kind == INJ_EXOSPHERE || ip.axis != 2 || ip.pos_wall) Z = Z >= Real(1) ? below_one<Real>() : Z;
-                A.x[p] = X; A.y[p] = Y; A.z[p] = Z;
-                A.vx[p] = (Real)v[0]; A.vy[p] = (Real)v[1]; A.vz[p] = (Real)v[2];
-                A.lc[p] = lc;
-            } else {
-                overflow = true;
-            }
+            if (p < cap) store_particle(ip, A, p, r, v, lc);
+            else overflow = true;
         }
         ++q;
     }
     return q;
+}
+
+// Solar wind: every listed cell owns `slots` = ceil(rate) fixed places past
+// the store's end (cell c: slots c*slots ..), so no count and no scan are
+// needed; one thread per place writes particle m of its cell if m < the
+// cell's probRound draw, else marks the place removed (x = -1, skipped by
+// every later pass).  The step's total goes to *total.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_inject_slots(Geom g, InjParams ip, int ncell, int slots,
+                                                      const int* __restrict__ cells,
+                                                      const uint32_t* __restrict__ np_ptr, PArrays<Real> A,
+                                                      uint32_t cap, unsigned long long* __restrict__ total,
+                                                      unsigned long long* __restrict__ counters) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    bool overflow = false, live = false;
+    if (t < ncell * slots) {
+        const int c = t / slots, m = t - c * slots;
+        const int n = cells[c];
+        RngStream cs(ip.k0, ip.k1, (uint32_t)n, ip.step, 0xFFFFFFFFu);
+        const int N = prob_round(ip.rate, cs.uniform());
+        const uint64_t p = (uint64_t)*np_ptr + (uint64_t)t;
+        if (p >= cap) {
+            overflow = m < N;
+        } else if (m < N) {
+            int i, j, k;
+            cell_ijk(g, n, i, j, k);
+            const uint8_t lc = (uint8_t)((i & (g.tx - 1)) + ((j & (g.ty - 1)) << g.ltx) +
+                                         ((k & (g.tz - 1)) << (g.ltx + g.lty)));
+            double r[3], v[3];
+            gen_particle(g, ip, n, i, j, k, (uint32_t)m, true, r, v);
+            store_particle(ip, A, (uint32_t)p, r, v, lc);
+            live = true;
+        } else {
+            A.x[p] = Real(-1);
+        }
+    }
+    // integer sums: the same total in any order
+    const unsigned long long b = __ballot(live);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(total, (unsigned long long)__popcll(b));
+    count_event(counters, C_RANK_OVERFLOW, overflow);
 }
 
 // Count of new particles per listed cell (rate: per listed cell, or the
