@@ -230,6 +230,31 @@ int rh_diagnostics(rh_ctx* ctx, int32_t species, double out[5]);
 int rh_counters(rh_ctx* ctx, double out[RH_N_COUNTERS]);
 int rh_synchronize(rh_ctx* ctx);
 
+/* ---- logs (src/diagnostics.h:187-816, src/user.cpp:2147-2256) ----------- */
+
+/* The values of one line of the reference's log files at simulation time t
+ * (logWriteParticleField): field[41] = the columns of field.log, pop[17 *
+ * nspecies] = the columns of each pop*.log, all in FP64 (particle sums over
+ * live particles including injected ones not yet pushed; field sums over
+ * cells whose 26 neighbours exist, inner-boundary cells only with
+ * include_inner_cells, as Hybrid::includeInnerCellsInFieldLog).  Rates divide
+ * the counters gathered since the previous line (or rh_log_begin) by the time
+ * since then; the counters then restart.  Column 31 repeats the node-Ue
+ * counter, as the reference does (src/diagnostics.h:589). */
+int rh_log_values(rh_ctx* ctx, double t, int32_t include_inner_cells, double field[41], double* pop);
+/* Restart the log counters at time t (Hybrid::logCounterTimeStart); a new
+ * context starts them at t = 0. */
+int rh_log_begin(rh_ctx* ctx, double t);
+/* Create field.log and pop<NNN>_<name>.log in `directory` with the
+ * reference's headers (src/user.cpp:2147-2231), `precision` digits in
+ * scientific notation with signs (Hybrid.log_precision, default 10). */
+int rh_log_open(rh_ctx* ctx, const char* directory, const char* const* species_names, int32_t precision,
+                int32_t include_inner_cells, double max_B_stop);
+/* Append one line to every log file at time t (rh_log_values); returns
+ * RH_ERR_STATE after writing when max(|faceB|) exceeds max_B_stop > 0, where
+ * the reference stops the run (Hybrid::maxBStopRun). */
+int rh_log_write(rh_ctx* ctx, double t);
+
 /* ---- measurement -------------------------------------------------------- */
 
 /* Per-kernel timing with HIP events on the context's stream (the profile::

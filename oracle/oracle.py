@@ -107,6 +107,8 @@ def lib():
         L.orc_timestep.argtypes = [vp]
         L.orc_injected.restype = C.c_int64
         L.orc_injected.argtypes = [vp, C.c_int]
+        L.orc_log_begin.argtypes = [vp, C.c_double]
+        L.orc_log_values.argtypes = [vp, C.c_double, C.c_int, vp, vp]
         L.orc_counter.restype = C.c_double
         L.orc_counter.argtypes = [vp, C.c_int]
         L.orc_get_fields.argtypes = [vp, C.c_int64, vp, vp, vp, vp]
@@ -280,6 +282,17 @@ class Oracle:
     def set_timestep(self, t: int) -> None: lib().orc_set_timestep(self._h, int(t))
     def timestep(self) -> int: return int(lib().orc_timestep(self._h))
     def injected(self, s: int) -> int: return int(lib().orc_injected(self._h, s))
+
+    def log_begin(self, t: float) -> None:
+        lib().orc_log_begin(self._h, float(t))
+
+    def log_values(self, t: float, include_inner: bool = False):
+        """(field[41], pop[nspecies][17]) of one log line at time t; the log
+        counters restart."""
+        field = np.zeros(41)
+        pop = np.zeros((self.nspecies, 17))
+        lib().orc_log_values(self._h, float(t), int(include_inner), _ptr(field), _ptr(pop))
+        return field, pop
 
     def set_exterior_order_reversed(self, r: bool) -> None:
         lib().orc_set_exterior_order_reversed(self._h, int(r))

@@ -65,6 +65,13 @@ struct orc_sim {
         double R2_exobase;
         int64_t injected;
     } inj[MAX_SPECIES];
+    /* log counters per species (Hybrid::logCounterParticle*): escape, impact
+     * and inject sums of w and w*v^2, injected macroparticles, maxVi clamps */
+    double esc_w[MAX_SPECIES], esc_wv2[MAX_SPECIES], imp_w[MAX_SPECIES], imp_wv2[MAX_SPECIES];
+    double inj_w[MAX_SPECIES], inj_wv2[MAX_SPECIES], inj_macro[MAX_SPECIES], maxvi[MAX_SPECIES];
+    /* counter values and time at the last log line */
+    double log_t0;
+    double log_cnt0[ORC_N_COUNTERS];
 };
 
 static const int FIELD_DIM[ORC_N_FIELDS] = {3, 3, 3, 3, 3, 3, 1, 1, 3, 3, 3, 3, 3, 1};
@@ -306,6 +313,7 @@ void orc_push(orc_sim* s, int sp) {
         }
     }
     s->counters[ORC_CNT_MAX_VI] += nvi;
+    s->maxvi[sp] += nvi;
 }
 
 /* -------------------------------------------------------- migration */
@@ -500,7 +508,14 @@ void orc_apply_boundary_conditions(orc_sim* s, int sp) {
         int64_t b = start0, e = cs[n + 1];
         start0 = e;
         cs[n] = out;
-        if (s->nflags[n] != ALL_NBRS) continue; /* exterior: delete all */
+        if (s->nflags[n] != ALL_NBRS) { /* exterior: delete all, counted as escapes */
+            for (int64_t p = b; p < e; ++p) {
+                const double* q = st + 7 * p;
+                s->esc_w[sp] += q[6];
+                s->esc_wv2[sp] += q[6] * (sqr(q[3]) + sqr(q[4]) + sqr(q[5]));
+            }
+            continue;
+        }
         int ci, cj, ck;
         decompose(&s->g, n, &ci, &cj, &ck);
         const double xb = s->g.x0 + ci * s->g.dx, yb = s->g.y0 + cj * s->g.dx, zb = s->g.z0 + ck * s->g.dx;
@@ -508,7 +523,11 @@ void orc_apply_boundary_conditions(orc_sim* s, int sp) {
             const double* q = st + 7 * p;
             if (s->flag[ORC_INNER_PARTICLE][n]) {
                 const double r2 = sqr(xb + q[0]) + sqr(yb + q[1]) + sqr(zb + q[2]);
-                if (r2 < S->R2_obstacle) continue;
+                if (r2 < S->R2_obstacle) { /* impact */
+                    s->imp_w[sp] += q[6];
+                    s->imp_wv2[sp] += q[6] * (sqr(q[3]) + sqr(q[4]) + sqr(q[5]));
+                    continue;
+                }
             }
             if (out != p) memmove(st + 7 * out, q, 7 * sizeof(double));
             ++out;
@@ -1237,6 +1256,12 @@ void orc_inject(orc_sim* s, int sp) {
             inject_cell(s, sp, n, step, tmp, cnt[n]);
             memcpy(nst + 7 * out, tmp, sizeof(double) * 7 * (size_t)cnt[n]);
             out += cnt[n];
+            for (int q = 0; q < cnt[n]; ++q) { /* src/particle_injector.cpp:573-575 */
+                const double* o = tmp + 7 * q;
+                s->inj_w[sp] += o[6];
+                s->inj_macro[sp] += 1.0;
+                s->inj_wv2[sp] += o[6] * (sqr(o[3]) + sqr(o[4]) + sqr(o[5]));
+            }
         }
     }
     cs[nc] = out;
@@ -1265,6 +1290,153 @@ void orc_step(orc_sim* s) {
     for (int sp = 0; sp < s->nspecies; ++sp) orc_apply_boundary_conditions(s, sp);
     for (int sp = 0; sp < s->nspecies; ++sp) orc_inject(s, sp);
     orc_propagate_B(s);
+}
+
+/* ------------------------------------------------------------ logs */
+
+/* Constants of the logs (CODATA; Corsair's table is not in the reference tree). */
+#define ORC_MP   1.672621716e-27
+#define ORC_ME   9.10938215e-31
+#define ORC_EPS0 8.854187817e-12
+
+void orc_log_begin(orc_sim* s, double t) {
+    s->log_t0 = t;
+    memcpy(s->log_cnt0, s->counters, sizeof(s->counters));
+    for (int sp = 0; sp < s->nspecies; ++sp) {
+        s->esc_w[sp] = s->esc_wv2[sp] = s->imp_w[sp] = s->imp_wv2[sp] = 0.0;
+        s->inj_w[sp] = s->inj_wv2[sp] = s->inj_macro[sp] = s->maxvi[sp] = 0.0;
+    }
+}
+
+/* logWriteParticleField (src/diagnostics.h:407-813) with logCalcParticle
+ * (:187-251) and logCalcField (:254-404): the values of one line of each
+ * pop log (17 per species) and of field.log (41), then the counters restart
+ * (:778-799).  Column 31 repeats the node-Ue counter, as :589 does. */
+void orc_log_values(orc_sim* s, double t, int include_inner, double field[41], double* pop) {
+    const int nc = s->nc;
+    const double dx = s->g.dx, dV = dx * dx * dx, dt = s->p.dt;
+    const double Dt = t - s->log_t0;
+    double* rhoM = (double*)calloc((size_t)nc, sizeof(double));
+    for (int sp = 0; sp < s->nspecies; ++sp) {
+        double Nr = 0, sx = 0, sy = 0, sz = 0, sv = 0, sv2 = 0, mx = 0;
+        for (int n = 0; n < nc; ++n)
+            for (int64_t p = s->cstart[sp][n]; p < s->cstart[sp][n + 1]; ++p) {
+                const double* q = s->pst[sp] + 7 * p;
+                const double w = q[6];
+                Nr += w;
+                sx += w * q[3]; sy += w * q[4]; sz += w * q[5];
+                const double v2 = sqr(q[3]) + sqr(q[4]) + sqr(q[5]), v = sqrt(v2);
+                sv += w * v; sv2 += w * v2;
+                if (v > mx) mx = v;
+                rhoM[n] += w * s->sp[sp].m;
+            }
+        const double m = s->sp[sp].m;
+        double* o = pop + 17 * sp;
+        o[0] = t; o[1] = Nr; o[2] = (double)s->np[sp];
+        if (Nr > 0) { o[3] = sx / Nr; o[4] = sy / Nr; o[5] = sz / Nr; o[6] = sv / Nr; }
+        else { o[3] = o[4] = o[5] = o[6] = 0.0; }
+        o[7] = 0.5 * m * sv2; o[8] = mx;
+        if (Dt > 0) {
+            o[9] = s->esc_w[sp] / Dt; o[10] = s->imp_w[sp] / Dt; o[11] = s->inj_w[sp] / Dt;
+            o[12] = s->inj_macro[sp] / Dt * dt;
+            o[13] = 0.5 * m * s->esc_wv2[sp] / Dt; o[14] = 0.5 * m * s->imp_wv2[sp] / Dt;
+            o[15] = 0.5 * m * s->inj_wv2[sp] / Dt; o[16] = s->maxvi[sp] / Dt * dt;
+        } else {
+            for (int q = 9; q < 17; ++q) o[q] = 0.0;
+        }
+    }
+    const double minRhoM = s->p.min_rho * ORC_MP / ORC_E;
+    for (int n = 0; n < nc; ++n) {
+        rhoM[n] /= dV;
+        if (rhoM[n] < minRhoM) rhoM[n] = minRhoM;
+    }
+    double N = 0, sB[3] = {0, 0, 0}, sBt = 0, mB = 0, sDiv = 0, mDiv = 0, mDivPerB = 0, sB2 = 0;
+    double sJ[3] = {0, 0, 0}, sJt = 0, mJ = 0, sEp[3] = {0, 0, 0}, sEpt = 0, mEp = 0, sEp2 = 0;
+    double sE[3] = {0, 0, 0}, sEt = 0, mE = 0, sE2 = 0, sCE2 = 0;
+    double minDe = DBL_MAX, maxDe = -DBL_MAX, minDi = DBL_MAX, maxDi = -DBL_MAX, minTL = DBL_MAX, maxVA = 0, maxUe = 0;
+    const double* fB = s->f[ORC_FACE_B];
+    const double* Ji = s->f[ORC_CELL_JI];
+    const double* Ep = s->f[ORC_CELL_EP];
+    const double* nE = s->f[ORC_NODE_E];
+    const double* Ue = s->f[ORC_CELL_UE];
+    const double* rq = s->f[ORC_CELL_RHOQ];
+    for (int n = 0; n < nc; ++n) {
+        if (s->nflags[n] != ALL_NBRS) continue;
+        if (!include_inner && s->flag[ORC_INNER_FIELD][n]) continue;
+        const int nxm = nbr(&s->g, n, -1, 0, 0), nym = nbr(&s->g, n, 0, -1, 0), nzm = nbr(&s->g, n, 0, 0, -1);
+        double divB = ((fB[3 * n + 0] - fB[3 * nxm + 0]) + (fB[3 * n + 1] - fB[3 * nym + 1]) +
+                       (fB[3 * n + 2] - fB[3 * nzm + 2])) / dx;
+        divB = fabs(divB);
+        const double Bx = 0.5 * (fB[3 * n + 0] + fB[3 * nxm + 0]);
+        const double By = 0.5 * (fB[3 * n + 1] + fB[3 * nym + 1]);
+        const double Bz = 0.5 * (fB[3 * n + 2] + fB[3 * nzm + 2]);
+        const double B2 = sqr(Bx) + sqr(By) + sqr(Bz), Bt = sqrt(B2);
+        const double dpb = Bt > 0.0 ? divB / Bt : 0.0;
+        sB[0] += Bx; sB[1] += By; sB[2] += Bz; sBt += Bt; sDiv += divB; sB2 += B2;
+        if (Bt > mB) mB = Bt;
+        if (fabs(divB) > fabs(mDiv)) mDiv = divB;
+        if (fabs(dpb) > fabs(mDivPerB)) mDivPerB = dpb;
+        const double Jt = sqrt(sqr(Ji[3 * n]) + sqr(Ji[3 * n + 1]) + sqr(Ji[3 * n + 2]));
+        for (int l = 0; l < 3; ++l) sJ[l] += Ji[3 * n + l];
+        sJt += Jt;
+        if (Jt > mJ) mJ = Jt;
+        const double Ep2 = sqr(Ep[3 * n]) + sqr(Ep[3 * n + 1]) + sqr(Ep[3 * n + 2]), Ept = sqrt(Ep2);
+        for (int l = 0; l < 3; ++l) sEp[l] += Ep[3 * n + l];
+        sEpt += Ept; sEp2 += Ep2;
+        if (Ept > mEp) mEp = Ept;
+        const double E2 = sqr(nE[3 * n]) + sqr(nE[3 * n + 1]) + sqr(nE[3 * n + 2]), Et = sqrt(E2);
+        for (int l = 0; l < 3; ++l) sE[l] += nE[3 * n + l];
+        sEt += Et; sE2 += E2;
+        if (Et > mE) mE = Et;
+        const double ux = Ue[3 * n], uy = Ue[3 * n + 1], uz = Ue[3 * n + 2];
+        const double cEx = -(uy * Bz - uz * By), cEy = -(uz * Bx - ux * Bz), cEz = -(ux * By - uy * Bx);
+        sCE2 += sqr(cEx) + sqr(cEy) + sqr(cEz);
+        const double Uet = sqrt(sqr(ux) + sqr(uy) + sqr(uz));
+        if (rq[n] > 0.0) {
+            const double de = sqrt(ORC_ME / (ORC_MU0 * ORC_E * rq[n]));
+            if (de > maxDe) maxDe = de;
+            if (de < minDe) minDe = de;
+        }
+        if (Bt > 0.0) {
+            const double tL = 2.0 * M_PI * ORC_MP / (ORC_E * Bt);
+            if (tL < minTL) minTL = tL;
+        }
+        if (rhoM[n] > 0.0) {
+            const double di = sqrt(sqr(ORC_MP) / (ORC_MU0 * sqr(ORC_E) * rhoM[n]));
+            const double vA = Bt / sqrt(ORC_MU0 * rhoM[n]);
+            if (di > maxDi) maxDi = di;
+            if (di < minDi) minDi = di;
+            if (vA > maxVA) maxVA = vA;
+        }
+        if (Uet > maxUe) maxUe = Uet;
+        N += 1.0;
+    }
+    free(rhoM);
+    field[0] = t;
+    for (int l = 0; l < 3; ++l) field[1 + l] = N > 0 ? sB[l] / N : 0.0;
+    field[4] = N > 0 ? sBt / N : 0.0;
+    field[5] = mB;
+    field[6] = N > 0 ? sDiv / N : 0.0;
+    field[7] = mDiv; field[8] = dx * mDivPerB; field[9] = sB2 * dV / (2.0 * ORC_MU0);
+    for (int l = 0; l < 3; ++l) field[10 + l] = N > 0 ? sJ[l] / N : 0.0;
+    field[13] = N > 0 ? sJt / N : 0.0;
+    field[14] = mJ;
+    for (int l = 0; l < 3; ++l) field[15 + l] = N > 0 ? sEp[l] / N : 0.0;
+    field[18] = N > 0 ? sEpt / N : 0.0;
+    field[19] = mEp; field[20] = sEp2 * dV * ORC_EPS0 / 2.0;
+    for (int l = 0; l < 3; ++l) field[21 + l] = N > 0 ? sE[l] / N : 0.0;
+    field[24] = N > 0 ? sEt / N : 0.0;
+    field[25] = mE; field[26] = sE2 * dV * ORC_EPS0 / 2.0; field[27] = sCE2 * dV * ORC_EPS0 / 2.0;
+    const double* c = s->counters;
+    const double* c0 = s->log_cnt0;
+    const double cnt[6] = {c[ORC_CNT_MAX_CELL_UE] - c0[ORC_CNT_MAX_CELL_UE], c[ORC_CNT_MAX_NODE_UE] - c0[ORC_CNT_MAX_NODE_UE],
+                           c[ORC_CNT_MAX_NODE_UE] - c0[ORC_CNT_MAX_NODE_UE], c[ORC_CNT_MAX_E] - c0[ORC_CNT_MAX_E],
+                           c[ORC_CNT_MIN_CELL_RHO] - c0[ORC_CNT_MIN_CELL_RHO],
+                           c[ORC_CNT_MIN_NODE_RHO] - c0[ORC_CNT_MIN_NODE_RHO]};
+    for (int q = 0; q < 6; ++q) field[28 + q] = Dt > 0 ? cnt[q] / Dt * dt : 0.0;
+    field[34] = minDe; field[35] = maxDe; field[36] = minDi; field[37] = maxDi;
+    field[38] = minTL; field[39] = maxVA; field[40] = maxUe;
+    orc_log_begin(s, t);
 }
 
 /* logCalcParticle sums (src/diagnostics.h:209-220). */

@@ -17,6 +17,7 @@
  *   conservation sums        src/diagnostics.h:209-220
  *   solar-wind injection     src/particle_injector.cpp:41-114,380-700
  *   exosphere injection      src/particle_injector.cpp:1489-1840
+ *   particle and field logs  src/diagnostics.h:187-816
  * Cell-to-cell particle migration lives in the un-vendored Corsair
  * ParticleListSkeleton; it is re-specified here (see orc_migrate).
  */
@@ -164,6 +165,13 @@ void    orc_set_timestep(orc_sim* s, int64_t t);
 int64_t orc_timestep(const orc_sim* s);
 /* macroparticles injected so far per species */
 int64_t orc_injected(const orc_sim* s, int species);
+
+/* ---- logs (src/diagnostics.h:187-816, src/user.cpp:2147-2256) ----------- */
+/* Start the log counters at time t (logCounterTimeStart). */
+void    orc_log_begin(orc_sim* s, double t);
+/* One line of every pop log (17 values per species, into pop) and of
+ * field.log (41 values) at time t; the log counters restart at t. */
+void    orc_log_values(orc_sim* s, double t, int include_inner, double field[41], double* pop);
 
 /* Field gather for one point (cell-local metres) as the push sees it. */
 void orc_get_fields(orc_sim* s, int64_t cell, const double r[3], double B[3], double Ue[3], double Ep[3]);

@@ -4,6 +4,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
+#include <fstream>
+#include <iomanip>
+#include <sstream>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -18,6 +22,7 @@
 #include "rh_fields.hpp"
 #include "rh_geom.hpp"
 #include "rh_inject.hpp"
+#include "rh_log.hpp"
 #include "rh_particles.hpp"
 
 namespace rh {
@@ -109,6 +114,11 @@ public:
     virtual void set_timestep(int64_t t) = 0;
     virtual int64_t timestep() = 0;
     virtual int64_t injected(int s) = 0;
+    virtual void log_begin(double t) = 0;
+    virtual void log_values(double t, int include_inner, double* field, double* pop) = 0;
+    virtual void log_open(const char* dir, const char* const* names, int precision, int include_inner,
+                          double max_B_stop) = 0;
+    virtual void log_write(double t) = 0;
 };
 
 // Boltzmann constant of vth = sqrt(kB T / m) (CODATA 2002, the same table as
@@ -202,6 +212,16 @@ public:
         DevBuf<uint32_t> inj_cbeg, inj_off;
         DevBuf<unsigned long long> inj_total;
         bool inj_pending = false;  // injected particles wait past the store's end
+        // log counters: the push's own counters (maxVi per species), per
+        // boundary cell (escapes, sum v^2, impacts, sum v^2), per injection
+        // place sum v^2; host carries of the accumulators when their layout
+        // changes; values at the last log line
+        DevBuf<unsigned long long> pcount;
+        DevBuf<double> bc_acc, inj_acc;
+        double carry_bc[4] = {0, 0, 0, 0};
+        double carry_inj_v2 = 0;
+        double log_maxvi0 = 0;
+        int64_t log_inj0 = 0;
         InjAppendix appendix() const {
             return inj_pending ? InjAppendix{inj_cbeg.p, inj_off.p} : InjAppendix{nullptr, nullptr};
         }
@@ -270,7 +290,7 @@ public:
         rebuild_bc_cells();
         dcount.alloc((size_t)C_N * COUNTER_SLOTS);
         RH_HIP(hipMemsetAsync(dcount.p, 0, sizeof(unsigned long long) * C_N * COUNTER_SLOTS, stream));
-        diag_partial.alloc(DIAG_BLOCKS * 5);
+        diag_partial.alloc(DIAG_BLOCKS * DIAG_Q);
         {
             std::vector<uint16_t> hl((size_t)g.next * 64, 0);
             std::vector<uint8_t> hc((size_t)g.next, 0);
@@ -439,6 +459,10 @@ public:
         RH_HIP(hipMemsetAsync(s->sub.p, 0, sizeof(uint32_t) * ((size_t)g.nkeys + 1), stream));
         s->cnt_tile.alloc((size_t)g.ntiles * g.extk);
         s->partial.alloc((size_t)g.ntiles * g.next * 4);
+        s->pcount.alloc((size_t)C_N * COUNTER_SLOTS);
+        RH_HIP(hipMemsetAsync(s->pcount.p, 0, sizeof(unsigned long long) * C_N * COUNTER_SLOTS, stream));
+        s->bc_acc.alloc((size_t)std::max(n_bc, 1) * 4);
+        fill(s->bc_acc.p, 0.0, s->bc_acc.n);
         ensure_capacity(*s, (size_t)std::max<int64_t>(d->capacity, 0));
         sp.push_back(std::move(s));
         RH_HIP(hipStreamSynchronize(stream));
@@ -482,7 +506,13 @@ public:
         RH_HIP(hipStreamSynchronize(stream));
         if (id == RH_INNER_PARTICLE) {
             for (int n = 0; n < g.nc; ++n) h_inner_particle[n] = data[n] ? 1 : 0;
+            for (auto& sp_ : sp) carry_acc(*sp_, true, false);
             rebuild_bc_cells();
+            for (auto& sp_ : sp) {
+                sp_->bc_acc.alloc((size_t)std::max(n_bc, 1) * 4);
+                fill(sp_->bc_acc.p, 0.0, sp_->bc_acc.n);
+            }
+            RH_HIP(hipStreamSynchronize(stream));
         }
     }
 
@@ -539,7 +569,7 @@ public:
     // marked, until the next sort; injected ones waiting for their first
     // push are included)
     int64_t num_particles(int s) override {
-        double t[5];
+        double t[DIAG_Q];
         diag_raw(s, t);
         return (int64_t)t[0];
     }
@@ -602,7 +632,7 @@ public:
         launch("push", k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, pp,
                (const uint32_t*)pf_tile.p,
                S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.appendix(), S_.cnt_tile.p,
-               (const uint32_t*)S_.sub_new.p, dcount.p, kbits, 0, (uint32_t)S_.cap);
+               (const uint32_t*)S_.sub_new.p, S_.pcount.p, kbits, 0, (uint32_t)S_.cap);
         S_.pp = pp;
         S_.pushed = true;
     }
@@ -617,7 +647,7 @@ public:
         prepare_push_fields();
         launch("sort_still", k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, pp,
                (const uint32_t*)pf_tile.p, S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.appendix(), S_.cnt_tile.p,
-               (const uint32_t*)S_.sub_new.p, dcount.p, kbits, 0, (uint32_t)S_.cap);
+               (const uint32_t*)S_.sub_new.p, S_.pcount.p, kbits, 0, (uint32_t)S_.cap);
         S_.pp = pp;
         S_.pushed = true;
         finish_sort(S_, "sort_still");
@@ -649,7 +679,7 @@ public:
         launch(move_name, k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, S_.pp,
                (const uint32_t*)pf_tile.p,
                S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.appendix(), S_.cnt_tile.p,
-               (const uint32_t*)S_.sub_new.p, dcount.p, kbits, 1, (uint32_t)S_.cap);
+               (const uint32_t*)S_.sub_new.p, S_.pcount.p, kbits, 1, (uint32_t)S_.cap);
         S_.swap_stores();
         S_.pushed = false;
         S_.inj_pending = false;
@@ -752,7 +782,7 @@ public:
         if (S_.pushed) throw StateError("boundary conditions between propagate and wait_particle_sends");
         flush_injected(S_);
         launch("boundary", k_boundary<Real>, cdiv(n_bc, BC_CELLS_PER_BLOCK), 256, 0, g, (const int*)bc_cells.p, n_bc,
-               (const uint32_t*)S_.sub.p, S_.A(), S_.d.R2_obstacle);
+               (const uint32_t*)S_.sub.p, S_.A(), S_.d.R2_obstacle, S_.bc_acc.p);
     }
 
     void neumann_cell(int id, int dim) {
@@ -838,8 +868,13 @@ public:
     // report it.
     void check_overflow() {
         unsigned long long h = 0;
-        RH_HIP(hipMemcpyAsync(&h, dcount.p + C_RANK_OVERFLOW, sizeof(h), hipMemcpyDeviceToHost, stream));
-        RH_HIP(hipStreamSynchronize(stream));
+        for (size_t b = 0; b <= sp.size(); ++b) {
+            unsigned long long x = 0;
+            const unsigned long long* src = b == 0 ? dcount.p : sp[b - 1]->pcount.p;
+            RH_HIP(hipMemcpyAsync(&x, src + C_RANK_OVERFLOW, sizeof(x), hipMemcpyDeviceToHost, stream));
+            RH_HIP(hipStreamSynchronize(stream));
+            h += x;
+        }
         if (h)
             throw CapError("particle sort overflow (a subcell holds more than 65535 particles of one species, "
                            "or the particle store is full)");
@@ -855,6 +890,7 @@ public:
         Species& S_ = S(s);
         if (S_.pushed) throw StateError("set_injector between propagate and wait_particle_sends");
         flush_injected(S_);
+        carry_acc(S_, false, true);
         S_.inj = InjParams{};
         S_.inj_ncell = 0;
         S_.inj_slots = 0;
@@ -953,6 +989,8 @@ public:
             for (size_t q = 0; q <= hc.size(); ++q) off[q] = (uint32_t)(q * (size_t)slots);
             RH_HIP(hipMemcpyAsync(S_.inj_off.p, off.data(), off.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
         }
+        S_.inj_acc.alloc(ip.kind == INJ_SOLAR_WIND ? std::max<size_t>(hc.size() * (size_t)slots, 1) : nl);
+        fill(S_.inj_acc.p, 0.0, S_.inj_acc.n);
         RH_HIP(hipStreamSynchronize(stream));
         S_.inj = ip;
         S_.inj_ncell = (int)hc.size();
@@ -972,7 +1010,8 @@ public:
             if (S_.inj_slots > 0)
                 launch("inject_slots", k_inject_slots<Real>, cdiv((size_t)S_.inj_ncell * S_.inj_slots, 256), 256, 0, g,
                        ip, S_.inj_ncell, S_.inj_slots, (const int*)S_.inj_cells.p,
-                       (const uint32_t*)(S_.sub.p + g.nkeys), S_.A(), (uint32_t)S_.cap, S_.inj_total.p, dcount.p);
+                       (const uint32_t*)(S_.sub.p + g.nkeys), S_.A(), (uint32_t)S_.cap, S_.inj_total.p, dcount.p,
+                       S_.inj_acc.p);
             S_.inj_pending = S_.inj_slots > 0;
             return;
         }
@@ -983,8 +1022,228 @@ public:
         scan(S_.inj_off.p, (uint32_t)S_.inj_ncell, S_.inj_off.p, 0);
         launch("inject_fill", k_inject_fill<Real>, nb, 256, 0, g, ip, S_.inj_ncell, (const int*)S_.inj_cells.p, rate,
                (const uint32_t*)S_.inj_off.p, (const uint32_t*)(S_.sub.p + g.nkeys), S_.A(), (uint32_t)S_.cap,
-               S_.inj_total.p, dcount.p);
+               S_.inj_total.p, dcount.p, S_.inj_acc.p);
         S_.inj_pending = true;
+    }
+
+    // ---- logs (logWriteParticleField, src/diagnostics.h:407-813) -------------
+
+    double log_t0 = 0.0;                 // Hybrid::logCounterTimeStart
+    unsigned long long log_cnt0[C_N] = {};
+    DevBuf<double> log_rhoM, log_partial;
+    std::ofstream log_field;
+    std::vector<std::unique_ptr<std::ofstream>> log_pop;
+    int log_precision = 10, log_inner = 0;
+    double log_max_B = 0.0;
+
+    // FP64 column sums of a rows x cols accumulator, in a fixed order.
+    void sum_cols(const double* a, size_t rows, int cols, double* out) {
+        if (!log_partial.p || log_partial.n < (size_t)LOG_BLOCKS * LQ_N_Q) log_partial.alloc((size_t)LOG_BLOCKS * LQ_N_Q);
+        for (int c = 0; c < cols; ++c) out[c] = 0.0;
+        if (rows == 0) return;
+        launch("log_sum", k_sum_cols, LOG_BLOCKS, 256, 0, a, rows, cols, log_partial.p);
+        std::vector<double> h((size_t)LOG_BLOCKS * cols);
+        RH_HIP(hipMemcpyAsync(h.data(), log_partial.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
+        RH_HIP(hipStreamSynchronize(stream));
+        for (int b = 0; b < LOG_BLOCKS; ++b)
+            for (int c = 0; c < cols; ++c) out[c] += h[(size_t)b * cols + c];
+    }
+
+    // Move the boundary / injection accumulators into the host carries
+    // (before their layout changes) and clear them.
+    void carry_acc(Species& S_, bool bc, bool inj) {
+        if (bc && S_.bc_acc.p) {
+            double t[4];
+            sum_cols(S_.bc_acc.p, (size_t)n_bc, 4, t);
+            for (int c = 0; c < 4; ++c) S_.carry_bc[c] += t[c];
+            fill(S_.bc_acc.p, 0.0, S_.bc_acc.n);
+        }
+        if (inj && S_.inj_acc.p) {
+            double t = 0.0;
+            sum_cols(S_.inj_acc.p, S_.inj_acc.n, 1, &t);
+            S_.carry_inj_v2 += t;
+            fill(S_.inj_acc.p, 0.0, S_.inj_acc.n);
+        }
+    }
+
+    void log_begin(double t) override {
+        log_t0 = t;
+        counter_totals(log_cnt0);
+        for (auto& sp_ : sp) {
+            Species& S_ = *sp_;
+            if (S_.bc_acc.p) fill(S_.bc_acc.p, 0.0, S_.bc_acc.n);
+            if (S_.inj_acc.p) fill(S_.inj_acc.p, 0.0, S_.inj_acc.n);
+            for (int c = 0; c < 4; ++c) S_.carry_bc[c] = 0.0;
+            S_.carry_inj_v2 = 0.0;
+            S_.log_maxvi0 = (double)species_counter(S_, C_MAX_VI);
+            S_.log_inj0 = injected_count(S_);
+        }
+        RH_HIP(hipStreamSynchronize(stream));
+    }
+
+    int64_t injected_count(Species& S_) {
+        if (!S_.inj_total.p) return 0;
+        unsigned long long h = 0;
+        RH_HIP(hipMemcpyAsync(&h, S_.inj_total.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+        RH_HIP(hipStreamSynchronize(stream));
+        return (int64_t)h;
+    }
+
+    // One line of every pop log (17 values per species) and of field.log
+    // (41 values, src/user.cpp:2161-2231 list the columns); the counters
+    // restart at t.  Column 31 repeats the node-Ue counter, as
+    // src/diagnostics.h:589 does.
+    void log_values(double t, int include_inner, double* field, double* pop) override {
+        for (auto& sp_ : sp)
+            if (sp_->pushed) throw StateError("log between propagate and wait_particle_sends");
+        for (auto& sp_ : sp) flush_injected(*sp_);
+        const double dt = P.dt, dV = g.dx * g.dx * g.dx, Dt = t - log_t0;
+        if (!log_rhoM.p) log_rhoM.alloc((size_t)g.nc);
+        if (sp.empty()) fill(log_rhoM.p, 0.0, log_rhoM.n);
+        unsigned long long cnt[C_N];
+        counter_totals(cnt);
+        for (size_t s = 0; s < sp.size(); ++s) {
+            Species& S_ = *sp[s];
+            double d[DIAG_Q];
+            diag_raw((int)s, d);
+            launch("log_mass", k_cell_mass<Real>, cdiv(g.nc, 256), 256, 0, g, (const uint32_t*)S_.sub.p, S_.A(),
+                   S_.d.m * S_.d.w, log_rhoM.p, s == 0 ? 1 : 0);
+            double bc[4], v2inj = 0.0;
+            sum_cols(S_.bc_acc.p, (size_t)n_bc, 4, bc);
+            for (int c = 0; c < 4; ++c) bc[c] += S_.carry_bc[c];
+            if (S_.inj_acc.p) sum_cols(S_.inj_acc.p, S_.inj_acc.n, 1, &v2inj);
+            v2inj += S_.carry_inj_v2;
+            const double ninj = (double)(injected_count(S_) - S_.log_inj0);
+            const double nvi = (double)species_counter(S_, C_MAX_VI) - S_.log_maxvi0;
+            const double w = S_.d.w, m = S_.d.m, Nr = d[0] * w;
+            double* o = pop + 17 * s;
+            o[0] = t; o[1] = Nr; o[2] = d[0];
+            if (Nr > 0) { o[3] = d[1] * w / Nr; o[4] = d[2] * w / Nr; o[5] = d[3] * w / Nr; o[6] = d[4] * w / Nr; }
+            else { o[3] = o[4] = o[5] = o[6] = 0.0; }
+            o[7] = 0.5 * m * (d[5] * w); o[8] = d[6];
+            if (Dt > 0) {
+                o[9] = bc[0] * w / Dt; o[10] = bc[2] * w / Dt; o[11] = ninj * w / Dt; o[12] = ninj / Dt * dt;
+                o[13] = 0.5 * m * (bc[1] * w) / Dt; o[14] = 0.5 * m * (bc[3] * w) / Dt;
+                o[15] = 0.5 * m * (v2inj * w) / Dt; o[16] = nvi / Dt * dt;
+            } else {
+                for (int q = 9; q < 17; ++q) o[q] = 0.0;
+            }
+        }
+        if (!log_partial.p || log_partial.n < (size_t)LOG_BLOCKS * LQ_N_Q) log_partial.alloc((size_t)LOG_BLOCKS * LQ_N_Q);
+        LogFields<Real> lf{v3(RH_FACE_B), v3(RH_CELL_JI), v3(RH_CELL_EP), v3(RH_NODE_E), v3(RH_CELL_UE),
+                           fld[RH_CELL_RHOQ].p, flg[RH_INNER_FIELD].p, log_rhoM.p};
+        const double min_rhoM = P.min_rho * LOG_MP / LOG_E;
+        launch("log_field", k_field_log<Real>, LOG_BLOCKS, 256, 0, g, lf, include_inner, min_rhoM, log_partial.p);
+        std::vector<double> h((size_t)LOG_BLOCKS * LQ_N_Q);
+        RH_HIP(hipMemcpyAsync(h.data(), log_partial.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
+        RH_HIP(hipStreamSynchronize(stream));
+        double v[LQ_N_Q];
+        for (int q = 0; q < LQ_N_Q; ++q) v[q] = h[q];
+        for (int b = 1; b < LOG_BLOCKS; ++b)
+            for (int q = 0; q < LQ_N_Q; ++q) {
+                const double x = h[(size_t)b * LQ_N_Q + q];
+                v[q] = q < LQ_NSUM ? v[q] + x : (q < LQ_NMAX_END ? std::max(v[q], x) : std::min(v[q], x));
+            }
+        const double N = v[LQ_N];
+        auto avg = [&](int q) { return N > 0 ? v[q] / N : 0.0; };
+        field[0] = t;
+        field[1] = avg(LQ_BX); field[2] = avg(LQ_BY); field[3] = avg(LQ_BZ); field[4] = avg(LQ_BT);
+        field[5] = v[LQ_MAX_B]; field[6] = avg(LQ_DIV); field[7] = v[LQ_MAX_DIV];
+        field[8] = g.dx * v[LQ_MAX_DIVPB]; field[9] = v[LQ_B2] * dV / (2.0 * LOG_MU0);
+        field[10] = avg(LQ_JX); field[11] = avg(LQ_JY); field[12] = avg(LQ_JZ); field[13] = avg(LQ_JT);
+        field[14] = v[LQ_MAX_J];
+        field[15] = avg(LQ_EPX); field[16] = avg(LQ_EPY); field[17] = avg(LQ_EPZ); field[18] = avg(LQ_EPT);
+        field[19] = v[LQ_MAX_EP]; field[20] = v[LQ_EP2] * dV * LOG_EPS0 / 2.0;
+        field[21] = avg(LQ_EX); field[22] = avg(LQ_EY); field[23] = avg(LQ_EZ); field[24] = avg(LQ_ET);
+        field[25] = v[LQ_MAX_E]; field[26] = v[LQ_E2] * dV * LOG_EPS0 / 2.0; field[27] = v[LQ_CE2] * dV * LOG_EPS0 / 2.0;
+        const int ids[6] = {C_MAX_CELL_UE, C_MAX_NODE_UE, C_MAX_NODE_UE, C_MAX_E, C_MIN_CELL_RHO, C_MIN_NODE_RHO};
+        for (int q = 0; q < 6; ++q) field[28 + q] = Dt > 0 ? (double)(cnt[ids[q]] - log_cnt0[ids[q]]) / Dt * dt : 0.0;
+        field[34] = v[LQ_MIN_DE]; field[35] = v[LQ_MAX_DE]; field[36] = v[LQ_MIN_DI]; field[37] = v[LQ_MAX_DI];
+        field[38] = v[LQ_MIN_TL]; field[39] = v[LQ_MAX_VA]; field[40] = v[LQ_MAX_UE];
+        log_begin(t);
+    }
+
+    // The files and headers of src/user.cpp:2147-2231.
+    void log_open(const char* dir, const char* const* names, int precision, int include_inner,
+                  double max_B_stop) override {
+        if (!dir || !names) throw ArgError("log directory and species names are required");
+        if (precision < 1) precision = 1;
+        log_precision = precision;
+        log_inner = include_inner;
+        log_max_B = max_B_stop;
+        const std::string d(dir);
+        log_pop.clear();
+        for (size_t s = 0; s < sp.size(); ++s) {
+            if (!names[s]) throw ArgError("null species name");
+            // the zero padding follows the 0-based index and the number is
+            // 1-based, as in the reference
+            const char* zero = s < 10 ? "00" : (s < 100 ? "0" : "");
+            std::ostringstream fn;
+            fn << d << "/pop" << zero << s + 1 << "_" << names[s] << ".log";
+            auto f = std::make_unique<std::ofstream>(fn.str().c_str(), std::ios_base::out);
+            if (!*f) throw ArgError("cannot open " + fn.str());
+            f->precision(precision);
+            *f << std::scientific << std::showpos;
+            *f << "% " << names[s] << std::endl
+               << "% m [kg] = " << sp[s]->d.m << std::endl
+               << "% q [C] = " << sp[s]->d.q << std::endl
+               << "% columns = 17" << std::endl
+               << "% 01. Time [s]" << std::endl << "% 02. Particles [#]" << std::endl
+               << "% 03. Macroparticles [#]" << std::endl << "% 04. avg(Vx) [m/s]" << std::endl
+               << "% 05. avg(Vy) [m/s]" << std::endl << "% 06. avg(Vz) [m/s]" << std::endl
+               << "% 07. avg(|V|) [m/s]" << std::endl << "% 08. Kinetic energy [J]" << std::endl
+               << "% 09. max(|V|) [m/s]" << std::endl << "% 10. Escape rate [#/s]" << std::endl
+               << "% 11. Impact rate [#/s]" << std::endl << "% 12. Inject rate [#/s]" << std::endl
+               << "% 13. Macroparticle inject rate [#/dt]" << std::endl
+               << "% 14. Kinetic energy escape rate [J/s]" << std::endl
+               << "% 15. Kinetic energy impact rate [J/s]" << std::endl
+               << "% 16. Kinetic energy inject rate [J/s]" << std::endl
+               << "% 17. Constraint maxVi rate [#/dt]" << std::endl;
+            log_pop.push_back(std::move(f));
+        }
+        if (log_field.is_open()) log_field.close();
+        log_field.open((d + "/field.log").c_str(), std::ios_base::out);
+        if (!log_field) throw ArgError("cannot open " + d + "/field.log");
+        log_field.precision(precision);
+        log_field << std::scientific << std::showpos;
+        static const char* cols[41] = {
+            "Time [s]", "avg(faceBx) [T]", "avg(faceBy) [T]", "avg(faceBz) [T]", "avg(|faceB|) [T]", "max(|faceB|) [T]",
+            "avg(div(faceB)) [T/m]", "max(div(faceB)) [T/m]", "max(dx*div(faceB)/faceB) [-]",
+            "energy(sum(dV*faceB^2/2*mu0)) [J]", "avg(cellJix) [A/m^2]", "avg(cellJiy) [A/m^2]",
+            "avg(cellJiz) [A/m^2]", "avg(|cellJi|) [A/m^2]", "max(|cellJi|) [A/m^2]", "avg(cellEpx) [V/m]",
+            "avg(cellEpy) [V/m]", "avg(cellEpz) [V/m]", "avg(|cellEp|) [V/m]", "max(|cellEp|) [V/m]",
+            "energy(sum(dV*cellEp^2*eps0/2)) [J]", "avg(nodeEx) [V/m]", "avg(nodeEy) [V/m]", "avg(nodeEz) [V/m]",
+            "avg(|nodeE|) [V/m]", "max(|nodeE|) [V/m]",
+            "energy(sum(dV*nodeE^2*eps0/2)), nodeE = -Ue x B + eta*J [J]",
+            "energy(sum(dV*cellE^2*eps0/2)), cellE = -Ue x B [J]", "Constraint maxUe rate (cell) [#/dt]",
+            "Constraint maxUe rate (node) [#/dt]", "Constraint maxVw rate (node) [#/dt]",
+            "Constraint maxE rate (node) [#/dt]", "Constraint minRhoQi rate (cell) [#/dt]",
+            "Constraint minRhoQi rate (node) [#/dt]", "Minimum electron inertial length [m]",
+            "Maximum electron inertial length [m]", "Minimum proton inertial length [m]",
+            "Maximum proton inertial length [m]", "Minimum ion Larmor period [s]", "Maximum Alfven speed [m/s]",
+            "Maximum electron fluid speed [m/s]"};
+        log_field << "% field" << std::endl << "% columns = 41" << std::endl;
+        for (int c = 0; c < 41; ++c) {
+            char num[8];
+            std::snprintf(num, sizeof(num), "%02d", c + 1);  // the stream shows signs
+            log_field << "% " << num << ". " << cols[c] << std::endl;
+        }
+    }
+
+    void log_write(double t) override {
+        if (!log_field.is_open()) throw StateError("log files are not open (rh_log_open)");
+        std::vector<double> field(41), pop(17 * sp.size());
+        log_values(t, log_inner, field.data(), pop.data());
+        for (size_t s = 0; s < sp.size(); ++s) {
+            std::ofstream& f = *log_pop[s];
+            for (int q = 0; q < 17; ++q) f << pop[17 * s + q] << " ";
+            f << std::endl;
+        }
+        // the reference writes max(|nodeE|) after an extra space
+        for (int q = 0; q < 41; ++q) log_field << (q == 25 ? " " : "") << field[q] << " ";
+        log_field << std::endl;
+        if (log_max_B > 0 && field[5] > log_max_B)
+            throw StateError("maximum B reached: the reference stops the run here (max_magnetic_field_stop_run)");
     }
 
     int64_t injected(int s) override {
@@ -996,33 +1255,55 @@ public:
         return (int64_t)h;
     }
 
-    void diag_raw(int s, double* t) {
+    // t[DIAG_Q]: count, sum vx, vy, vz, sum |v|, sum v^2, max |v| (unweighted)
+    void diag_raw(int s, double (&t)[DIAG_Q]) {
         Species& S_ = S(s);
         if (S_.pushed) throw StateError("diagnostics between propagate and wait_particle_sends");
         const uint32_t* extra = S_.inj_pending ? (const uint32_t*)(S_.inj_off.p + S_.inj_ncell) : nullptr;
         launch("diag", k_diag<Real>, DIAG_BLOCKS, 256, 0, S_.A(), (const uint32_t*)(S_.sub.p + g.nkeys), extra,
                (uint32_t)S_.cap, diag_partial.p);
-        std::vector<double> h(DIAG_BLOCKS * 5);
+        std::vector<double> h(DIAG_BLOCKS * DIAG_Q);
         RH_HIP(hipMemcpyAsync(h.data(), diag_partial.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
         RH_HIP(hipStreamSynchronize(stream));
-        for (int q = 0; q < 5; ++q) t[q] = 0;
-        for (int b = 0; b < DIAG_BLOCKS; ++b)
-            for (int q = 0; q < 5; ++q) t[q] += h[b * 5 + q];
+        for (int q = 0; q < DIAG_Q; ++q) t[q] = 0;
+        for (int b = 0; b < DIAG_BLOCKS; ++b) {
+            for (int q = 0; q < DIAG_Q - 1; ++q) t[q] += h[b * DIAG_Q + q];
+            t[DIAG_Q - 1] = std::max(t[DIAG_Q - 1], h[b * DIAG_Q + DIAG_Q - 1]);
+        }
     }
 
     void diagnostics(int s, double* out) override {
-        double t[5];
+        double t[DIAG_Q];
         diag_raw(s, t);
-        for (int q = 0; q < 5; ++q) out[q] = t[q] * S(s).d.w;
+        const double w = S(s).d.w;
+        out[0] = t[0] * w; out[1] = t[1] * w; out[2] = t[2] * w; out[3] = t[3] * w; out[4] = t[5] * w;
+    }
+
+    // counter totals: the context's own and every species' push counters
+    void counter_totals(unsigned long long* h) {
+        std::vector<unsigned long long> hs((size_t)C_N * COUNTER_SLOTS);
+        for (int q = 0; q < C_N; ++q) h[q] = 0;
+        for (size_t b = 0; b <= sp.size(); ++b) {
+            const unsigned long long* src = b == 0 ? dcount.p : sp[b - 1]->pcount.p;
+            RH_HIP(hipMemcpyAsync(hs.data(), src, sizeof(unsigned long long) * hs.size(), hipMemcpyDeviceToHost, stream));
+            RH_HIP(hipStreamSynchronize(stream));
+            for (int sl = 0; sl < COUNTER_SLOTS; ++sl)
+                for (int q = 0; q < C_N; ++q) h[q] += hs[(size_t)sl * C_N + q];
+        }
+    }
+
+    unsigned long long species_counter(Species& S_, int id) {
+        std::vector<unsigned long long> hs((size_t)C_N * COUNTER_SLOTS);
+        RH_HIP(hipMemcpyAsync(hs.data(), S_.pcount.p, sizeof(unsigned long long) * hs.size(), hipMemcpyDeviceToHost, stream));
+        RH_HIP(hipStreamSynchronize(stream));
+        unsigned long long h = 0;
+        for (int sl = 0; sl < COUNTER_SLOTS; ++sl) h += hs[(size_t)sl * C_N + id];
+        return h;
     }
 
     void counters(double* out) override {
-        std::vector<unsigned long long> hs((size_t)C_N * COUNTER_SLOTS);
-        RH_HIP(hipMemcpyAsync(hs.data(), dcount.p, sizeof(unsigned long long) * hs.size(), hipMemcpyDeviceToHost, stream));
-        RH_HIP(hipStreamSynchronize(stream));
-        unsigned long long h[C_N] = {};
-        for (int sl = 0; sl < COUNTER_SLOTS; ++sl)
-            for (int q = 0; q < C_N; ++q) h[q] += hs[(size_t)sl * C_N + q];
+        unsigned long long h[C_N];
+        counter_totals(h);
         for (int q = 0; q < RH_N_COUNTERS; ++q) out[q] = (double)h[q];
         if (h[C_RANK_OVERFLOW])
             throw CapError("particle sort overflow (a subcell holds more than 65535 particles of one species, "
@@ -1210,6 +1491,20 @@ int rh_get_timestep(rh_ctx* ctx, int64_t* t) {
 int rh_injected(rh_ctx* ctx, int32_t s, int64_t* n) {
     if (!n) return RH_ERR_ARG;
     return guarded(ctx, [&](rh::Ctx& c) { *n = c.injected(s); });
+}
+int rh_log_begin(rh_ctx* ctx, double t) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.log_begin(t); });
+}
+int rh_log_values(rh_ctx* ctx, double t, int32_t include_inner_cells, double field[41], double* pop) {
+    if (!field || !pop) return RH_ERR_ARG;
+    return guarded(ctx, [&](rh::Ctx& c) { c.log_values(t, include_inner_cells, field, pop); });
+}
+int rh_log_open(rh_ctx* ctx, const char* directory, const char* const* species_names, int32_t precision,
+                int32_t include_inner_cells, double max_B_stop) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.log_open(directory, species_names, precision, include_inner_cells, max_B_stop); });
+}
+int rh_log_write(rh_ctx* ctx, double t) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.log_write(t); });
 }
 int rh_layout_info(rh_ctx* ctx, int64_t out[6]) {
     if (!out) return RH_ERR_ARG;

@@ -137,18 +137,22 @@ __device__ __forceinline__ bool gen_particle(const Geom& g, const InjParams& ip,
     return true;
 }
 
+// Stores the particle; returns v^2 of the stored velocity (for the inject
+// kinetic energy counter, src/particle_injector.cpp:575).
 template <typename Real>
-__device__ __forceinline__ void store_particle(const InjParams& ip, PArrays<Real> A, uint32_t p, const double (&r)[3],
-                                               const double (&v)[3], uint8_t lc) {
+__device__ __forceinline__ double store_particle(const InjParams& ip, PArrays<Real> A, uint32_t p, const double (&r)[3],
+                                                 const double (&v)[3], uint8_t lc) {
     Real X = (Real)r[0], Y = (Real)r[1], Z = (Real)r[2];
     // a uniform draw just below 1 can round up to 1 in FP32 (the -axis
     // wall's coordinate is exactly 1 and stays)
     if (ip.kind == INJ_EXOSPHERE || ip.axis != 0 || ip.pos_wall) X = X >= Real(1) ? below_one<Real>() : X;
     if (ip.kind == INJ_EXOSPHERE || ip.axis != 1 || ip.pos_wall) Y = Y >= Real(1) ? below_one<Real>() : Y;
     if (ip.kind == INJ_EXOSPHERE || ip.axis != 2 || ip.pos_wall) Z = Z >= Real(1) ? below_one<Real>() : Z;
+    const Real vx = (Real)v[0], vy = (Real)v[1], vz = (Real)v[2];
     A.x[p] = X; A.y[p] = Y; A.z[p] = Z;
-    A.vx[p] = (Real)v[0]; A.vy[p] = (Real)v[1]; A.vz[p] = (Real)v[2];
+    A.vx[p] = vx; A.vy[p] = vy; A.vz[p] = vz;
     A.lc[p] = lc;
+    return (double)vx * (double)vx + (double)vy * (double)vy + (double)vz * (double)vz;
 }
 
 // The new particles of listed cell n: their number (WRITE = false), or
@@ -163,7 +167,7 @@ __device__ __forceinline__ void store_particle(const InjParams& ip, PArrays<Real
 //     per component.
 template <typename Real, bool WRITE>
 __device__ uint32_t inject_cell(const Geom& g, const InjParams& ip, int n, double rate, PArrays<Real> A,
-                                uint32_t at, uint32_t cap, uint8_t lc, bool& overflow) {
+                                uint32_t at, uint32_t cap, uint8_t lc, bool& overflow, double& v2sum) {
 #pragma clang fp contract(off)
     RngStream cs(ip.k0, ip.k1, (uint32_t)n, ip.step, 0xFFFFFFFFu);
     const int N = prob_round(rate, cs.uniform());
@@ -176,7 +180,7 @@ __device__ uint32_t inject_cell(const Geom& g, const InjParams& ip, int n, doubl
         if (!gen_particle(g, ip, n, i, j, k, (uint32_t)m, WRITE, r, v)) continue;
         if (WRITE) {
             const uint32_t p = at + q;
-            if (p < cap) store_particle(ip, A, p, r, v, lc);
+            if (p < cap) v2sum += store_particle(ip, A, p, r, v, lc);
             else overflow = true;
         }
         ++q;
@@ -188,13 +192,15 @@ __device__ uint32_t inject_cell(const Geom& g, const InjParams& ip, int n, doubl
 // the store's end (cell c: slots c*slots ..), so no count and no scan are
 // needed; one thread per place writes particle m of its cell if m < the
 // cell's probRound draw, else marks the place removed (x = -1, skipped by
-// every later pass).  The step's total goes to *total.
+// every later pass).  The step's total goes to *total, each place's v^2 to
+// v2acc[place].
 template <typename Real>
 __global__ __launch_bounds__(256) void k_inject_slots(Geom g, InjParams ip, int ncell, int slots,
                                                       const int* __restrict__ cells,
                                                       const uint32_t* __restrict__ np_ptr, PArrays<Real> A,
                                                       uint32_t cap, unsigned long long* __restrict__ total,
-                                                      unsigned long long* __restrict__ counters) {
+                                                      unsigned long long* __restrict__ counters,
+                                                      double* __restrict__ v2acc) {
     const int t = blockIdx.x * 256 + threadIdx.x;
     bool overflow = false, live = false;
     if (t < ncell * slots) {
@@ -212,7 +218,7 @@ __global__ __launch_bounds__(256) void k_inject_slots(Geom g, InjParams ip, int 
                                          ((k & (g.tz - 1)) << (g.ltx + g.lty)));
             double r[3], v[3];
             gen_particle(g, ip, n, i, j, k, (uint32_t)m, true, r, v);
-            store_particle(ip, A, (uint32_t)p, r, v, lc);
+            v2acc[t] += store_particle(ip, A, (uint32_t)p, r, v, lc);
             live = true;
         } else {
             A.x[p] = Real(-1);
@@ -232,19 +238,22 @@ __global__ __launch_bounds__(256) void k_inject_count(Geom g, InjParams ip, int 
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= ncell) return;
     bool unused = false;
+    double unused_v2 = 0.0;
     PArrays<Real> none{};
-    count[c] = inject_cell<Real, false>(g, ip, cells[c], rate ? rate[c] : ip.rate, none, 0, 0, 0, unused);
+    count[c] = inject_cell<Real, false>(g, ip, cells[c], rate ? rate[c] : ip.rate, none, 0, 0, 0, unused, unused_v2);
 }
 
 // The new particles of each listed cell at np + off[c] (np = the sorted
 // store's size); counts the step's total into *total (integer, so the sum is
-// exact) and overflows of the store into the sort's overflow counter.
+// exact), overflows of the store into the sort's overflow counter, and adds
+// the cell's sum of v^2 to v2acc[c].
 template <typename Real>
 __global__ __launch_bounds__(256) void k_inject_fill(Geom g, InjParams ip, int ncell, const int* __restrict__ cells,
                                                      const double* __restrict__ rate, const uint32_t* __restrict__ off,
                                                      const uint32_t* __restrict__ np_ptr, PArrays<Real> A, uint32_t cap,
                                                      unsigned long long* __restrict__ total,
-                                                     unsigned long long* __restrict__ counters) {
+                                                     unsigned long long* __restrict__ counters,
+                                                     double* __restrict__ v2acc) {
     const int c = blockIdx.x * 256 + threadIdx.x;
     bool overflow = false;
     if (c < ncell) {
@@ -254,8 +263,10 @@ __global__ __launch_bounds__(256) void k_inject_fill(Geom g, InjParams ip, int n
         const uint8_t lc = (uint8_t)((i & (g.tx - 1)) + ((j & (g.ty - 1)) << g.ltx) +
                                      ((k & (g.tz - 1)) << (g.ltx + g.lty)));
         const uint64_t at = (uint64_t)*np_ptr + off[c];
+        double v2 = 0.0;
         inject_cell<Real, true>(g, ip, n, rate ? rate[c] : ip.rate, A, at > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)at,
-                                cap, lc, overflow);
+                                cap, lc, overflow, v2);
+        v2acc[c] += v2;
         if (c == 0) atomicAdd(total, (unsigned long long)off[ncell]);
     }
     if (overflow) atomicAdd(counters + C_RANK_OVERFLOW, 1ull);  // slot 0

@@ -814,10 +814,14 @@ __global__ __launch_bounds__(256) void k_fold(Geom g, FoldSources<Real> src, dou
 // them.  One wave per listed cell, its lanes striding through the cell's
 // particles (coalesced at any ppc).
 constexpr int BC_CELLS_PER_BLOCK = 4;
+// The removed particles are counted for the logs (logCounterParticleEscape
+// / Impact and their kinetic energies, :121-122,153-154): per listed cell,
+// acc[4q..4q+3] += (escapes, sum v^2 of escapes, impacts, sum v^2 of
+// impacts), one wave per cell, summed in a fixed butterfly.
 template <typename Real>
 __global__ __launch_bounds__(256) void k_boundary(Geom g, const int* __restrict__ cells, int ncells,
                                                  const uint32_t* __restrict__ sub, PArrays<Real> A,
-                                                 double R2) {
+                                                 double R2, double* __restrict__ acc) {
     const int q = blockIdx.x * BC_CELLS_PER_BLOCK + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (q >= ncells) return;
@@ -828,41 +832,61 @@ __global__ __launch_bounds__(256) void k_boundary(Geom g, const int* __restrict_
     const uint32_t p0 = sub[(size_t)kc * 8], p1 = sub[(size_t)kc * 8 + 8];
     const bool exterior = !all_nbrs(g, i, j, k);
     const double xb = g.x0 + i * g.dx, yb = g.y0 + j * g.dx, zb = g.z0 + k * g.dx;
+    double c[4] = {0, 0, 0, 0};
     for (uint32_t p = p0 + lane; p < p1; p += 64) {
-        if (exterior) { A.x[p] = Real(-1); continue; }
         if (A.x[p] < Real(0)) continue;
+        const double vx = A.vx[p], vy = A.vy[p], vz = A.vz[p];
+        const double v2 = vx * vx + vy * vy + vz * vz;
+        if (exterior) {
+            A.x[p] = Real(-1);
+            c[0] += 1.0; c[1] += v2;
+            continue;
+        }
         const double x = xb + (double)A.x[p] * g.dx, y = yb + (double)A.y[p] * g.dx, z = zb + (double)A.z[p] * g.dx;
-        if (x * x + y * y + z * z < R2) A.x[p] = Real(-1);
+        if (x * x + y * y + z * z < R2) {
+            A.x[p] = Real(-1);
+            c[2] += 1.0; c[3] += v2;
+        }
     }
+    for (int t = 0; t < 4; ++t)
+        for (int off = 32; off > 0; off >>= 1) c[t] += __shfl_xor(c[t], off, 64);
+    if (lane == 0)
+        for (int t = 0; t < 4; ++t) acc[(size_t)q * 4 + t] += c[t];
 }
 
-// FP64 conservation sums (logCalcParticle, src/diagnostics.h:209-220), one
-// partial per block, summed on the host in block order.
+// FP64 particle sums (logCalcParticle, src/diagnostics.h:209-228), one
+// partial per block, summed on the host in block order: count, sum vx, vy,
+// vz, sum |v|, sum v^2 (unweighted; one weight per species) and max |v|.
 constexpr int DIAG_BLOCKS = 1024;
+constexpr int DIAG_Q = 7;
 // Injected particles waiting past the store's end (*extra of them, extra may
 // be null) are included.
 template <typename Real>
 __global__ __launch_bounds__(256) void k_diag(PArrays<Real> A, const uint32_t* __restrict__ np_ptr,
                                              const uint32_t* __restrict__ extra, uint32_t cap,
                                              double* __restrict__ partial) {
-    __shared__ double red[5][256];
+    __shared__ double red[DIAG_Q][256];
     uint32_t np = *np_ptr;
     if (extra) np = (uint64_t)np + *extra > cap ? cap : np + *extra;
-    double s[5] = {0, 0, 0, 0, 0};
+    double s[DIAG_Q] = {0, 0, 0, 0, 0, 0, 0};
     for (uint32_t p = blockIdx.x * 256u + threadIdx.x; p < np; p += 256u * gridDim.x) {
         if (A.x[p] < Real(0)) continue;
         const double vx = A.vx[p], vy = A.vy[p], vz = A.vz[p];
-        s[0] += 1.0; s[1] += vx; s[2] += vy; s[3] += vz; s[4] += vx * vx + vy * vy + vz * vz;
+        const double v2 = vx * vx + vy * vy + vz * vz, v = sqrt(v2);
+        s[0] += 1.0; s[1] += vx; s[2] += vy; s[3] += vz; s[4] += v; s[5] += v2;
+        s[6] = fmax(s[6], v);
     }
-    for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = s[q];
+    for (int q = 0; q < DIAG_Q; ++q) red[q][threadIdx.x] = s[q];
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off)
-            for (int q = 0; q < 5; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + off];
+        if ((int)threadIdx.x < off) {
+            for (int q = 0; q < DIAG_Q - 1; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + off];
+            red[DIAG_Q - 1][threadIdx.x] = fmax(red[DIAG_Q - 1][threadIdx.x], red[DIAG_Q - 1][threadIdx.x + off]);
+        }
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int q = 0; q < 5; ++q) partial[blockIdx.x * 5 + q] = red[q][0];
+        for (int q = 0; q < DIAG_Q; ++q) partial[blockIdx.x * DIAG_Q + q] = red[q][0];
 }
 
 template <typename T>

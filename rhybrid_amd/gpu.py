@@ -36,7 +36,8 @@ EXPORTS = ["rh_create", "rh_destroy", "rh_last_error", "rh_add_species", "rh_upl
            "rh_apply_boundary_conditions", "rh_propagate_B", "rh_step", "rh_diagnostics",
            "rh_counters", "rh_synchronize", "rh_set_timing", "rh_kernel_times", "rh_reset_timing",
            "rh_layout_info", "rh_denoise_moments", "rh_set_denoiser", "rh_set_injector",
-           "rh_inject_particles", "rh_set_timestep", "rh_get_timestep", "rh_injected"]
+           "rh_inject_particles", "rh_set_timestep", "rh_get_timestep", "rh_injected", "rh_log_values",
+           "rh_log_begin", "rh_log_open", "rh_log_write"]
 
 
 class GridDesc(C.Structure):
@@ -125,6 +126,10 @@ def load_library(path: str = LIB_PATH):
     L.rh_set_timestep.argtypes = [vp, i64]
     L.rh_get_timestep.argtypes = [vp, C.POINTER(i64)]
     L.rh_injected.argtypes = [vp, i32, C.POINTER(i64)]
+    L.rh_log_values.argtypes = [vp, C.c_double, i32, vp, vp]
+    L.rh_log_begin.argtypes = [vp, C.c_double]
+    L.rh_log_open.argtypes = [vp, C.c_char_p, C.POINTER(C.c_char_p), i32, i32, C.c_double]
+    L.rh_log_write.argtypes = [vp, C.c_double]
     for name in EXPORTS:
         if name not in ("rh_last_error",):
             getattr(L, name).restype = C.c_int
@@ -288,6 +293,28 @@ class Context:
         n = C.c_int64()
         self._check(load_library().rh_injected(self._h, s, C.byref(n)))
         return n.value
+
+    # -- logs (the reference's field.log and pop*.log) ----------------------
+    def log_values(self, t: float, include_inner: bool = False):
+        """(field[41], pop[nspecies][17]) of one log line at time t; the log
+        counters restart."""
+        field = np.zeros(41)
+        pop = np.zeros((len(self.species), 17))
+        self._check(load_library().rh_log_values(self._h, float(t), int(include_inner), _ptr(field), _ptr(pop)))
+        return field, pop
+
+    def log_begin(self, t: float) -> None:
+        self._check(load_library().rh_log_begin(self._h, float(t)))
+
+    def log_open(self, directory: str, names=None, precision: int = 10, include_inner: bool = False,
+                 max_B_stop: float = 0.0) -> None:
+        names = names or [sp.name for sp in self.problem.species]
+        arr = (C.c_char_p * len(names))(*[n.encode() for n in names])
+        self._check(load_library().rh_log_open(self._h, directory.encode(), arr, int(precision),
+                                               int(include_inner), float(max_B_stop)))
+
+    def log_write(self, t: float) -> None:
+        self._check(load_library().rh_log_write(self._h, float(t)))
 
     def counters(self) -> Dict[str, float]:
         out = np.zeros(len(COUNTER_NAMES))
