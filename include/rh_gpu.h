@@ -255,6 +255,22 @@ int rh_log_open(rh_ctx* ctx, const char* directory, const char* const* species_n
  * the reference stops the run (Hybrid::maxBStopRun). */
 int rh_log_write(rh_ctx* ctx, double t);
 
+/* ---- output (UserDataOP::writeData, src/operator_userdata.cpp:55-311) ---- */
+
+/* Write a VLSV file of the selected cell variables at time t: the grid
+ * arrays by name ("faceB", "cellRhoQi", "cellB", "cellJ", "cellUe",
+ * "cellJi", "cellEp" (with electron pressure), "nodeRhoQi", "nodeE",
+ * "nodeB", "nodeJ", "nodeUe", "nodeJi", "nodeEta"; FP64, interleaved n*3+l),
+ * the inner-boundary flags (one byte per cell), "ne" (cellRhoQi/e),
+ * "cellDivB", "cellNPles", and "n", "v", "T" for every output population
+ * (n_<pop>, v_<pop>, T_<pop>; population_names[s] names species s's
+ * population, species with one name are combined).  The file carries the
+ * mesh "SpatialGrid" (MESH_BBOX, MESH_NODE_CRDS_X/Y/Z, CellID = cell index)
+ * and the parameters time, timestep, dt and the box extent, as the
+ * reference's reader (tools/rhybridpy.py) expects. */
+int rh_write_vlsv(rh_ctx* ctx, const char* filename, double t, int64_t timestep, const char* const* variables,
+                  int32_t nvariables, const char* const* population_names);
+
 /* ---- measurement -------------------------------------------------------- */
 
 /* Per-kernel timing with HIP events on the context's stream (the profile::

@@ -23,6 +23,7 @@
 #include "rh_geom.hpp"
 #include "rh_inject.hpp"
 #include "rh_log.hpp"
+#include "rh_vlsv.hpp"
 #include "rh_particles.hpp"
 
 namespace rh {
@@ -119,6 +120,8 @@ public:
     virtual void log_open(const char* dir, const char* const* names, int precision, int include_inner,
                           double max_B_stop) = 0;
     virtual void log_write(double t) = 0;
+    virtual void write_vlsv(const char* path, double t, int64_t timestep, const char* const* vars, int nvars,
+                            const char* const* pops) = 0;
 };
 
 // Boltzmann constant of vth = sqrt(kB T / m) (CODATA 2002, the same table as
@@ -1246,6 +1249,171 @@ public:
             throw StateError("maximum B reached: the reference stops the run here (max_magnetic_field_stop_run)");
     }
 
+    // ---- VLSV output (UserDataOP::writeData, src/operator_userdata.cpp:55-311)
+
+    // The requested cell variables, in the reference's order, on the mesh
+    // "SpatialGrid" (cells n = i + nx*(j + ny*k), CellID = n), plus the mesh
+    // description the reference's reader uses (MESH_BBOX, node coordinates,
+    // CellID, parameters).  pops[s] names the output population of species s
+    // (species with one name are combined, as Hybrid::outputPopVarStr groups
+    // them) for the n_, v_ and T_ variables.
+    void write_vlsv(const char* path, double t, int64_t timestep, const char* const* vars, int nvars,
+                    const char* const* pops) override {
+        if (!path || (nvars > 0 && !vars)) throw ArgError("null VLSV path or variable list");
+        for (auto& sp_ : sp)
+            if (sp_->pushed) throw StateError("VLSV output between propagate and wait_particle_sends");
+        std::map<std::string, bool> want;
+        for (int q = 0; q < nvars; ++q) {
+            if (!vars[q]) throw ArgError("null VLSV variable name");
+            want[vars[q]] = true;
+        }
+        static const char* known[] = {"faceB", "cellRhoQi", "cellB", "cellJ", "cellUe", "cellJi", "cellEp",
+                                      "nodeRhoQi", "nodeE", "nodeB", "nodeJ", "nodeUe", "nodeJi", "nodeEta",
+                                      "innerFlagField", "innerFlagNode", "innerFlagParticle", "innerFlagCellEp",
+                                      "ne", "cellDivB", "cellNPles", "n", "v", "T"};
+        for (const auto& kv : want) {
+            bool ok = false;
+            for (const char* k : known) ok = ok || kv.first == k;
+            if (!ok) throw ArgError("unknown VLSV variable " + kv.first);
+        }
+        if ((want.count("n") || want.count("v") || want.count("T")) && !pops)
+            throw ArgError("population names are required for n, v and T");
+        for (auto& sp_ : sp) flush_injected(*sp_);
+        const uint64_t nc = (uint64_t)g.nc;
+        VlsvWriter W(path);
+        auto param = [&](const char* name, const void* v, const char* type, uint64_t size) {
+            W.array("PARAMETER", {{"name", name}}, v, 1, 1, type, size);
+        };
+        const double xmax = g.x0 + g.nx * g.dx, ymax = g.y0 + g.ny * g.dx, zmax = g.z0 + g.nz * g.dx;
+        const uint64_t ts = (uint64_t)timestep, cx = (uint64_t)g.nx, cy = (uint64_t)g.ny, cz = (uint64_t)g.nz;
+        param("time", &t, "float", 8);
+        param("t", &t, "float", 8);
+        param("timestep", &ts, "uint", 8);
+        param("dt", &P.dt, "float", 8);
+        param("xmin", &g.x0, "float", 8); param("ymin", &g.y0, "float", 8); param("zmin", &g.z0, "float", 8);
+        param("xmax", &xmax, "float", 8); param("ymax", &ymax, "float", 8); param("zmax", &zmax, "float", 8);
+        param("xcells_ini", &cx, "uint", 8); param("ycells_ini", &cy, "uint", 8); param("zcells_ini", &cz, "uint", 8);
+        const std::map<std::string, std::string> mesh{{"mesh", "SpatialGrid"}};
+        const uint64_t bbox[6] = {cx, cy, cz, 1, 1, 1};
+        W.array("MESH_BBOX", mesh, bbox, 6, 1, "uint", 8);
+        const double o3[3] = {g.x0, g.y0, g.z0};
+        const int dims[3] = {g.nx, g.ny, g.nz};
+        const char* crd_tags[3] = {"MESH_NODE_CRDS_X", "MESH_NODE_CRDS_Y", "MESH_NODE_CRDS_Z"};
+        for (int d = 0; d < 3; ++d) {
+            std::vector<double> c((size_t)dims[d] + 1);
+            for (int q = 0; q <= dims[d]; ++q) c[q] = o3[d] + q * g.dx;
+            W.array(crd_tags[d], mesh, c.data(), c.size(), 1, "float", 8);
+        }
+        std::vector<uint64_t> ids(nc);
+        for (uint64_t n = 0; n < nc; ++n) ids[n] = n;
+        W.array("MESH", {{"name", "SpatialGrid"}, {"type", "multi_ucd"}}, ids.data(), nc, 1, "uint", 8);
+        W.array("VARIABLE", {{"name", "CellID"}, {"mesh", "SpatialGrid"}, {"type", "celldata"}}, ids.data(), nc, 1,
+                "uint", 8);
+        auto var = [&](const std::string& name, const void* data, uint64_t vec, const char* type, uint64_t size) {
+            W.array("VARIABLE", {{"name", name}, {"mesh", "SpatialGrid"}, {"type", "celldata"}}, data, nc, vec, type,
+                    size);
+        };
+        std::vector<double> h;
+        auto field_var = [&](const char* name, int id) {
+            if (!want.count(name)) return;
+            h.assign(nc * FIELD_DIM[id], 0.0);
+            download_field(id, h.data());
+            var(name, h.data(), FIELD_DIM[id], "float", 8);
+        };
+        field_var("faceB", RH_FACE_B);
+        field_var("cellRhoQi", RH_CELL_RHOQ);
+        field_var("cellB", RH_CELL_B);
+        field_var("cellJ", RH_CELL_J);
+        field_var("cellUe", RH_CELL_UE);
+        field_var("cellJi", RH_CELL_JI);
+        if (P.use_e_pressure) field_var("cellEp", RH_CELL_EP);
+        field_var("nodeRhoQi", RH_NODE_RHOQ);
+        field_var("nodeE", RH_NODE_E);
+        field_var("nodeB", RH_NODE_B);
+        field_var("nodeJ", RH_NODE_J);
+        field_var("nodeUe", RH_NODE_UE);
+        field_var("nodeJi", RH_NODE_JI);
+        field_var("nodeEta", RH_NODE_ETA);
+        const char* flag_names[RH_N_FLAGS] = {"innerFlagField", "innerFlagNode", "innerFlagParticle", "innerFlagCellEp"};
+        for (int f = 0; f < RH_N_FLAGS; ++f) {
+            if (!want.count(flag_names[f])) continue;
+            std::vector<uint8_t> b(nc);
+            RH_HIP(hipMemcpyAsync(b.data(), flg[f].p, nc, hipMemcpyDeviceToHost, stream));
+            RH_HIP(hipStreamSynchronize(stream));
+            var(flag_names[f], b.data(), 1, "uint", 1);  // bool arrays, one byte each
+        }
+        if (want.count("ne")) {
+            h.assign(nc, 0.0);
+            download_field(RH_CELL_RHOQ, h.data());
+            for (auto& x : h) x /= LOG_E;
+            var("ne", h.data(), 1, "float", 8);
+        }
+        if (want.count("cellDivB")) {
+            // calcCellDiv (:338-352): cells with all 26 neighbours, 0 elsewhere
+            std::vector<double> fb(nc * 3), dv(nc, 0.0);
+            download_field(RH_FACE_B, fb.data());
+            for (int n = 0; n < g.nc; ++n) {
+                int i, j, k;
+                cell_ijk(g, n, i, j, k);
+                if (!all_nbrs(g, i, j, k)) continue;
+                const int xm = nbr(g, i, j, k, -1, 0, 0), ym = nbr(g, i, j, k, 0, -1, 0), zm = nbr(g, i, j, k, 0, 0, -1);
+                dv[n] = ((fb[3 * n] - fb[3 * xm]) + (fb[3 * n + 1] - fb[3 * ym + 1]) + (fb[3 * n + 2] - fb[3 * zm + 2])) / g.dx;
+            }
+            var("cellDivB", dv.data(), 1, "float", 8);
+        }
+        if (!log_rhoM.p) log_rhoM.alloc(nc);
+        if (want.count("cellNPles")) {
+            // calcCellNPles (:355-373): macroparticles of all species per cell
+            if (sp.empty()) fill(log_rhoM.p, 0.0, log_rhoM.n);
+            for (size_t s = 0; s < sp.size(); ++s)
+                launch("vlsv_count", k_cell_mass<Real>, cdiv(g.nc, 256), 256, 0, g, (const uint32_t*)sp[s]->sub.p,
+                       sp[s]->A(), 1.0, log_rhoM.p, s == 0 ? 1 : 0);
+            h.assign(nc, 0.0);
+            RH_HIP(hipMemcpyAsync(h.data(), log_rhoM.p, nc * sizeof(double), hipMemcpyDeviceToHost, stream));
+            RH_HIP(hipStreamSynchronize(stream));
+            var("cellNPles", h.data(), 1, "float", 8);
+        }
+        if (want.count("n") || want.count("v") || want.count("T")) {
+            // calcCellParticleBulkParameters (:376-447) per output population,
+            // in order of first appearance
+            std::vector<std::string> order;
+            for (size_t s = 0; s < sp.size(); ++s) {
+                if (!pops[s]) throw ArgError("null population name");
+                if (std::find(order.begin(), order.end(), std::string(pops[s])) == order.end()) order.push_back(pops[s]);
+            }
+            DevBuf<double> acc, tacc;
+            acc.alloc(nc * 4);
+            tacc.alloc(nc);
+            for (const auto& name : order) {
+                std::vector<size_t> members;
+                for (size_t s = 0; s < sp.size(); ++s)
+                    if (name == pops[s]) members.push_back(s);
+                for (int pass = 0; pass < 2; ++pass)
+                    for (size_t q = 0; q < members.size(); ++q) {
+                        Species& S_ = *sp[members[q]];
+                        launch("vlsv_bulk", k_cell_bulk<Real>, cdiv(g.nc, 256), 256, 0, g, (const uint32_t*)S_.sub.p,
+                               S_.A(), S_.d.w, pass, acc.p, tacc.p, q == 0 ? 1 : 0);
+                    }
+                std::vector<double> ha(nc * 4), ht(nc);
+                RH_HIP(hipMemcpyAsync(ha.data(), acc.p, ha.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+                RH_HIP(hipMemcpyAsync(ht.data(), tacc.p, ht.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+                RH_HIP(hipStreamSynchronize(stream));
+                const double m = sp[members[0]]->d.m, dV = g.dx * g.dx * g.dx;
+                std::vector<double> dens(nc), temp(nc), vel(nc * 3);
+                for (uint64_t n = 0; n < nc; ++n) {
+                    const double s0 = ha[4 * n];
+                    for (int l = 0; l < 3; ++l) vel[3 * n + l] = s0 > 0.0 ? ha[4 * n + 1 + l] / s0 : 0.0;
+                    temp[n] = s0 > 0.0 ? ht[n] * (m / (3 * s0 * LOG_KB)) : 0.0;
+                    dens[n] = s0 / dV;
+                }
+                if (want.count("n")) var("n_" + name, dens.data(), 1, "float", 8);
+                if (want.count("T")) var("T_" + name, temp.data(), 1, "float", 8);
+                if (want.count("v")) var("v_" + name, vel.data(), 3, "float", 8);
+            }
+        }
+        W.close();
+    }
+
     int64_t injected(int s) override {
         Species& S_ = S(s);
         if (!S_.inj_total.p) return 0;
@@ -1505,6 +1673,11 @@ int rh_log_open(rh_ctx* ctx, const char* directory, const char* const* species_n
 }
 int rh_log_write(rh_ctx* ctx, double t) {
     return guarded(ctx, [&](rh::Ctx& c) { c.log_write(t); });
+}
+int rh_write_vlsv(rh_ctx* ctx, const char* filename, double t, int64_t timestep, const char* const* variables,
+                  int32_t nvariables, const char* const* population_names) {
+    if (nvariables < 0) return RH_ERR_ARG;
+    return guarded(ctx, [&](rh::Ctx& c) { c.write_vlsv(filename, t, timestep, variables, nvariables, population_names); });
 }
 int rh_layout_info(rh_ctx* ctx, int64_t out[6]) {
     if (!out) return RH_ERR_ARG;

@@ -1,6 +1,8 @@
 // rh_log.hpp -- device reductions behind the reference's log files
 // (logWriteParticleField, src/diagnostics.h:187-816): the per-cell ion mass
-// density of logCalcParticle and the field sums and extremes of logCalcField.
+// density of logCalcParticle and the field sums and extremes of logCalcField;
+// and the per-cell particle moments of its VLSV output
+// (src/operator_userdata.cpp:355-447).
 // FP64 throughout; every sum is per thread over a fixed cell assignment, then
 // a fixed butterfly per wave, a fixed order per block, and the host adds the
 // block partials in block order -- the same bits every run.
@@ -23,6 +25,7 @@ constexpr double LOG_ME = 9.10938215e-31;
 constexpr double LOG_E = 1.60217653e-19;
 constexpr double LOG_MU0 = 4.0e-7 * 3.14159265358979323846;
 constexpr double LOG_EPS0 = 8.854187817e-12;
+constexpr double LOG_KB = 1.3806505e-23;
 
 // rhoM[n] (+)= m*w * (live particles of the species in cell n); removed
 // particles (x < 0) are not counted.
@@ -38,6 +41,46 @@ __global__ __launch_bounds__(256) void k_cell_mass(Geom g, const uint32_t* __res
     uint32_t c = 0;
     for (uint32_t p = p0; p < p1; ++p) c += A.x[p] >= Real(0);
     rhoM[n] = (first ? 0.0 : rhoM[n]) + mw * (double)c;
+}
+
+// Per-cell bulk parameters of one species (UserDataOP::
+// calcCellParticleBulkParameters, src/operator_userdata.cpp:376-447), added
+// to the output population's sums.  pass 0: acc[4n..4n+3] += (sum w, sum w vx,
+// sum w vy, sum w vz); pass 1 (after the population's pass 0): tacc[n] +=
+// sum w |v - U|^2 with U = the population's mean velocity in the cell (0
+// where the cell is empty).  Removed particles (x < 0) are skipped.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_cell_bulk(Geom g, const uint32_t* __restrict__ sub, PArrays<Real> A, double w,
+                                                   int pass, double* __restrict__ acc, double* __restrict__ tacc,
+                                                   int first) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= g.nc) return;
+    int i, j, k;
+    cell_ijk(g, n, i, j, k);
+    const uint32_t kc = cell_key(g, i, j, k);
+    const uint32_t p0 = sub[(size_t)kc * 8], p1 = sub[(size_t)kc * 8 + 8];
+    if (pass == 0) {
+        double s0 = 0, sx = 0, sy = 0, sz = 0;
+        for (uint32_t p = p0; p < p1; ++p) {
+            if (A.x[p] < Real(0)) continue;
+            s0 += w;
+            sx += (double)A.vx[p] * w; sy += (double)A.vy[p] * w; sz += (double)A.vz[p] * w;
+        }
+        double* a = acc + (size_t)n * 4;
+        if (first) { a[0] = s0; a[1] = sx; a[2] = sy; a[3] = sz; }
+        else { a[0] += s0; a[1] += sx; a[2] += sy; a[3] += sz; }
+    } else {
+        const double* a = acc + (size_t)n * 4;
+        const double ux = a[0] > 0.0 ? a[1] / a[0] : 0.0, uy = a[0] > 0.0 ? a[2] / a[0] : 0.0,
+                     uz = a[0] > 0.0 ? a[3] / a[0] : 0.0;
+        double t = 0;
+        for (uint32_t p = p0; p < p1; ++p) {
+            if (A.x[p] < Real(0)) continue;
+            const double dx = (double)A.vx[p] - ux, dy = (double)A.vy[p] - uy, dz = (double)A.vz[p] - uz;
+            t += w * (dx * dx + dy * dy + dz * dz);
+        }
+        tacc[n] = first ? t : tacc[n] + t;
+    }
 }
 
 // The field quantities of logCalcField (:254-404), in this order: 22 sums,

@@ -37,7 +37,7 @@ EXPORTS = ["rh_create", "rh_destroy", "rh_last_error", "rh_add_species", "rh_upl
            "rh_counters", "rh_synchronize", "rh_set_timing", "rh_kernel_times", "rh_reset_timing",
            "rh_layout_info", "rh_denoise_moments", "rh_set_denoiser", "rh_set_injector",
            "rh_inject_particles", "rh_set_timestep", "rh_get_timestep", "rh_injected", "rh_log_values",
-           "rh_log_begin", "rh_log_open", "rh_log_write"]
+           "rh_log_begin", "rh_log_open", "rh_log_write", "rh_write_vlsv"]
 
 
 class GridDesc(C.Structure):
@@ -130,6 +130,7 @@ def load_library(path: str = LIB_PATH):
     L.rh_log_begin.argtypes = [vp, C.c_double]
     L.rh_log_open.argtypes = [vp, C.c_char_p, C.POINTER(C.c_char_p), i32, i32, C.c_double]
     L.rh_log_write.argtypes = [vp, C.c_double]
+    L.rh_write_vlsv.argtypes = [vp, C.c_char_p, C.c_double, i64, C.POINTER(C.c_char_p), i32, C.POINTER(C.c_char_p)]
     for name in EXPORTS:
         if name not in ("rh_last_error",):
             getattr(L, name).restype = C.c_int
@@ -315,6 +316,15 @@ class Context:
 
     def log_write(self, t: float) -> None:
         self._check(load_library().rh_log_write(self._h, float(t)))
+
+    # -- output ------------------------------------------------------------------
+    def write_vlsv(self, filename: str, t: float, timestep: int, variables, populations=None) -> None:
+        """VLSV file of the selected cell variables (UserDataOP::writeData)."""
+        pops = populations or [sp.name for sp in self.problem.species]
+        v = (C.c_char_p * max(len(variables), 1))(*[x.encode() for x in variables])
+        p = (C.c_char_p * len(pops))(*[x.encode() for x in pops])
+        self._check(load_library().rh_write_vlsv(self._h, filename.encode(), float(t), int(timestep), v,
+                                                 len(variables), p))
 
     def counters(self) -> Dict[str, float]:
         out = np.zeros(len(COUNTER_NAMES))
