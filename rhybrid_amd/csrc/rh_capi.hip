@@ -368,6 +368,7 @@ public:
         if (g.extk > MAX_EXTK) throw ArgError("tile extended region too large");
         const uint64_t nk = (uint64_t)g.ntiles * g.tc * 8;
         if (nk >= 0xFFFFFFFFull) throw ArgError("grid too large for 32-bit subcell keys");
+        if ((uint64_t)g.ntiles * g.extk >= 0xFFFFFFFFull) throw ArgError("tiles too small for this grid (slot table)");
         g.nkeys = (uint32_t)nk;
     }
 

@@ -409,7 +409,40 @@ __global__ __launch_bounds__(512) void k_slot_offsets(Geom g, uint16_t* __restri
     int tx[6], ex[6], ty[6], ey[6], tz[6], ez[6];
     const bool fx = !g.per[0] || g.ntx >= 3, fy = !g.per[1] || g.nty >= 3, fz = !g.per[2] || g.ntz >= 3;
     uint32_t run = 0;
-    if (fx && fy && fz) {
+    if (fx && fy && fz && g.tx >= 2 && g.ty >= 2 && g.tz >= 2) {
+        // tile sides >= 2: a cell is on at most one face of its tile per
+        // axis, so at most two candidates per axis; all eight loads are
+        // issued before the ordered accumulation (32-bit slot offsets: the
+        // context rejects tilings whose slot table has 2^32 entries or more)
+        const int mx = axis_candidates_fast(i, g.ltx, g.ntx, g.per[0], tx, ex);
+        const int my = axis_candidates_fast(j, g.lty, g.nty, g.per[1], ty, ey);
+        const int mz = axis_candidates_fast(k, g.ltz, g.ntz, g.per[2], tz, ez);
+        uint32_t slot[8], v[8];
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int a = 0; a < 2; ++a) {
+                    const int q = a + 2 * b + 4 * c;
+                    const bool ok = a < mx && b < my && c < mz;
+                    const int TT = ok ? tx[a] + g.ntx * (ty[b] + g.nty * tz[c]) : T;
+                    slot[q] = (uint32_t)TT * (uint32_t)g.extk +
+                              (uint32_t)((ok ? ex[a] + g.ex * (ey[b] + g.ey * ez[c]) : 0) * 8 + o);
+                    v[q] = ok ? (uint32_t)cnt_tile[slot[q]] : 0u;
+                }
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int a = 0; a < 2; ++a) {
+                    const int q = a + 2 * b + 4 * c;
+                    if (!(a < mx && b < my && c < mz)) continue;
+                    if (WRITE_BASE) cnt_tile[slot[q]] = (uint16_t)run;  // offset inside the subcell's range
+                    run += v[q];
+                }
+    } else if (fx && fy && fz) {
         // at most three candidates per axis: all loads issued before the
         // ordered accumulation
         const int mx = axis_candidates_fast(i, g.ltx, g.ntx, g.per[0], tx, ex);
@@ -633,6 +666,14 @@ inline int deposit_gather_lists(const Geom& g, uint16_t* list, uint8_t* count) {
     return 0;
 }
 
+// Particles per chunk when the one-lane-per-subcell path stages the tile's
+// particles (6 arrays) through LDS: as many as fit in the subcell-sum
+// staging, which the chunks alias (it is written only after the last one).
+template <typename Real>
+RH_HD int deposit_chunk(const Geom& g) {
+    return ((16 * 8 * deposit_octant_block(g)) / 6) & ~63;
+}
+
 template <typename Real>
 struct alignas(4 * sizeof(Real)) Real4 { Real v[4]; };
 
@@ -722,6 +763,7 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
     const int ls = threadIdx.x;
     const uint32_t key = (uint32_t)(T * NS + ls);
     const uint32_t p0 = sub[key], p1 = sub[key + 1];
+    const uint32_t t0 = sub[(size_t)T * NS], t1 = sub[(size_t)(T + 1) * NS];
     const int o = ls & 7, cell = ls >> 3;
     const bool hx = o & 1, hy = (o >> 1) & 1, hz = (o >> 2) & 1;
     Real acc[4][8];
@@ -729,7 +771,27 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
     for (int q = 0; q < 4; ++q)
 #pragma unroll
         for (int c = 0; c < 8; ++c) acc[q][c] = Real(0);
-    for (uint32_t p = p0; p < p1; ++p) cic_accumulate(acc, hx, hy, hz, A.x[p], A.y[p], A.z[p], A.vx[p], A.vy[p], A.vz[p]);
+    // The tile's particles go through LDS in chunks read with coalesced
+    // loads; each thread then takes its subcell's particles of the chunk, in
+    // particle order (the same sums as reading them directly).
+    const int CH = deposit_chunk<Real>(g);
+    Real* L = S;
+    for (uint32_t cs = t0; cs < t1; cs += (uint32_t)CH) {
+        const int n = (int)min((uint32_t)CH, t1 - cs);
+        __syncthreads();  // the previous chunk is consumed
+        for (int q = ls; q < n; q += blockDim.x) {
+            const uint32_t p = cs + (uint32_t)q;
+            L[q] = A.x[p]; L[CH + q] = A.y[p]; L[2 * CH + q] = A.z[p];
+            L[3 * CH + q] = A.vx[p]; L[4 * CH + q] = A.vy[p]; L[5 * CH + q] = A.vz[p];
+        }
+        __syncthreads();
+        const uint32_t a = max(p0, cs), b = min(p1, cs + (uint32_t)n);
+        for (uint32_t p = a; p < b; ++p) {
+            const int q = (int)(p - cs);
+            cic_accumulate(acc, hx, hy, hz, L[q], L[CH + q], L[2 * CH + q], L[3 * CH + q], L[4 * CH + q], L[5 * CH + q]);
+        }
+    }
+    __syncthreads();  // the staging is read; S may be written
     Real4<Real>* out = reinterpret_cast<Real4<Real>*>(partial) + (size_t)T * g.next;
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
