@@ -75,8 +75,9 @@ template <> __device__ __forceinline__ double below_one<double>() { return 0x1.f
 // integer LDS atomics.  In pass 1 the rank of a particle is its position among
 // the tile's particles with the same slot in memory order (the migration is a
 // stable counting sort): each wave ranks its lanes with ballots and publishes
-// its per-slot group sizes; a lane's rank adds the slot's running count and
-// the group sizes of the earlier waves.
+// its per-slot group sizes; a lane's destination is the slot's running
+// counter (started, once per tile, at the slot's first destination index)
+// plus its rank in the wave plus the group sizes of the earlier waves.
 // The tile's field block is copied to LDS; removed particles carry x < 0.
 constexpr int PM_THREADS = 256;
 constexpr int PM_WAVES = PM_THREADS / 64;
@@ -195,7 +196,29 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
     const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int ox = blockIdx.x << g.ltx, oy = blockIdx.y << g.lty, oz = blockIdx.z << g.ltz;
-    for (int e = tid; e < g.extk; e += PM_THREADS) cnt[e] = 0;
+    if (pass == 0) {
+        for (int e = tid; e < g.extk; e += PM_THREADS) cnt[e] = 0;
+    } else {
+        // each slot's first destination index: the destination subcell's
+        // start (key_base) plus the slot's offset inside it (cnt_tile); the
+        // running counter then grows by the slot's particles in order
+        for (int e = tid; e < g.extk; e += PM_THREADS) {
+            const int o = e & 7, ec = e >> 3;
+            const int lx = ec % g.ex - 1, ly = (ec / g.ex) % g.ey - 1, lz = ec / (g.ex * g.ey) - 1;
+            int tX = (int)blockIdx.x + (lx < 0 ? -1 : (lx >= g.tx ? 1 : 0));
+            int tY = (int)blockIdx.y + (ly < 0 ? -1 : (ly >= g.ty ? 1 : 0));
+            int tZ = (int)blockIdx.z + (lz < 0 ? -1 : (lz >= g.tz ? 1 : 0));
+            // (outside a non-periodic grid the slot is never used; the wrap
+            // keeps the index in range)
+            tX += tX < 0 ? g.ntx : (tX >= g.ntx ? -g.ntx : 0);
+            tY += tY < 0 ? g.nty : (tY >= g.nty ? -g.nty : 0);
+            tZ += tZ < 0 ? g.ntz : (tZ >= g.ntz ? -g.ntz : 0);
+            const uint32_t nl = (uint32_t)((lx & (g.tx - 1)) + ((ly & (g.ty - 1)) << g.ltx) +
+                                           ((lz & (g.tz - 1)) << (g.ltx + g.lty)));
+            const uint32_t gk = ((uint32_t)(tX + g.ntx * (tY + g.nty * tZ)) * (uint32_t)g.tc + nl) * 8u + (uint32_t)o;
+            cnt[e] = key_base[gk] + cnt_tile[(size_t)T * g.extk + e];
+        }
+    }
     for (int e = tid; e < PM_WAVES * g.extk; e += PM_THREADS) grp[e] = 0;
     {
         const int nw = (int)push_field_words<Real>(g);
@@ -243,7 +266,6 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
             }
         }
         uint32_t key = deadkey;
-        uint32_t gkey = 0;   // destination subcell key of the whole grid (pass 1)
         bool clamped = false, dropped = false;
         uint8_t nlcell = 0;  // destination cell inside the destination tile
         if (act && X >= Real(0)) {
@@ -316,16 +338,6 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
                 // tiles divide the grid, so a periodic wrap keeps the local index
                 nlcell = (uint8_t)(((li + sx) & (g.tx - 1)) + (((lj + sy) & (g.ty - 1)) << g.ltx) +
                                    (((lk + sz) & (g.tz - 1)) << (g.ltx + g.lty)));
-                if (pass == 1) {
-                    // destination tile: the neighbour tile where the local coordinate left the tile
-                    int tX = (int)blockIdx.x + (li + sx < 0 ? -1 : (li + sx >= g.tx ? 1 : 0));
-                    int tY = (int)blockIdx.y + (lj + sy < 0 ? -1 : (lj + sy >= g.ty ? 1 : 0));
-                    int tZ = (int)blockIdx.z + (lk + sz < 0 ? -1 : (lk + sz >= g.tz ? 1 : 0));
-                    tX += tX < 0 ? g.ntx : (tX >= g.ntx ? -g.ntx : 0);
-                    tY += tY < 0 ? g.nty : (tY >= g.nty ? -g.nty : 0);
-                    tZ += tZ < 0 ? g.ntz : (tZ >= g.ntz ? -g.ntz : 0);
-                    gkey = ((uint32_t)(tX + g.ntx * (tY + g.nty * tZ)) * (uint32_t)g.tc + nlcell) * 8u + (uint32_t)o;
-                }
             } else {
                 dropped = true;
             }
@@ -351,9 +363,8 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
         if (leader) grp[w * g.extk + key] = (uint8_t)gsz;
         __syncthreads();
         if (key != deadkey) {
-            uint32_t rank = cnt[key] + rin;
-            for (int v = 0; v < w; ++v) rank += grp[v * g.extk + key];
-            const uint32_t dst = key_base[gkey] + cnt_tile[(size_t)T * g.extk + key] + rank;
+            uint32_t dst = cnt[key] + rin;  // the slot's next destination index
+            for (int v = 0; v < w; ++v) dst += grp[v * g.extk + key];
             if (dst < cap) {
                 D.x[dst] = X; D.y[dst] = Y; D.z[dst] = Z;
                 D.vx[dst] = vx; D.vy[dst] = vy; D.vz[dst] = vz;
