@@ -73,7 +73,7 @@ def measured_traffic(kernel: str):
     """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, each scaled by the factor a 1 GiB copy
     gives); None if the summary does not list the kernel."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_obstacle128x96x96_ppc32.txt")
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_obstacle128x96x96_ppc32_inflow.txt")
     name = PROFILE_KERNELS.get(kernel)
     if name is None or not os.path.exists(path):
         return None
@@ -289,14 +289,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-denoiser", action="store_true")
+    ap.add_argument("--no-in-loop", action="store_true", help="skip the denoiser's in-loop comparison runs")
+    ap.add_argument("--only-denoiser", action="store_true", help="the denoiser section alone (its kernels' profile)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one domain per GPU; "
               "the halo and migration exchange between GPUs is not built).", file=sys.stderr)
         sys.exit(2)
+    if args.only_denoiser:
+        print(json.dumps({"denoiser": denoiser_bench()}), flush=True)
+        return
     res = gpu_bench(args)
     dn = None if args.no_denoiser else denoiser_bench()
-    if dn is not None:
+    if dn is not None and not args.no_in_loop:
         from rhybrid_amd import denoise_loop
         dn["in_loop"] = denoise_loop.compare()
     cpu, err = None, None

@@ -921,6 +921,8 @@ __global__ __launch_bounds__(256) void k_boundary(Geom g, const int* __restrict_
             c[2] += 1.0; c[3] += v2;
         }
     }
+    // most listed cells remove nothing in a step: then the wave is done
+    if (!__ballot(c[0] != 0.0 || c[2] != 0.0)) return;
     for (int t = 0; t < 4; ++t)
         for (int off = 32; off > 0; off >>= 1) c[t] += __shfl_xor(c[t], off, 64);
     if (lane == 0)
