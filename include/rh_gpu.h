@@ -162,7 +162,9 @@ int rh_step(rh_ctx* ctx, int32_t nsteps);
 /* Injector kinds: the solar-wind inflow through one outer wall
  * (InjectorSolarWind, src/particle_injector.cpp:380-700) and photoions from a
  * per-cell rate table (InjectorExosphere, :1489-1840). */
-typedef enum rh_injector_kind { RH_INJ_NONE = 0, RH_INJ_SOLAR_WIND = 1, RH_INJ_EXOSPHERE = 2 } rh_injector_kind;
+typedef enum rh_injector_kind {
+    RH_INJ_NONE = 0, RH_INJ_SOLAR_WIND = 1, RH_INJ_EXOSPHERE = 2, RH_INJ_UNIFORM = 3
+} rh_injector_kind;
 
 /* Random numbers: Philox4x32-10 with key = seed and counter = (cell,
  * timestep, particle, block), so a step's new particles do not depend on
@@ -188,6 +190,12 @@ typedef struct rh_injector_desc {
      * copied) is cellExosphere after its normalisation (:1772-1781). */
     const double* cell_rate;   /* macroparticles per cell per dt */
     double exobase_radius;     /* [m] */
+    /* RH_INJ_UNIFORM (InjectorUniform, :159-345): at timestep 1 only,
+     * probRound(ppc) particles in every cell whose centre lies in box (xmin
+     * xmax ymin ymax zmin zmax [m]), uniform in the cell shrunk by epsCell,
+     * velocity `velocity` + a Maxwellian of `temperature`; the species weight
+     * must be density*dx^3/ppc. */
+    double box[6];
 } rh_injector_desc;
 
 /* Attach (or with NULL / RH_INJ_NONE detach) the injector of a species

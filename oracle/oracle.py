@@ -49,10 +49,10 @@ class OrcParams(C.Structure):
 class OrcInjector(C.Structure):
     _fields_ = [("kind", C.c_int32), ("seed", C.c_uint64), ("temperature", C.c_double),
                 ("velocity", C.c_double * 3), ("density", C.c_double), ("ppc", C.c_double),
-                ("cell_rate", C.c_void_p), ("exobase_radius", C.c_double)]
+                ("cell_rate", C.c_void_p), ("exobase_radius", C.c_double), ("box", C.c_double * 6)]
 
 
-INJ_NONE, INJ_SOLAR_WIND, INJ_EXOSPHERE = 0, 1, 2
+INJ_NONE, INJ_SOLAR_WIND, INJ_EXOSPHERE, INJ_UNIFORM = 0, 1, 2, 3
 
 
 class OrcSpecies(C.Structure):
@@ -167,6 +167,8 @@ def injector_struct(inj) -> OrcInjector:
         ci._rate = np.ascontiguousarray(inj.cell_rate, dtype=np.float64)
         ci.cell_rate = ci._rate.ctypes.data
     ci.exobase_radius = inj.exobase_radius
+    for l in range(6):
+        ci.box[l] = inj.box[l]
     return ci
 
 

@@ -63,6 +63,7 @@ struct orc_sim {
         double x0, xm, c, d, cxm; /* sampleDerivGaussian constants for x0 = U/vth */
         double* cell_rate;        /* exosphere */
         double R2_exobase;
+        double U[3], box[6];      /* uniform: bulk velocity, filled box of cell centres */
         int64_t injected;
     } inj[MAX_SPECIES];
     /* log counters per species (Hybrid::logCounterParticle*): escape, impact
@@ -1151,6 +1152,15 @@ int orc_set_injector(orc_sim* s, int sp, const orc_injector* in) {
         I->rate = in->ppc * U * s->p.dt / s->g.dx;
         I->x0 = U / I->vth;
         deriv_gauss_consts(I->x0, &I->xm, &I->c, &I->d, &I->cxm);
+    } else if (in->kind == ORC_INJ_UNIFORM) {
+        /* InjectorUniform::initialize (src/particle_injector.cpp:267-345) */
+        if (!(in->ppc > 0) || !(in->density > 0)) return -2;
+        for (int l = 0; l < 3; ++l) {
+            I->U[l] = in->velocity[l];
+            if (!(in->box[2 * l] < in->box[2 * l + 1])) return -2;
+        }
+        memcpy(I->box, in->box, sizeof(I->box));
+        I->rate = in->ppc;
     } else if (in->kind == ORC_INJ_EXOSPHERE) {
         if (!in->cell_rate) return -2;
         I->cell_rate = (double*)malloc(sizeof(double) * s->nc);
@@ -1176,6 +1186,16 @@ static int sw_injection_cell(const orc_sim* s, const struct orc_inj* I, int n) {
     return 1;
 }
 
+/* The cell centre lies in the uniform injector's box (src/particle_injector.cpp:207-210). */
+static int uniform_cell(const orc_sim* s, const struct orc_inj* I, int n) {
+    int i, j, k;
+    decompose(&s->g, n, &i, &j, &k);
+    const double c[3] = {s->g.x0 + (i + 0.5) * s->g.dx, s->g.y0 + (j + 0.5) * s->g.dx, s->g.z0 + (k + 0.5) * s->g.dx};
+    for (int l = 0; l < 3; ++l)
+        if (c[l] < I->box[2 * l] || c[l] > I->box[2 * l + 1]) return 0;
+    return 1;
+}
+
 /* One injection cell's new particles (cell-local metres, state[7]) written
  * to out; returns their number.  Stream of the cell's count: particle index
  * 0xFFFFFFFF; stream of trial particle m: index m. */
@@ -1184,7 +1204,7 @@ static int inject_cell(const orc_sim* s, int sp, int n, uint32_t step, double* o
     const double dx = s->g.dx;
     orc_stream st;
     stream_init(&st, I->key, (uint32_t)n, step, 0xFFFFFFFFu);
-    const double rate = I->kind == ORC_INJ_SOLAR_WIND ? I->rate : I->cell_rate[n];
+    const double rate = I->kind == ORC_INJ_EXOSPHERE ? I->cell_rate[n] : I->rate;
     const int N = prob_round(rate, stream_uniform(&st));
     int q = 0;
     for (int m = 0; m < N; ++m) {
@@ -1202,6 +1222,12 @@ static int inject_cell(const orc_sim* s, int sp, int n, uint32_t step, double* o
             v[a] = I->pos_wall ? -vn : vn;
             v[l1] = I->vth * stream_gauss(&st);
             v[l2] = I->vth * stream_gauss(&st);
+        } else if (I->kind == ORC_INJ_UNIFORM) {
+            /* InjectorUniform::injectParticles (src/particle_injector.cpp:187-252):
+             * uniform in the cell shrunk by epsCell, drifting Maxwellian */
+            const double eps = 1.0e-4;
+            for (int l = 0; l < 3; ++l) r[l] = 0.5 * dx + (1.0 - eps) * dx * (stream_uniform(&st) - 0.5);
+            for (int l = 0; l < 3; ++l) v[l] = I->U[l] + I->vth * stream_gauss(&st);
         } else {
             /* InjectorExosphere::injectParticles (src/particle_injector.cpp:1522-1591):
              * uniform in the cell shrunk by epsCell, discarded below the exobase */
@@ -1231,12 +1257,15 @@ void orc_inject(orc_sim* s, int sp) {
     if (sp < 0 || sp >= s->nspecies) return;
     struct orc_inj* I = &s->inj[sp];
     if (I->kind == ORC_INJ_NONE || s->timestep <= 0) return;
+    if (I->kind == ORC_INJ_UNIFORM && s->timestep != 1) return; /* :181 */
     const uint32_t step = (uint32_t)s->timestep;
     const int nc = s->nc;
     int* cnt = (int*)calloc((size_t)nc, sizeof(int));
     int64_t total = 0, cap = 0;
     for (int n = 0; n < nc; ++n) {
-        if (I->kind == ORC_INJ_SOLAR_WIND ? !sw_injection_cell(s, I, n) : !(I->cell_rate[n] > 0)) continue;
+        if (I->kind == ORC_INJ_SOLAR_WIND && !sw_injection_cell(s, I, n)) continue;
+        if (I->kind == ORC_INJ_EXOSPHERE && !(I->cell_rate[n] > 0)) continue;
+        if (I->kind == ORC_INJ_UNIFORM && !uniform_cell(s, I, n)) continue;
         cnt[n] = inject_cell(s, sp, n, step, NULL, 0);
         total += cnt[n];
         if (cnt[n] > cap) cap = cnt[n];

@@ -142,7 +142,7 @@ void orc_calc_cell_Ep(orc_sim* s);
  * SC'11) with key = seed and counter = (cell, timestep, particle, block), so
  * that any order of evaluation gives the same particles.  Two doubles of 53
  * random bits per generator call. */
-enum { ORC_INJ_NONE = 0, ORC_INJ_SOLAR_WIND = 1, ORC_INJ_EXOSPHERE = 2 };
+enum { ORC_INJ_NONE = 0, ORC_INJ_SOLAR_WIND = 1, ORC_INJ_EXOSPHERE = 2, ORC_INJ_UNIFORM = 3 };
 typedef struct orc_injector {
     int32_t kind;
     uint64_t seed;
@@ -152,6 +152,7 @@ typedef struct orc_injector {
     double ppc;                /* solar wind: macroparticles per cell */
     const double* cell_rate;   /* exosphere: nc values, macroparticles per cell per dt (copied) */
     double exobase_radius;     /* exosphere: trial positions below it are discarded */
+    double box[6];             /* uniform: xmin xmax ymin ymax zmin zmax of the filled cell centres */
 } orc_injector;
 
 void    orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import itertools
 import time
-from typing import Dict, List, Optional, Sequence, Tuple
+from typing import Callable, Dict, List, Optional, Sequence, Tuple
 
 from . import problem as P
 from .gpu import RH_FP32, Context
@@ -35,15 +35,18 @@ def candidates(grid: P.Grid, precision: int = RH_FP32, min_tiles: int = 2048,
         if (grid.nx // tx) * (grid.ny // ty) * (grid.nz // tz) < min_tiles:
             continue
         out.append((tx, ty, tz))
-    # larger tiles first: fewer halo slots per particle
-    out.sort(key=lambda t: (-t[0] * t[1] * t[2], t))
+    # larger tiles first (fewer halo slots per particle), then the most
+    # cubic (smallest extended region for the cells)
+    out.sort(key=lambda t: (-t[0] * t[1] * t[2], (t[0] + 2) * (t[1] + 2) * (t[2] + 2), t))
     return out
 
 
 def tune(problem: P.Problem, precision: int = RH_FP32, steps: int = 3,
-         shapes: Optional[Sequence[Tile]] = None, max_shapes: int = 8) -> Dict[str, object]:
-    """Time `steps` full steps per candidate shape; returns the best shape and
-    every shape's milliseconds per step."""
+         shapes: Optional[Sequence[Tile]] = None, max_shapes: int = 8,
+         prepare: Optional[Callable[[Context], None]] = None) -> Dict[str, object]:
+    """Time `steps` full steps per candidate shape (after `prepare(ctx)`, e.g.
+    a device fill); returns the best shape and every shape's milliseconds per
+    step."""
     if shapes is None:
         shapes = candidates(problem.grid, precision)[:max_shapes]
     if not shapes:
@@ -51,6 +54,8 @@ def tune(problem: P.Problem, precision: int = RH_FP32, steps: int = 3,
     times = {}
     for t in shapes:
         ctx = Context(problem, precision=precision, tile=t)
+        if prepare is not None:
+            prepare(ctx)
         ctx.step(1)
         ctx.synchronize()
         t0 = time.perf_counter()

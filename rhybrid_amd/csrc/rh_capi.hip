@@ -934,6 +934,29 @@ public:
                     if (l != axis) ok = has_pos(c3[l], dims[l], g.per[l]) && has_neg(c3[l], g.per[l]);
                 if (ok) cells.push_back(n);
             }
+        } else if (d->kind == RH_INJ_UNIFORM) {
+            // InjectorUniform::initialize (src/particle_injector.cpp:267-345)
+            if (!(d->ppc > 0) || !(d->density > 0)) throw ArgError("uniform density and ppc must be positive");
+            const double w = d->density * g.dx * g.dx * g.dx / d->ppc;
+            if (!(std::fabs(w - S_.d.w) <= 1e-9 * std::fabs(w)))
+                throw ArgError("species weight differs from density*dV/ppc of its uniform injector");
+            for (int l = 0; l < 3; ++l)
+                if (!(d->box[2 * l] < d->box[2 * l + 1])) throw ArgError("uniform injector box: min >= max");
+            ip.kind = INJ_UNIFORM;
+            ip.rate = d->ppc;
+            ip.eps = 1.0e-4;  // epsCell (src/particle_injector.cpp:31)
+            for (int l = 0; l < 3; ++l) ip.U[l] = d->velocity[l];
+            const double o3[3] = {g.x0, g.y0, g.z0};
+            for (int n = 0; n < g.nc; ++n) {
+                int c3[3];
+                cell_ijk(g, n, c3[0], c3[1], c3[2]);
+                bool in = true;
+                for (int l = 0; l < 3; ++l) {
+                    const double c = o3[l] + (c3[l] + 0.5) * g.dx;
+                    in = in && !(c < d->box[2 * l] || c > d->box[2 * l + 1]);
+                }
+                if (in) cells.push_back(n);
+            }
         } else if (d->kind == RH_INJ_EXOSPHERE) {
             if (!d->cell_rate) throw ArgError("exosphere injector needs cell_rate");
             ip.kind = INJ_EXOSPHERE;
@@ -983,17 +1006,18 @@ public:
         RH_HIP(hipMemcpyAsync(S_.inj_cbeg.p, cbeg.data(), cbeg.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
         int slots = 0;
         std::vector<uint32_t> off;
-        if (ip.kind == INJ_SOLAR_WIND) {
+        if (ip.kind == INJ_SOLAR_WIND || ip.kind == INJ_UNIFORM) {
             // fixed places: probRound never exceeds ceil(rate)
             const double cr = std::ceil(ip.rate);
             if (cr * (double)std::max<size_t>(hc.size(), 1) >= 2147483647.0)
-                throw ArgError("solar wind injection rate too large for the fixed places");
+                throw ArgError("injection rate too large for the fixed places");
             slots = (int)cr;
             off.resize(hc.size() + 1);
             for (size_t q = 0; q <= hc.size(); ++q) off[q] = (uint32_t)(q * (size_t)slots);
             RH_HIP(hipMemcpyAsync(S_.inj_off.p, off.data(), off.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
         }
-        S_.inj_acc.alloc(ip.kind == INJ_SOLAR_WIND ? std::max<size_t>(hc.size() * (size_t)slots, 1) : nl);
+        // inject energy accumulators: per block of places (fixed places), per listed cell (count and fill)
+        S_.inj_acc.alloc(slots > 0 ? std::max<size_t>(cdiv(hc.size() * (size_t)slots, 256), 1) : nl);
         fill(S_.inj_acc.p, 0.0, S_.inj_acc.n);
         RH_HIP(hipStreamSynchronize(stream));
         S_.inj = ip;
@@ -1007,10 +1031,11 @@ public:
         Species& S_ = S(s);
         if (S_.pushed) throw StateError("inject between propagate and wait_particle_sends");
         if (S_.inj.kind == INJ_NONE || S_.inj_ncell == 0 || tstep <= 0) return;
+        if (S_.inj.kind == INJ_UNIFORM && tstep != 1) return;  // src/particle_injector.cpp:181
         flush_injected(S_);  // a second injection before the next push
         InjParams ip = S_.inj;
         ip.step = (uint32_t)tstep;
-        if (ip.kind == INJ_SOLAR_WIND) {
+        if (ip.kind == INJ_SOLAR_WIND || ip.kind == INJ_UNIFORM) {
             if (S_.inj_slots > 0)
                 launch("inject_slots", k_inject_slots<Real>, cdiv((size_t)S_.inj_ncell * S_.inj_slots, 256), 256, 0, g,
                        ip, S_.inj_ncell, S_.inj_slots, (const int*)S_.inj_cells.p,

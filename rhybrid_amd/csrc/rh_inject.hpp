@@ -73,7 +73,7 @@ struct RngStream {
     }
 };
 
-enum InjKind { INJ_NONE = 0, INJ_SOLAR_WIND = 1, INJ_EXOSPHERE = 2 };
+enum InjKind { INJ_NONE = 0, INJ_SOLAR_WIND = 1, INJ_EXOSPHERE = 2, INJ_UNIFORM = 3 };
 
 // Injector of one species for one step.  The constants of
 // sampleDerivGaussian (x0 = U/vth, its mode xm, the normalisation c, d =
@@ -85,8 +85,9 @@ struct InjParams {
     double rate;            // solar wind: macroparticles per listed cell per dt
     double vth;
     double x0, xm, c, d, cxm;
-    double eps;             // exosphere: the cell is shrunk by eps (epsCell)
+    double eps;             // exosphere, uniform: the cell is shrunk by eps (epsCell)
     double R2_exobase;      // exosphere: trial positions below the exobase are discarded
+    double U[3];            // uniform: bulk velocity
 };
 
 RH_HD int prob_round(double x, double r) {
@@ -129,6 +130,13 @@ __device__ __forceinline__ bool gen_particle(const Geom& g, const InjParams& ip,
     const double dx = g.dx;
     double rm[3];
     for (int l = 0; l < 3; ++l) rm[l] = 0.5 * dx + (1.0 - ip.eps) * dx * (st.uniform() - 0.5);
+    if (ip.kind == INJ_UNIFORM) {
+        // InjectorUniform::injectParticles (:187-252): drifting Maxwellian
+        if (!need_velocity) return true;
+        for (int l = 0; l < 3; ++l) r[l] = rm[l] / dx;
+        for (int l = 0; l < 3; ++l) v[l] = ip.U[l] + ip.vth * st.gauss();
+        return true;
+    }
     const double gx = (g.x0 + i * dx) + rm[0], gy = (g.y0 + j * dx) + rm[1], gz = (g.z0 + k * dx) + rm[2];
     if (gx * gx + gy * gy + gz * gz < ip.R2_exobase) return false;
     if (!need_velocity) return true;
@@ -145,9 +153,10 @@ __device__ __forceinline__ double store_particle(const InjParams& ip, PArrays<Re
     Real X = (Real)r[0], Y = (Real)r[1], Z = (Real)r[2];
     // a uniform draw just below 1 can round up to 1 in FP32 (the -axis
     // wall's coordinate is exactly 1 and stays)
-    if (ip.kind == INJ_EXOSPHERE || ip.axis != 0 || ip.pos_wall) X = X >= Real(1) ? below_one<Real>() : X;
-    if (ip.kind == INJ_EXOSPHERE || ip.axis != 1 || ip.pos_wall) Y = Y >= Real(1) ? below_one<Real>() : Y;
-    if (ip.kind == INJ_EXOSPHERE || ip.axis != 2 || ip.pos_wall) Z = Z >= Real(1) ? below_one<Real>() : Z;
+    const bool sw = ip.kind == INJ_SOLAR_WIND;
+    if (!sw || ip.axis != 0 || ip.pos_wall) X = X >= Real(1) ? below_one<Real>() : X;
+    if (!sw || ip.axis != 1 || ip.pos_wall) Y = Y >= Real(1) ? below_one<Real>() : Y;
+    if (!sw || ip.axis != 2 || ip.pos_wall) Z = Z >= Real(1) ? below_one<Real>() : Z;
     const Real vx = (Real)v[0], vy = (Real)v[1], vz = (Real)v[2];
     A.x[p] = X; A.y[p] = Y; A.z[p] = Z;
     A.vx[p] = vx; A.vy[p] = vy; A.vz[p] = vz;
@@ -188,12 +197,12 @@ __device__ uint32_t inject_cell(const Geom& g, const InjParams& ip, int n, doubl
     return q;
 }
 
-// Solar wind: every listed cell owns `slots` = ceil(rate) fixed places past
+// Solar wind and uniform fill: every listed cell owns `slots` = ceil(rate) fixed places past
 // the store's end (cell c: slots c*slots ..), so no count and no scan are
 // needed; one thread per place writes particle m of its cell if m < the
 // cell's probRound draw, else marks the place removed (x = -1, skipped by
-// every later pass).  The step's total goes to *total, each place's v^2 to
-// v2acc[place].
+// every later pass).  The step's total goes to *total; the block's sum of
+// v^2 (fixed butterfly per wave, waves in order) to v2acc[block].
 template <typename Real>
 __global__ __launch_bounds__(256) void k_inject_slots(Geom g, InjParams ip, int ncell, int slots,
                                                       const int* __restrict__ cells,
@@ -203,6 +212,7 @@ __global__ __launch_bounds__(256) void k_inject_slots(Geom g, InjParams ip, int 
                                                       double* __restrict__ v2acc) {
     const int t = blockIdx.x * 256 + threadIdx.x;
     bool overflow = false, live = false;
+    double v2 = 0.0;
     if (t < ncell * slots) {
         const int c = t / slots, m = t - c * slots;
         const int n = cells[c];
@@ -218,7 +228,7 @@ __global__ __launch_bounds__(256) void k_inject_slots(Geom g, InjParams ip, int 
                                          ((k & (g.tz - 1)) << (g.ltx + g.lty)));
             double r[3], v[3];
             gen_particle(g, ip, n, i, j, k, (uint32_t)m, true, r, v);
-            v2acc[t] += store_particle(ip, A, (uint32_t)p, r, v, lc);
+            v2 = store_particle(ip, A, (uint32_t)p, r, v, lc);
             live = true;
         } else {
             A.x[p] = Real(-1);
@@ -228,6 +238,11 @@ __global__ __launch_bounds__(256) void k_inject_slots(Geom g, InjParams ip, int 
     const unsigned long long b = __ballot(live);
     if ((threadIdx.x & 63) == 0 && b) atomicAdd(total, (unsigned long long)__popcll(b));
     count_event(counters, C_RANK_OVERFLOW, overflow);
+    __shared__ double wsum[4];
+    for (int off = 32; off > 0; off >>= 1) v2 += __shfl_xor(v2, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v2;
+    __syncthreads();
+    if (threadIdx.x == 0) v2acc[blockIdx.x] += ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
 }
 
 // Count of new particles per listed cell (rate: per listed cell, or the

@@ -68,7 +68,7 @@ class SpeciesDesc(C.Structure):
 class InjectorDesc(C.Structure):
     _fields_ = [("kind", C.c_int32), ("seed", C.c_uint64), ("temperature", C.c_double),
                 ("velocity", C.c_double * 3), ("density", C.c_double), ("ppc", C.c_double),
-                ("cell_rate", C.c_void_p), ("exobase_radius", C.c_double)]
+                ("cell_rate", C.c_void_p), ("exobase_radius", C.c_double), ("box", C.c_double * 6)]
 
 
 class RhError(RuntimeError):
@@ -274,6 +274,8 @@ class Context:
             for l in range(3):
                 d.velocity[l] = inj.velocity[l]
             d.density, d.ppc, d.exobase_radius = inj.density, inj.ppc, inj.exobase_radius
+            for l in range(6):
+                d.box[l] = inj.box[l]
             rate = None
             if inj.cell_rate is not None:
                 rate = np.ascontiguousarray(inj.cell_rate, dtype=np.float64)
@@ -283,6 +285,19 @@ class Context:
         self._check(load_library().rh_set_injector(self._h, s, C.byref(d)))
 
     def inject(self, s: int) -> None: self._check(load_library().rh_inject_particles(self._h, s))
+
+    def fill(self, s: int, injector, timesteps=(1,)) -> None:
+        """Run `injector` once per timestep in `timesteps` for species s (e.g.
+        the reference's InjectorUniform at timestep 1), then restore the
+        species' own injector and the timestep."""
+        t0 = self.timestep()
+        self.set_injector(s, injector)
+        for t in timesteps:
+            self.set_timestep(t)
+            self.inject(s)
+        own = (self.problem.injectors or [None] * (s + 1))
+        self.set_injector(s, own[s] if s < len(own) else None)
+        self.set_timestep(t0)
     def set_timestep(self, t: int) -> None: self._check(load_library().rh_set_timestep(self._h, int(t)))
 
     def timestep(self) -> int:

@@ -119,7 +119,8 @@ class Particles:
         return int(self.cell.shape[0])
 
 
-INJ_NONE, INJ_SOLAR_WIND, INJ_EXOSPHERE = 0, 1, 2
+INJ_NONE, INJ_SOLAR_WIND, INJ_EXOSPHERE, INJ_UNIFORM = 0, 1, 2, 3
+ALL_SPACE = (-1e50, 1e50, -1e50, 1e50, -1e50, 1e50)
 
 
 @dataclass
@@ -136,6 +137,7 @@ class Injector:
     ppc: float = 0.0
     cell_rate: Optional[np.ndarray] = None   # exosphere: macroparticles per cell per dt
     exobase_radius: float = 0.0
+    box: Tuple[float, float, float, float, float, float] = ALL_SPACE  # uniform: filled cell centres
 
 
 @dataclass
@@ -307,6 +309,13 @@ def config_obstacle(nx: int = 128, ny: int = 96, nz: int = 96, ppc: int = 32,
                    injectors=injectors)
 
 
+def uniform_injector(velocity, density: float, T: float, ppc: float, seed: int, box=ALL_SPACE) -> Injector:
+    """InjectorUniform (src/particle_injector.cpp:159-345): at timestep 1,
+    probRound(ppc) particles in every cell whose centre is in the box; the
+    species weight must be density*dV/ppc."""
+    return Injector(INJ_UNIFORM, seed, T, tuple(float(v) for v in velocity), density, float(ppc), box=tuple(box))
+
+
 def solar_wind_injector(velocity, density: float, T: float, ppc: float, seed: int) -> Injector:
     """InjectorSolarWind: inflow through the wall the bulk velocity points
     away from; the species weight must be density*dV/ppc."""
@@ -348,6 +357,46 @@ def exosphere_injector(grid: Grid, neutrals: np.ndarray, T: float, macro_per_dt:
     factor = np.where((xc < 0) & (yc * yc + zc * zc < R_shadow * R_shadow), shadow_factor, 1.0)
     rate = macro_per_dt * neutrals * factor / neutrals.sum()
     return Injector(INJ_EXOSPHERE, seed, T, cell_rate=rate, exobase_radius=R_exobase)
+
+
+def config_exosphere(nx: int = 256, ny: int = 192, nz: int = 192, ppc: int = 24, dx: float = 200e3,
+                     dt: float = 0.04, R_cells: float = 12.0, H_cells: float = 5.0, n0: float = 1e12,
+                     ionization_rate: float = 1e-6, o_relative: float = 4.0, n_sw: float = 3e6,
+                     seed: int = 6461674, steps: int = 500, sub: int = 2) -> Problem:
+    """Uneven-load box: solar-wind H+ (ppc per cell, 450 km/s, T=1e5 K, IMF
+    on the +x wall) around an absorbing sphere of R_cells cells with an
+    exponential oxygen corona (n0 at r0 = R, scale height H_cells cells,
+    exobase at R) photoionised at ionization_rate into O+ (16 m_p, +e, 1e4 K).
+    The O+ macroparticles per step are o_relative times the solar wind's
+    (InjectorExosphere's macroparticles_per_cell_relative x the upstream
+    ratio, src/particle_injector.cpp:1680-1688).  Particles start empty: H+
+    is filled on the device (meta["fill"], InjectorUniform at timestep 1),
+    then enters through the +x wall; O+ enters by photoionisation."""
+    grid = Grid(nx, ny, nz, (0, 0, 0), dx, -0.5 * nx * dx, -0.5 * ny * dx, -0.5 * nz * dx)
+    T, Te = 1e5, 1e5
+    U = (-450e3, 0.0, 0.0)
+    B0 = (-1.5e-9, 2.5e-9, 0.0)
+    R, H = R_cells * dx, H_cells * dx
+    interior = (nx - 2) * (ny - 2) * (nz - 2)
+    spH = Species("H+", QE, MP, n_sw * grid.dV / ppc, R2_obstacle=R * R, capacity=int(1.3 * ppc * grid.nc))
+    # upstream ratio: solar-wind macroparticles per step over ppc (:698-701)
+    macro_per_dt = o_relative * (abs(U[0]) * dt / dx) * (ny - 2) * (nz - 2)
+    neut = exosphere_neutrals(grid, n0, R, H, R_exobase=R, sub=sub)
+    total_rate = ionization_rate * neut.sum()
+    spO = Species("O+", QE, 16 * MP, total_rate * dt / macro_per_dt, R2_obstacle=R * R,
+                  capacity=int(4 * macro_per_dt * 400 + 1e6))
+    p = Params(dt=dt, use_hall=1, use_e_pressure=1, isothermal=1, e_pressure_coeff=KB * Te / QE, imf=B0,
+               imf_cell=(1, 0, 0, 0, 0, 0), imf_face=(1, 0, 0, 0, 0, 0))
+    _constraints(p, dx, dt, 1500e3, 1500e3, n_sw * QE, 0.05, 1500e3)
+    empty = [Particles(np.zeros(0, dtype=np.int64), np.zeros((0, 7))) for _ in range(2)]
+    injectors = [solar_wind_injector(U, n_sw, T, ppc, seed=seed + 1),
+                 exosphere_injector(grid, neut, 1e4, macro_per_dt, R_exobase=R, seed=seed + 3)]
+    return Problem("exosphere%dx%dx%d_ppc%d" % (nx, ny, nz, ppc), grid, p, [spH, spO], make_fields(grid, B0),
+                   make_flags(grid, R_field=R, R_particle=R), empty, steps=steps,
+                   meta={"ppc": ppc, "density": n_sw, "U": U, "T": T, "R_cells": R_cells, "interior": interior,
+                         "O_macro_per_dt": macro_per_dt, "O_total_rate": total_rate,
+                         "fill": [uniform_injector(U, n_sw, T, ppc, seed=seed + 5), None]},
+                   injectors=injectors)
 
 
 def perturb(problem: Problem, seed: int = 1, scale: float = 0.2) -> Problem:

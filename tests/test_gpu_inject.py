@@ -95,6 +95,20 @@ def test_exosphere_matches_oracle(gpu_mod, oracle_mod, prec):
     compare_sets(ctx, ora, prec, pr.grid.dx)
 
 
+@pytest.mark.parametrize("prec", [1, 0])
+def test_uniform_fill_matches_oracle(gpu_mod, oracle_mod, prec):
+    g = P.Grid(16, 8, 8, (0, 0, 0), 1e5, -8e5, -4e5, -4e5)
+    p = P.Params(dt=0.05)
+    n, ppc = 2e6, 5.5
+    sp = P.Species("H+", P.QE, P.MP, n * g.dV / ppc, capacity=200000)
+    inj = P.uniform_injector((-3e5, 1e4, 0.0), n, 1e5, ppc, seed=33, box=(-6e5, 6e5, -1e50, 1e50, -1e50, 1e50))
+    empty = P.Particles(np.zeros(0, dtype=np.int64), np.zeros((0, 7)))
+    pr = P.Problem("uniform", g, p, [sp], P.make_fields(g), P.make_flags(g), [empty], injectors=[inj])
+    ctx, ora = injected_pair(gpu_mod, oracle_mod, pr, prec, steps=[0, 2, 1])
+    compare_sets(ctx, ora, prec, pr.grid.dx)
+    assert ctx.injected(0) == ora.injected(0) > 0
+
+
 def test_pending_particles_in_diagnostics(gpu_mod, oracle_mod):
     pr = wall_box((-3e5, 0, 0))
     ctx, ora = injected_pair(gpu_mod, oracle_mod, pr, 1, steps=[4])

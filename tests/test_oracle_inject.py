@@ -229,3 +229,37 @@ def test_same_particles_in_any_order(oracle_mod):
     cb, sb = b2.particles(0)
     assert np.array_equal(ca, cb) and np.array_equal(sa, sb)
     assert b.num_particles(0) != 0
+
+
+def test_uniform_fill(oracle_mod):
+    """InjectorUniform (src/particle_injector.cpp:159-345): only at timestep
+    1, probRound(ppc) particles in every cell whose centre is in the box,
+    uniform in the shrunk cell, velocity U + vth N(0,1)."""
+    g = P.Grid(10, 8, 6, (0, 0, 0), 1e5, -5e5, -4e5, -3e5)
+    p = P.Params(dt=0.05)
+    n, ppc, T, U = 2e6, 7.3, 3e5, (-2e5, 5e4, 0.0)
+    sp = P.Species("H+", P.QE, P.MP, n * g.dV / ppc)
+    box = (-2e5, 3e5, -1e50, 1e50, -1e50, 1e50)
+    inj = P.uniform_injector(U, n, T, ppc, seed=8, box=box)
+    empty = P.Particles(np.zeros(0, dtype=np.int64), np.zeros((0, 7)))
+    pr = P.Problem("uni", g, p, [sp], P.make_fields(g), P.make_flags(g), [empty], injectors=[inj])
+    o = oracle_mod.Oracle(pr, threads=1)
+    o.set_timestep(2)
+    o.inject(0)
+    assert o.num_particles(0) == 0
+    o.set_timestep(1)
+    o.inject(0)
+    cell, st = o.particles(0)
+    i = cell % g.nx
+    xc = g.x0 + (i + 0.5) * g.dx
+    assert np.all((xc >= box[0]) & (xc <= box[1]))
+    listed = [c for c in range(g.nc) if box[0] <= g.x0 + (c % g.nx + 0.5) * g.dx <= box[1]]
+    counts = np.bincount(cell, minlength=g.nc)[listed]
+    assert set(np.unique(counts)) <= {7, 8}
+    assert abs(counts.mean() - ppc) < 5 * math.sqrt(0.3 * 0.7 / len(listed))
+    lo, hi = 0.5 * g.dx * P.EPS_CELL, g.dx * (1 - 0.5 * P.EPS_CELL)
+    assert np.all((st[:, 0:3] >= lo * 0.999999) & (st[:, 0:3] <= hi * 1.000001))
+    vth = math.sqrt(P.KB * T / P.MP)
+    z = ((st[:, 3:6] - np.array(U)) / vth).ravel()
+    assert stats.kstest(z, "norm").pvalue > 1e-4
+    assert np.all(st[:, 6] == sp.w)
