@@ -339,6 +339,19 @@ def exosphere_bench(steps: int = 10, warmup: int = 2, prefill: int = 600, tune_s
             "kernel_us_avg": kt, "seconds_setup": round(t_gen, 1), "seconds_fill": round(t_fill, 2)}
 
 
+def calibration_copy(gib: float) -> None:
+    """A plain device-to-device copy of `gib` GiB (one copy dispatch of known
+    size): hardware traffic counters (FETCH_SIZE, WRITE_SIZE) of the other
+    dispatches are scaled by what the counters read for it."""
+    import torch
+    x = torch.ones(int(gib * (1 << 28)), dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -349,6 +362,8 @@ def main():
     ap.add_argument("--no-in-loop", action="store_true", help="skip the denoiser's in-loop comparison runs")
     ap.add_argument("--only-denoiser", action="store_true", help="the denoiser section alone (its kernels' profile)")
     ap.add_argument("--no-exosphere", action="store_true", help="skip the uneven-load exosphere box")
+    ap.add_argument("--calibration-copy-gib", type=float, default=0.0,
+                    help="first copy this many GiB device to device (a traffic counter calibration dispatch)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one domain per GPU; "
@@ -357,6 +372,8 @@ def main():
     if args.only_denoiser:
         print(json.dumps({"denoiser": denoiser_bench()}), flush=True)
         return
+    if args.calibration_copy_gib > 0:
+        calibration_copy(args.calibration_copy_gib)
     res = gpu_bench(args)
     dn = None if args.no_denoiser else denoiser_bench()
     if dn is not None and not args.no_in_loop:

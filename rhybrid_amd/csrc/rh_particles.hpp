@@ -181,7 +181,7 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
                                                          const uint32_t* __restrict__ key_base,
                                                          unsigned long long* __restrict__ counters, int kbits,
                                                          int pass, uint32_t cap) {
-    extern __shared__ unsigned char pm_smem[];
+    extern __shared__ __align__(16) unsigned char pm_smem[];
     const int NS = g.tc * 8;
     const int hx = g.tx + 1, hy = g.ty + 1, nh = (g.tx + 1) * (g.ty + 1) * (g.tz + 1);
     uint32_t* cnt = reinterpret_cast<uint32_t*>(pm_smem);     // extk running slot counts
@@ -196,37 +196,7 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
     const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int ox = blockIdx.x << g.ltx, oy = blockIdx.y << g.lty, oz = blockIdx.z << g.ltz;
-    if (pass == 0) {
-        for (int e = tid; e < g.extk; e += PM_THREADS) cnt[e] = 0;
-    } else {
-        // each slot's first destination index: the destination subcell's
-        // start (key_base) plus the slot's offset inside it (cnt_tile); the
-        // running counter then grows by the slot's particles in order
-        for (int e = tid; e < g.extk; e += PM_THREADS) {
-            const int o = e & 7, ec = e >> 3;
-            const int lx = ec % g.ex - 1, ly = (ec / g.ex) % g.ey - 1, lz = ec / (g.ex * g.ey) - 1;
-            int tX = (int)blockIdx.x + (lx < 0 ? -1 : (lx >= g.tx ? 1 : 0));
-            int tY = (int)blockIdx.y + (ly < 0 ? -1 : (ly >= g.ty ? 1 : 0));
-            int tZ = (int)blockIdx.z + (lz < 0 ? -1 : (lz >= g.tz ? 1 : 0));
-            // (outside a non-periodic grid the slot is never used; the wrap
-            // keeps the index in range)
-            tX += tX < 0 ? g.ntx : (tX >= g.ntx ? -g.ntx : 0);
-            tY += tY < 0 ? g.nty : (tY >= g.nty ? -g.nty : 0);
-            tZ += tZ < 0 ? g.ntz : (tZ >= g.ntz ? -g.ntz : 0);
-            const uint32_t nl = (uint32_t)((lx & (g.tx - 1)) + ((ly & (g.ty - 1)) << g.ltx) +
-                                           ((lz & (g.tz - 1)) << (g.ltx + g.lty)));
-            const uint32_t gk = ((uint32_t)(tX + g.ntx * (tY + g.nty * tZ)) * (uint32_t)g.tc + nl) * 8u + (uint32_t)o;
-            cnt[e] = key_base[gk] + cnt_tile[(size_t)T * g.extk + e];
-        }
-    }
-    for (int e = tid; e < PM_WAVES * g.extk; e += PM_THREADS) grp[e] = 0;
-    {
-        const int nw = (int)push_field_words<Real>(g);
-        const uint32_t* src = pfb + (size_t)T * nw;
-        uint32_t* dst = reinterpret_cast<uint32_t*>(fl);
-        for (int q = tid; q < nw; q += PM_THREADS) dst[q] = src[q];
-    }
-    __syncthreads();
+    // the tile's particle range first: its loads overlap the set-up below
     const uint32_t p0 = sub[(size_t)T * NS], p1 = sub[(size_t)(T + 1) * NS];
     // the tile's injected particles (past the store's end) follow its own
     uint32_t a0 = 0, na = 0;
@@ -248,6 +218,43 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
             else nX = Real(-1);
         }
     }
+    // slot counters: the 8 octant slots of an extended cell are 8 consecutive
+    // entries (16-byte vectors of counts, 32-byte vectors of subcell starts)
+    uint4* cnt4 = reinterpret_cast<uint4*>(cnt);
+    if (pass == 0) {
+        for (int e = tid; e < g.extk / 4; e += PM_THREADS) cnt4[e] = make_uint4(0u, 0u, 0u, 0u);
+    } else {
+        // each slot's first destination index: the destination subcell's
+        // start (key_base) plus the slot's offset inside it (cnt_tile); the
+        // running counter then grows by the slot's particles in order
+        for (int ec = tid; ec < g.next; ec += PM_THREADS) {
+            const int lx = ec % g.ex - 1, ly = (ec / g.ex) % g.ey - 1, lz = ec / (g.ex * g.ey) - 1;
+            int tX = (int)blockIdx.x + (lx < 0 ? -1 : (lx >= g.tx ? 1 : 0));
+            int tY = (int)blockIdx.y + (ly < 0 ? -1 : (ly >= g.ty ? 1 : 0));
+            int tZ = (int)blockIdx.z + (lz < 0 ? -1 : (lz >= g.tz ? 1 : 0));
+            // (outside a non-periodic grid the slot is never used; the wrap
+            // keeps the index in range)
+            tX += tX < 0 ? g.ntx : (tX >= g.ntx ? -g.ntx : 0);
+            tY += tY < 0 ? g.nty : (tY >= g.nty ? -g.nty : 0);
+            tZ += tZ < 0 ? g.ntz : (tZ >= g.ntz ? -g.ntz : 0);
+            const uint32_t nl = (uint32_t)((lx & (g.tx - 1)) + ((ly & (g.ty - 1)) << g.ltx) +
+                                           ((lz & (g.tz - 1)) << (g.ltx + g.lty)));
+            const uint32_t gk = ((uint32_t)(tX + g.ntx * (tY + g.nty * tZ)) * (uint32_t)g.tc + nl) * 8u;
+            const uint4 b0 = *reinterpret_cast<const uint4*>(key_base + gk);
+            const uint4 b1 = *reinterpret_cast<const uint4*>(key_base + gk + 4);
+            const uint4 c = *reinterpret_cast<const uint4*>(cnt_tile + ((size_t)T * g.next + ec) * 8);
+            cnt4[2 * ec] = make_uint4(b0.x + (c.x & 0xFFFFu), b0.y + (c.x >> 16), b0.z + (c.y & 0xFFFFu), b0.w + (c.y >> 16));
+            cnt4[2 * ec + 1] = make_uint4(b1.x + (c.z & 0xFFFFu), b1.y + (c.z >> 16), b1.z + (c.w & 0xFFFFu), b1.w + (c.w >> 16));
+        }
+    }
+    for (int e = tid; e < PM_WAVES * g.extk / 16; e += PM_THREADS) reinterpret_cast<uint4*>(grp)[e] = make_uint4(0u, 0u, 0u, 0u);
+    {
+        const int nw4 = (int)push_field_words<Real>(g) / 4;
+        const uint4* src = reinterpret_cast<const uint4*>(pfb) + (size_t)T * nw4;
+        uint4* dst = reinterpret_cast<uint4*>(fl);
+        for (int q = tid; q < nw4; q += PM_THREADS) dst[q] = src[q];
+    }
+    __syncthreads();
     for (uint32_t ub = 0; ub < ntot; ub += PM_THREADS) {
         const uint32_t u = ub + tid;
         const bool act = u < ntot;
@@ -342,6 +349,14 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
                 dropped = true;
             }
         }
+        // The next chunk's loads were issued before this chunk's push: take
+        // their values here, before pass 1's stores.  The vector memory
+        // counter retires loads and stores in issue order, so a wait for those
+        // loads placed after the stores (at the loop's end) would also wait
+        // for the stores to complete, once per chunk.  (Here, before the
+        // passes part, so that neither path reaches the loop's end with the
+        // loads outstanding.)
+        asm volatile("" ::"v"(nX), "v"(nY), "v"(nZ), "v"(nvx), "v"(nvy), "v"(nvz), "v"(nlc));
         if (pass == 0) {
             // counts only: integer LDS atomics, so the order does not matter
             if (key != deadkey) atomicAdd(&cnt[key], 1u);
@@ -382,18 +397,22 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
     }
     __syncthreads();
     if (pass == 0)
-        for (int e = tid; e < g.extk; e += PM_THREADS) {
-            const uint32_t c = cnt[e];
-            if (c > 0xFFFFu) atomicAdd(counters + C_RANK_OVERFLOW, 1ull);  // slot 0
-            cnt_tile[(size_t)T * g.extk + e] = (uint16_t)c;
+        for (int ec = tid; ec < g.next; ec += PM_THREADS) {
+            const uint4 a = cnt4[2 * ec], b = cnt4[2 * ec + 1];
+            const uint32_t hi = (a.x | a.y | a.z | a.w | b.x | b.y | b.z | b.w) >> 16;
+            if (hi) atomicAdd(counters + C_RANK_OVERFLOW, 1ull);  // slot 0
+            *reinterpret_cast<uint4*>(cnt_tile + ((size_t)T * g.next + ec) * 8) =
+                make_uint4((a.x & 0xFFFFu) | (a.y << 16), (a.z & 0xFFFFu) | (a.w << 16),
+                           (b.x & 0xFFFFu) | (b.y << 16), (b.z & 0xFFFFu) | (b.w << 16));
         }
 }
 
 // Number of particles arriving in each subcell: the slot counts of every tile
 // whose extended region holds the cell, in a fixed order (out = totals, to be
-// scanned into the subcells' first destination index).  With WRITE_BASE the
-// slot counts are also replaced by each slot's offset inside its subcell's
-// range.  One block per tile (3D grid of tiles), one thread per subcell.
+// scanned into the subcells' first destination index); the slot counts are
+// replaced by each slot's offset inside its subcell's range.  One thread per
+// cell: a cell's eight octant slots are eight consecutive 16-bit entries of
+// every candidate tile's table, read and written as one 16-byte vector.
 // axis_candidates for an axis with at least three tiles or without wrap: the
 // previous tile (cell on the tile's low face), the own tile, the next tile
 // (cell on the high face), in that order.
@@ -406,20 +425,32 @@ __device__ __forceinline__ int axis_candidates_fast(int i, int lt, int nt, int p
     return m;
 }
 
-template <bool WRITE_BASE>
-__global__ __launch_bounds__(512) void k_slot_offsets(Geom g, uint16_t* __restrict__ cnt_tile,
-                                                     uint32_t* __restrict__ out,
-                                                     unsigned long long* __restrict__ counters) {
-    const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
-    const int tid = threadIdx.x;
-    const uint32_t key = (uint32_t)T * (uint32_t)(g.tc * 8) + tid;
-    const int o = tid & 7;
+// The slot offsets of one candidate (eight packed 16-bit counts) before its
+// counts are added to the running totals.
+__device__ __forceinline__ uint4 slot_offsets8(uint32_t (&run)[8], uint4 v) {
+    const uint4 off = make_uint4((run[0] & 0xFFFFu) | (run[1] << 16), (run[2] & 0xFFFFu) | (run[3] << 16),
+                                 (run[4] & 0xFFFFu) | (run[5] << 16), (run[6] & 0xFFFFu) | (run[7] << 16));
+    run[0] += v.x & 0xFFFFu; run[1] += v.x >> 16; run[2] += v.y & 0xFFFFu; run[3] += v.y >> 16;
+    run[4] += v.z & 0xFFFFu; run[5] += v.z >> 16; run[6] += v.w & 0xFFFFu; run[7] += v.w >> 16;
+    return off;
+}
+
+constexpr int SLOT_THREADS = 256;
+
+__global__ __launch_bounds__(SLOT_THREADS) void k_slot_offsets(Geom g, uint16_t* __restrict__ cnt_tile,
+                                                              uint32_t* __restrict__ out,
+                                                              unsigned long long* __restrict__ counters) {
+    const uint32_t id = blockIdx.x * (uint32_t)SLOT_THREADS + threadIdx.x;  // = tile * tc + local cell
+    if (id >= (uint32_t)g.ntiles * (uint32_t)g.tc) return;
+    const int T = (int)(id >> (g.ltx + g.lty + g.ltz)), lc = (int)(id & (uint32_t)(g.tc - 1));
     int li, lj, lk;
-    local_ijk(g, tid >> 3, li, lj, lk);
-    const int i = (blockIdx.x << g.ltx) + li, j = (blockIdx.y << g.lty) + lj, k = (blockIdx.z << g.ltz) + lk;
+    local_ijk(g, lc, li, lj, lk);
+    const int i = ((T % g.ntx) << g.ltx) + li, j = (((T / g.ntx) % g.nty) << g.lty) + lj,
+              k = ((T / (g.ntx * g.nty)) << g.ltz) + lk;
     int tx[6], ex[6], ty[6], ey[6], tz[6], ez[6];
     const bool fx = !g.per[0] || g.ntx >= 3, fy = !g.per[1] || g.nty >= 3, fz = !g.per[2] || g.ntz >= 3;
-    uint32_t run = 0;
+    uint4* tab = reinterpret_cast<uint4*>(cnt_tile);  // [tile][extended cell]: eight octant slots
+    uint32_t run[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
     if (fx && fy && fz && g.tx >= 2 && g.ty >= 2 && g.tz >= 2) {
         // tile sides >= 2: a cell is on at most one face of its tile per
         // axis, so at most two candidates per axis; all eight loads are
@@ -428,7 +459,8 @@ __global__ __launch_bounds__(512) void k_slot_offsets(Geom g, uint16_t* __restri
         const int mx = axis_candidates_fast(i, g.ltx, g.ntx, g.per[0], tx, ex);
         const int my = axis_candidates_fast(j, g.lty, g.nty, g.per[1], ty, ey);
         const int mz = axis_candidates_fast(k, g.ltz, g.ntz, g.per[2], tz, ez);
-        uint32_t slot[8], v[8];
+        uint32_t slot[8];
+        uint4 v[8];
 #pragma unroll
         for (int c = 0; c < 2; ++c)
 #pragma unroll
@@ -438,9 +470,8 @@ __global__ __launch_bounds__(512) void k_slot_offsets(Geom g, uint16_t* __restri
                     const int q = a + 2 * b + 4 * c;
                     const bool ok = a < mx && b < my && c < mz;
                     const int TT = ok ? tx[a] + g.ntx * (ty[b] + g.nty * tz[c]) : T;
-                    slot[q] = (uint32_t)TT * (uint32_t)g.extk +
-                              (uint32_t)((ok ? ex[a] + g.ex * (ey[b] + g.ey * ez[c]) : 0) * 8 + o);
-                    v[q] = ok ? (uint32_t)cnt_tile[slot[q]] : 0u;
+                    slot[q] = (uint32_t)TT * (uint32_t)g.next + (uint32_t)(ok ? ex[a] + g.ex * (ey[b] + g.ey * ez[c]) : 0);
+                    v[q] = ok ? tab[slot[q]] : make_uint4(0u, 0u, 0u, 0u);
                 }
 #pragma unroll
         for (int c = 0; c < 2; ++c)
@@ -450,39 +481,7 @@ __global__ __launch_bounds__(512) void k_slot_offsets(Geom g, uint16_t* __restri
                 for (int a = 0; a < 2; ++a) {
                     const int q = a + 2 * b + 4 * c;
                     if (!(a < mx && b < my && c < mz)) continue;
-                    if (WRITE_BASE) cnt_tile[slot[q]] = (uint16_t)run;  // offset inside the subcell's range
-                    run += v[q];
-                }
-    } else if (fx && fy && fz) {
-        // at most three candidates per axis: all loads issued before the
-        // ordered accumulation
-        const int mx = axis_candidates_fast(i, g.ltx, g.ntx, g.per[0], tx, ex);
-        const int my = axis_candidates_fast(j, g.lty, g.nty, g.per[1], ty, ey);
-        const int mz = axis_candidates_fast(k, g.ltz, g.ntz, g.per[2], tz, ez);
-        size_t slot[27];
-        uint32_t v[27];
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int b = 0; b < 3; ++b)
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const int q = a + 3 * b + 9 * c;
-                    const bool ok = a < mx && b < my && c < mz;
-                    const int TT = ok ? tx[a] + g.ntx * (ty[b] + g.nty * tz[c]) : T;
-                    slot[q] = (size_t)TT * g.extk + (size_t)((ok ? ex[a] + g.ex * (ey[b] + g.ey * ez[c]) : 0) * 8 + o);
-                    v[q] = ok ? (uint32_t)cnt_tile[slot[q]] : 0u;
-                }
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int b = 0; b < 3; ++b)
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const int q = a + 3 * b + 9 * c;
-                    if (!(a < mx && b < my && c < mz)) continue;
-                    if (WRITE_BASE) cnt_tile[slot[q]] = (uint16_t)run;  // offset inside the subcell's range
-                    run += v[q];
+                    tab[slot[q]] = slot_offsets8(run, v[q]);  // offsets inside the subcells' ranges
                 }
     } else {
         const int mx = fx ? axis_candidates_fast(i, g.ltx, g.ntx, g.per[0], tx, ex)
@@ -495,14 +494,15 @@ __global__ __launch_bounds__(512) void k_slot_offsets(Geom g, uint16_t* __restri
             for (int b = 0; b < my; ++b)
                 for (int a = 0; a < mx; ++a) {
                     const int TT = tx[a] + g.ntx * (ty[b] + g.nty * tz[c]);
-                    const size_t slot = (size_t)TT * g.extk + (size_t)((ex[a] + g.ex * (ey[b] + g.ey * ez[c])) * 8 + o);
-                    const uint32_t v = cnt_tile[slot];
-                    if (WRITE_BASE) cnt_tile[slot] = (uint16_t)run;
-                    run += v;
+                    const size_t slot = (size_t)TT * g.next + (size_t)(ex[a] + g.ex * (ey[b] + g.ey * ez[c]));
+                    tab[slot] = slot_offsets8(run, tab[slot]);
                 }
     }
-    out[key] = run;
-    if (run > 0xFFFFu) atomicAdd(counters + C_RANK_OVERFLOW, 1ull);  // offsets are 16-bit
+    uint4* o4 = reinterpret_cast<uint4*>(out + (size_t)id * 8);
+    o4[0] = make_uint4(run[0], run[1], run[2], run[3]);
+    o4[1] = make_uint4(run[4], run[5], run[6], run[7]);
+    if ((run[0] | run[1] | run[2] | run[3] | run[4] | run[5] | run[6] | run[7]) > 0xFFFFu)
+        atomicAdd(counters + C_RANK_OVERFLOW, 1ull);  // offsets are 16-bit
 }
 
 // Exclusive scan of uint32 counts (three passes: block sums, scan of the block
