@@ -286,6 +286,12 @@ int rh_write_vlsv(rh_ctx* ctx, const char* filename, double t, int64_t timestep,
  * bracketed by events; rh_kernel_times returns, for the kernel name, the
  * accumulated milliseconds and the number of launches since the last reset. */
 int rh_set_timing(rh_ctx* ctx, int32_t enable);
+/* Restrict the timing to the launches of one kernel name (NULL or "": every
+ * launch).  Each timed launch adds two event records to the stream, which
+ * costs a few microseconds of device time per launch; a measurement of the
+ * whole step times only the kernel it needs.  An extension of the timing
+ * above, with no counterpart in the reference. */
+int rh_set_timing_only(rh_ctx* ctx, const char* kernel);
 int rh_kernel_times(rh_ctx* ctx, const char* kernel, double* ms, int64_t* launches);
 int rh_reset_timing(rh_ctx* ctx);
 /* Sorted-store layout facts for roofline accounting: out = {n_tiles,

@@ -107,6 +107,7 @@ public:
     virtual void set_denoiser(rh_denoiser* dn, double max_rel) = 0;
     virtual void synchronize() = 0;
     virtual void set_timing(bool on) = 0;
+    virtual void set_timing_only(const char* name) = 0;
     virtual void kernel_time(const char* name, double* ms, int64_t* launches) = 0;
     virtual void reset_timing() = 0;
     virtual void layout(int64_t* out) = 0;
@@ -254,6 +255,7 @@ public:
     DevBuf<uint8_t> dep_count;
     int kbits = 12;
     bool timing = false;
+    std::string timing_only;  // when not empty, only launches of this name are timed
     std::map<std::string, Timer> timers;
     std::vector<hipEvent_t> event_pool;
     std::vector<uint8_t> h_inner_particle;
@@ -415,14 +417,15 @@ public:
     void launch(const char* name, K kernel, dim3 grid, dim3 block, size_t shmem, Args... args) {
         if ((size_t)grid.x * grid.y * grid.z == 0) return;
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (timing) {
+        const bool timed = timing && (timing_only.empty() || timing_only == name);
+        if (timed) {
             e0 = get_event();
             e1 = get_event();
             RH_HIP(hipEventRecord(e0, stream));
         }
         hipLaunchKernelGGL(kernel, grid, block, shmem, stream, args...);
         RH_HIP(hipGetLastError());
-        if (timing) {
+        if (timed) {
             RH_HIP(hipEventRecord(e1, stream));
             Timer& t = timers[name];
             t.pending.emplace_back(e0, e1);
@@ -1512,6 +1515,8 @@ public:
         timing = on;
     }
 
+    void set_timing_only(const char* name) override { timing_only = name ? name : ""; }
+
     void kernel_time(const char* name, double* ms, int64_t* launches) override {
         resolve_timers();
         auto it = timers.find(name ? name : "");
@@ -1657,6 +1662,9 @@ int rh_synchronize(rh_ctx* ctx) {
 }
 int rh_set_timing(rh_ctx* ctx, int32_t on) {
     return guarded(ctx, [&](rh::Ctx& c) { c.set_timing(on != 0); });
+}
+int rh_set_timing_only(rh_ctx* ctx, const char* kernel) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.set_timing_only(kernel); });
 }
 int rh_kernel_times(rh_ctx* ctx, const char* kernel, double* ms, int64_t* launches) {
     return guarded(ctx, [&](rh::Ctx& c) { c.kernel_time(kernel, ms, launches); });

@@ -34,7 +34,8 @@ EXPORTS = ["rh_create", "rh_destroy", "rh_last_error", "rh_add_species", "rh_upl
            "rh_download_particles", "rh_setup_get_fields", "rh_propagate", "rh_wait_particle_sends",
            "rh_clear_accumulation_arrays", "rh_accumulate", "rh_add_remote_updates",
            "rh_apply_boundary_conditions", "rh_propagate_B", "rh_step", "rh_diagnostics",
-           "rh_counters", "rh_synchronize", "rh_set_timing", "rh_kernel_times", "rh_reset_timing",
+           "rh_counters", "rh_synchronize", "rh_set_timing", "rh_set_timing_only", "rh_kernel_times",
+           "rh_reset_timing",
            "rh_layout_info", "rh_denoise_moments", "rh_set_denoiser", "rh_set_injector",
            "rh_inject_particles", "rh_set_timestep", "rh_get_timestep", "rh_injected", "rh_log_values",
            "rh_log_begin", "rh_log_open", "rh_log_write", "rh_write_vlsv"]
@@ -120,6 +121,7 @@ def load_library(path: str = LIB_PATH):
     L.rh_denoise_moments.argtypes = [vp, vp, C.c_double]
     L.rh_set_denoiser.argtypes = [vp, vp, C.c_double]
     L.rh_kernel_times.argtypes = [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(i64)]
+    L.rh_set_timing_only.argtypes = [vp, C.c_char_p]
     L.rh_layout_info.argtypes = [vp, vp]
     L.rh_set_injector.argtypes = [vp, i32, C.POINTER(InjectorDesc)]
     L.rh_inject_particles.argtypes = [vp, i32]
@@ -348,6 +350,11 @@ class Context:
 
     # -- measurement --------------------------------------------------------
     def set_timing(self, on: bool) -> None: self._check(load_library().rh_set_timing(self._h, int(on)))
+
+    def set_timing_only(self, name) -> None:
+        """Time only the launches of kernel `name` (None: every launch)."""
+        self._check(load_library().rh_set_timing_only(self._h, name.encode() if name else None))
+
     def reset_timing(self) -> None: self._check(load_library().rh_reset_timing(self._h))
 
     def kernel_time(self, name: str):
