@@ -127,8 +127,8 @@ def gpu_bench(args):
     ctx.set_timing(False)
     names = ["push_fields", "push", "sort_slots", "sort_count", "scan", "sort_base", "move", "deposit",
              "deposit_dense", "fold", "boundary", "inject_slots", "inject_count", "inject_fill", "sort_still",
-             "face2cell", "neumann", "imf", "cell2node", "node_J", "node2cell", "cell_Ue", "node_Ue",
-             "upwind_B", "node_E", "node_avg", "face_curl", "cell_Ep", "fill", "diag"]
+             "cell_B_bc", "neumann", "imf", "cell2node", "node_J", "node2cell", "cell_J_Ue", "node_Ue_B_E",
+             "node_avg", "face_curl", "cell_Ep", "fill", "diag"]
     ktimes = {}
     for nm in names:
         ms, n = ctx.kernel_time(nm)

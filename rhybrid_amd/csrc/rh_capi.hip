@@ -813,29 +813,32 @@ public:
         const int imf_face = P.imf_face[0] | P.imf_face[1] << 1 | P.imf_face[2] << 2 | P.imf_face[3] << 3 |
                              P.imf_face[4] << 4 | P.imf_face[5] << 5;
         const Real bx = (Real)P.imf[0], by = (Real)P.imf[1], bz = (Real)P.imf[2];
-        launch("face2cell", k_face2cell<Real>, cb, bb, 0, g, v3(RH_FACE_B), v3(RH_CELL_B));
-        neumann_cell(RH_CELL_B, 3);
-        neumann_cell(RH_CELL_JI, 3);
-        neumann_cell(RH_CELL_RHOQ, 1);
-        if (imf_cell) launch("imf", k_imf<Real>, eb, 256, 0, g, (const int*)ext_cells.p, n_ext, v3(RH_CELL_B), 0, imf_cell, bx, by, bz);
-        cell2node(RH_CELL_B, RH_NODE_B, 3);
-        cell2node(RH_CELL_RHOQ, RH_NODE_RHOQ, 1);
-        cell2node(RH_CELL_JI, RH_NODE_JI, 3);
+        // face2Cell, Neumann on cellB, cellJi, cellRhoQi and the IMF on cellB
+        launch("cell_B_bc", k_cell_B_bc<Real>, cb, bb, 0, g, v3(RH_FACE_B), v3(RH_CELL_B), v3(RH_CELL_JI),
+               fld[RH_CELL_RHOQ].p, imf_cell, bx, by, bz);
+        {
+            Cell2NodeSet<Real> set{};
+            const int from[3] = {RH_CELL_B, RH_CELL_RHOQ, RH_CELL_JI}, to[3] = {RH_NODE_B, RH_NODE_RHOQ, RH_NODE_JI};
+            const int dim[3] = {3, 1, 3};
+            for (int q = 0; q < 3; ++q) { set.from[q] = fld[from[q]].p; set.to[q] = fld[to[q]].p; set.dim[q] = dim[q]; }
+            set.n = 3;
+            launch("cell2node", k_cell2node_set<Real>, cb, bb, 0, g, set);
+        }
         launch("neumann", k_neumann_face<Real>, eb, 256, 0, g, (const int*)ext_cells.p, n_ext, v3(RH_FACE_B));
         if (imf_face) launch("imf", k_imf<Real>, eb, 256, 0, g, (const int*)ext_cells.p, n_ext, v3(RH_FACE_B), 1, imf_face, bx, by, bz);
         launch("node_J", k_node_J<Real>, cb, bb, 0, g, v3(RH_FACE_B), v3(RH_NODE_B), (const Real*)fld[RH_NODE_RHOQ].p,
                v3(RH_NODE_J), (Real)P.max_whistler, dcount.p);
-        launch("node2cell", k_node2cell<Real>, cb, bb, 0, g, v3(RH_NODE_J), v3(RH_CELL_J));
-        launch("cell_Ue", k_ue<Real, false>, cb, bb, 0, g, v3(RH_CELL_J), v3(RH_CELL_JI), fld[RH_CELL_RHOQ].p,
-               (const uint8_t*)flg[RH_INNER_FIELD].p, v3(RH_CELL_UE), P.use_hall, (Real)P.obstacle_ue[0],
-               (Real)P.obstacle_ue[1], (Real)P.obstacle_ue[2], (Real)P.max_electron_speed2, (Real)P.min_rho, dcount.p);
-        neumann_cell(RH_CELL_UE, 3);
-        launch("node_Ue", k_ue<Real, true>, cb, bb, 0, g, v3(RH_NODE_J), v3(RH_NODE_JI), fld[RH_NODE_RHOQ].p,
+        // node2Cell of nodeJ, calcCellUe, Neumann on cellUe
+        launch("cell_J_Ue", k_cell_J_Ue<Real>, cb, bb, 0, g, v3(RH_NODE_J), v3(RH_CELL_J), v3(RH_CELL_JI),
+               fld[RH_CELL_RHOQ].p, (const uint8_t*)flg[RH_INNER_FIELD].p, v3(RH_CELL_UE), P.use_hall,
+               (Real)P.obstacle_ue[0], (Real)P.obstacle_ue[1], (Real)P.obstacle_ue[2], (Real)P.max_electron_speed2,
+               (Real)P.min_rho, dcount.p);
+        // calcNodeUe, upwindNodeB, calcNodeE
+        launch("node_Ue_B_E", k_node_Ue_B_E<Real>, cb, bb, 0, g, v3(RH_NODE_J), v3(RH_NODE_JI), fld[RH_NODE_RHOQ].p,
                (const uint8_t*)flg[RH_INNER_NODE].p, v3(RH_NODE_UE), P.use_hall, (Real)P.obstacle_ue[0],
-               (Real)P.obstacle_ue[1], (Real)P.obstacle_ue[2], (Real)P.max_electron_speed2, (Real)P.min_rho, dcount.p);
-        launch("upwind_B", k_upwind_B<Real>, cb, bb, 0, g, v3(RH_CELL_B), v3(RH_NODE_UE), v3(RH_NODE_B));
-        launch("node_E", k_node_E<Real>, cb, bb, 0, g, v3(RH_NODE_UE), v3(RH_NODE_B), (const Real*)fld[RH_NODE_ETA].p,
-               v3(RH_NODE_J), v3(RH_NODE_E), (Real)P.max_E2, dcount.p);
+               (Real)P.obstacle_ue[1], (Real)P.obstacle_ue[2], (Real)P.max_electron_speed2, (Real)P.min_rho,
+               v3(RH_CELL_B), v3(RH_NODE_B), (const Real*)fld[RH_NODE_ETA].p, v3(RH_NODE_E), (Real)P.max_E2,
+               dcount.p);
         for (int it = 0; it < P.n_linear_smoothings; ++it) {
             launch("node2cell", k_node2cell<Real>, cb, bb, 0, g, v3(RH_NODE_E), v3(RH_CELL_J));
             neumann_cell(RH_CELL_J, 3);

@@ -34,14 +34,13 @@ struct V3 {
     RH_CELL_IJK          \
     if (!in_grid) return;
 
-// face2Cell (:1043-1056)
+// face2Cell (:1043-1056) at cell (i, j, k) = n
 template <typename Real>
-__global__ __launch_bounds__(256) void k_face2cell(Geom g, V3<Real> face, V3<Real> cell) {
-    RH_CELL_PROLOGUE
-    if (!all_nbrs(g, i, j, k)) return;
-    cell.x[n] = Real(0.5) * (face.x[n] + face.x[nbr(g, i, j, k, -1, 0, 0)]);
-    cell.y[n] = Real(0.5) * (face.y[n] + face.y[nbr(g, i, j, k, 0, -1, 0)]);
-    cell.z[n] = Real(0.5) * (face.z[n] + face.z[nbr(g, i, j, k, 0, 0, -1)]);
+__device__ __forceinline__ void face2cell_at(const Geom& g, int i, int j, int k, int n, const V3<Real>& face,
+                                             Real& bx, Real& by, Real& bz) {
+    bx = Real(0.5) * (face.x[n] + face.x[nbr(g, i, j, k, -1, 0, 0)]);
+    by = Real(0.5) * (face.y[n] + face.y[nbr(g, i, j, k, 0, -1, 0)]);
+    bz = Real(0.5) * (face.z[n] + face.z[nbr(g, i, j, k, 0, 0, -1)]);
 }
 
 // cell2Node (:1059-1089): node = mean of the 8 cells around the upper corner
@@ -59,25 +58,42 @@ __global__ __launch_bounds__(256) void k_cell2node(Geom g, const Real* __restric
     }
 }
 
-// node2Cell (:1092-1112): cell = mean of its 8 corner nodes
+// cell2Node of several arrays in one launch (independent arrays: the same
+// arithmetic as one launch each)
 template <typename Real>
-__global__ __launch_bounds__(256) void k_node2cell(Geom g, V3<Real> node, V3<Real> cell) {
+struct Cell2NodeSet {
+    const Real* from[3];
+    Real* to[3];
+    int dim[3];
+    int n;
+};
+template <typename Real>
+__global__ __launch_bounds__(256) void k_cell2node_set(Geom g, Cell2NodeSet<Real> set) {
     RH_CELL_PROLOGUE
-    if (!all_nbrs(g, i, j, k)) return;
-    const int m100 = nbr(g, i, j, k, -1, 0, 0), m110 = nbr(g, i, j, k, -1, -1, 0), m010 = nbr(g, i, j, k, 0, -1, 0);
-    const int m001 = nbr(g, i, j, k, 0, 0, -1), m011 = nbr(g, i, j, k, 0, -1, -1), m101 = nbr(g, i, j, k, -1, 0, -1);
-    const int m111 = nbr(g, i, j, k, -1, -1, -1);
-    for (int l = 0; l < 3; ++l) {
-        const Real* a = l == 0 ? node.x : (l == 1 ? node.y : node.z);
-        (l == 0 ? cell.x : (l == 1 ? cell.y : cell.z))[n] =
-            Real(0.125) * (a[n] + a[m100] + a[m110] + a[m010] + a[m001] + a[m011] + a[m101] + a[m111]);
-    }
+    if (upper_missing(g, i, j, k)) return;
+    const int c001 = nbr(g, i, j, k, 0, 0, 1), c010 = nbr(g, i, j, k, 0, 1, 0), c100 = nbr(g, i, j, k, 1, 0, 0);
+    const int c011 = nbr(g, i, j, k, 0, 1, 1), c110 = nbr(g, i, j, k, 1, 1, 0), c101 = nbr(g, i, j, k, 1, 0, 1);
+    const int c111 = nbr(g, i, j, k, 1, 1, 1);
+    for (int s = 0; s < set.n; ++s)
+        for (int l = 0; l < set.dim[s]; ++l) {
+            const Real* a = set.from[s] + (size_t)l * g.nc;
+            set.to[s][(size_t)l * g.nc + n] =
+                Real(0.125) * (a[n] + a[c001] + a[c010] + a[c100] + a[c011] + a[c110] + a[c101] + a[c111]);
+        }
 }
+
 
 // neumannCell (:443-655).  Walls, then edges, then corners overwrite one
 // another, so an exterior cell ends with the value of the neighbour lying
 // opposite to all of its missing directions; that neighbour is never an
 // exterior cell, which makes the cells independent.
+__device__ __forceinline__ int neumann_source(const Geom& g, int i, int j, int k) {
+    const int sx = has_neg(i, g.per[0]) ? (has_pos(i, g.nx, g.per[0]) ? 0 : -1) : 1;
+    const int sy = has_neg(j, g.per[1]) ? (has_pos(j, g.ny, g.per[1]) ? 0 : -1) : 1;
+    const int sz = has_neg(k, g.per[2]) ? (has_pos(k, g.nz, g.per[2]) ? 0 : -1) : 1;
+    return nbr(g, i, j, k, sx, sy, sz);
+}
+
 template <typename Real>
 __global__ __launch_bounds__(256) void k_neumann_cell(Geom g, const int* __restrict__ ext, int next,
                                                      Real* __restrict__ D, int dim) {
@@ -86,10 +102,7 @@ __global__ __launch_bounds__(256) void k_neumann_cell(Geom g, const int* __restr
     const int n = ext[q];
     int i, j, k;
     cell_ijk(g, n, i, j, k);
-    const int sx = has_neg(i, g.per[0]) ? (has_pos(i, g.nx, g.per[0]) ? 0 : -1) : 1;
-    const int sy = has_neg(j, g.per[1]) ? (has_pos(j, g.ny, g.per[1]) ? 0 : -1) : 1;
-    const int sz = has_neg(k, g.per[2]) ? (has_pos(k, g.nz, g.per[2]) ? 0 : -1) : 1;
-    const int src = nbr(g, i, j, k, sx, sy, sz);
+    const int src = neumann_source(g, i, j, k);
     if (src < 0) return;
     for (int l = 0; l < dim; ++l) D[(size_t)l * g.nc + n] = D[(size_t)l * g.nc + src];
 }
@@ -155,13 +168,8 @@ __global__ __launch_bounds__(256) void k_neumann_face(Geom g, const int* __restr
 
 // setIMFCell / setIMFFace (:910-1040); wall order +x -x +y -y +z -z
 template <typename Real>
-__global__ __launch_bounds__(256) void k_imf(Geom g, const int* __restrict__ ext, int next, V3<Real> D,
-                                            int face, int flags, Real bx, Real by, Real bz) {
-    const int q = blockIdx.x * 256 + threadIdx.x;
-    if (q >= next) return;
-    const int n = ext[q];
-    int i, j, k;
-    cell_ijk(g, n, i, j, k);
+__device__ __forceinline__ void imf_at(const Geom& g, int i, int j, int k, int n, const V3<Real>& D, int face,
+                                       int flags, Real bx, Real by, Real bz) {
     const int miss = missing_mask(g, i, j, k);
     // missing-mask bit of each wall in the +x -x +y -y +z -z order
     const int wall_bit[6] = {2, 1, 8, 4, 32, 16};
@@ -172,6 +180,45 @@ __global__ __launch_bounds__(256) void k_imf(Geom g, const int* __restrict__ ext
         if (!face || axis != 1) D.y[n] = by;
         if (!face || axis != 2) D.z[n] = bz;
     }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_imf(Geom g, const int* __restrict__ ext, int next, V3<Real> D,
+                                            int face, int flags, Real bx, Real by, Real bz) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= next) return;
+    const int n = ext[q];
+    int i, j, k;
+    cell_ijk(g, n, i, j, k);
+    imf_at(g, i, j, k, n, D, face, flags, bx, by, bz);
+}
+
+// face2Cell, then the exterior cells of cellB, cellJi and cellRhoQi by
+// neumannCell, then setIMFCell on cellB (:122-155), in one launch.  An
+// exterior cell's Neumann source is never exterior, so its face2Cell value is
+// computed from faceB here as the launch's own thread for it computes it;
+// cellJi and cellRhoQi are not written by this launch.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_cell_B_bc(Geom g, V3<Real> face, V3<Real> cellB, V3<Real> Ji,
+                                                  Real* __restrict__ rho, int imf_flags, Real bx, Real by, Real bz) {
+    RH_CELL_PROLOGUE
+    if (all_nbrs(g, i, j, k)) {
+        Real cx, cy, cz;
+        face2cell_at(g, i, j, k, n, face, cx, cy, cz);
+        cellB.x[n] = cx; cellB.y[n] = cy; cellB.z[n] = cz;
+        return;
+    }
+    const int src = neumann_source(g, i, j, k);
+    if (src >= 0) {
+        int si, sj, sk;
+        cell_ijk(g, src, si, sj, sk);
+        Real cx, cy, cz;
+        face2cell_at(g, si, sj, sk, src, face, cx, cy, cz);
+        cellB.x[n] = cx; cellB.y[n] = cy; cellB.z[n] = cz;
+        Ji.x[n] = Ji.x[src]; Ji.y[n] = Ji.y[src]; Ji.z[n] = Ji.z[src];
+        rho[n] = rho[src];
+    }
+    if (imf_flags) imf_at(g, i, j, k, n, cellB, 0, imf_flags, bx, by, bz);
 }
 
 // calcNodeJ (:1449-1522): nodeJ = curl(faceB)/mu0 on edges averaged to the
@@ -211,52 +258,94 @@ __global__ __launch_bounds__(256) void k_node_J(Geom g, V3<Real> F, V3<Real> nod
 
 // calcCellUe (:1342-1387) and calcNodeUe (:1524-1584; NODE clamps rho in place)
 template <typename Real, bool NODE>
-__global__ __launch_bounds__(256) void k_ue(Geom g, V3<Real> J, V3<Real> Ji, Real* __restrict__ rho,
-                                           const uint8_t* __restrict__ inner, V3<Real> Ue, int use_hall,
-                                           Real ox_, Real oy_, Real oz_, Real max_ue2, Real min_rho,
-                                           unsigned long long* counters) {
+__device__ __forceinline__ void ue_at(int n, const Real (&jj)[3], const V3<Real>& Ji, Real* __restrict__ rho,
+                                      const uint8_t* __restrict__ inner, int use_hall, Real ox_, Real oy_, Real oz_,
+                                      Real max_ue2, Real min_rho, Real (&u)[3], bool& c_ue, bool& c_rho) {
+    if (inner[n]) {
+        u[0] = ox_; u[1] = oy_; u[2] = oz_;
+        return;
+    }
+    Real r = rho[n];
+    if (NODE && r < min_rho) { r = min_rho; rho[n] = r; c_rho = true; }
+    const Real ji[3] = {Ji.x[n], Ji.y[n], Ji.z[n]};
+    for (int l = 0; l < 3; ++l) {
+        if (fabs(r) > Real(0)) u[l] = use_hall ? -(jj[l] - ji[l]) / r : ji[l] / r;
+        else u[l] = Real(0);
+    }
+    const Real u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    if (u2 > max_ue2) {
+        const Real norm = sqrt(max_ue2 / u2);
+        u[0] *= norm; u[1] *= norm; u[2] *= norm;
+        c_ue = true;
+    }
+}
+
+// node2Cell of nodeJ into cellJ, calcCellUe, and the exterior cells of
+// cellUe by neumannCell (:226-297), in one launch.  An exterior cell's
+// Neumann source is never exterior: its thread computes the source's cellJ
+// and Ue as the source's own thread does (without counting the clamp twice).
+template <typename Real>
+__device__ __forceinline__ void node2cell_at(const Geom& g, int i, int j, int k, int n, const V3<Real>& node,
+                                             Real (&c)[3]) {
+    const int m100 = nbr(g, i, j, k, -1, 0, 0), m110 = nbr(g, i, j, k, -1, -1, 0), m010 = nbr(g, i, j, k, 0, -1, 0);
+    const int m001 = nbr(g, i, j, k, 0, 0, -1), m011 = nbr(g, i, j, k, 0, -1, -1), m101 = nbr(g, i, j, k, -1, 0, -1);
+    const int m111 = nbr(g, i, j, k, -1, -1, -1);
+    for (int l = 0; l < 3; ++l) {
+        const Real* a = l == 0 ? node.x : (l == 1 ? node.y : node.z);
+        c[l] = Real(0.125) * (a[n] + a[m100] + a[m110] + a[m010] + a[m001] + a[m011] + a[m101] + a[m111]);
+    }
+}
+
+// node2Cell (:1092-1112): cell = mean of its 8 corner nodes
+template <typename Real>
+__global__ __launch_bounds__(256) void k_node2cell(Geom g, V3<Real> node, V3<Real> cell) {
+    RH_CELL_PROLOGUE
+    if (!all_nbrs(g, i, j, k)) return;
+    Real c[3];
+    node2cell_at(g, i, j, k, n, node, c);
+    cell.x[n] = c[0]; cell.y[n] = c[1]; cell.z[n] = c[2];
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_cell_J_Ue(Geom g, V3<Real> nodeJ, V3<Real> cellJ, V3<Real> Ji,
+                                                  Real* __restrict__ rho, const uint8_t* __restrict__ inner,
+                                                  V3<Real> Ue, int use_hall, Real ox_, Real oy_, Real oz_,
+                                                  Real max_ue2, Real min_rho, unsigned long long* counters) {
     RH_CELL_IJK
     bool c_ue = false, c_rho = false;
     if (in_grid) {
-        const bool run = NODE ? !upper_missing(g, i, j, k) : all_nbrs(g, i, j, k);
-        if (run) {
-            if (inner[n]) {
-                Ue.x[n] = ox_; Ue.y[n] = oy_; Ue.z[n] = oz_;
-            } else {
-                Real r = rho[n];
-                if (NODE && r < min_rho) { r = min_rho; rho[n] = r; c_rho = true; }
-                Real u[3];
-                const Real jj[3] = {J.x[n], J.y[n], J.z[n]};
-                const Real ji[3] = {Ji.x[n], Ji.y[n], Ji.z[n]};
-                for (int l = 0; l < 3; ++l) {
-                    if (fabs(r) > Real(0)) u[l] = use_hall ? -(jj[l] - ji[l]) / r : ji[l] / r;
-                    else u[l] = Real(0);
-                }
-                const Real u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-                if (u2 > max_ue2) {
-                    const Real norm = sqrt(max_ue2 / u2);
-                    u[0] *= norm; u[1] *= norm; u[2] *= norm;
-                    c_ue = true;
-                }
+        if (all_nbrs(g, i, j, k)) {
+            Real cj[3], u[3];
+            node2cell_at(g, i, j, k, n, nodeJ, cj);
+            cellJ.x[n] = cj[0]; cellJ.y[n] = cj[1]; cellJ.z[n] = cj[2];
+            ue_at<Real, false>(n, cj, Ji, rho, inner, use_hall, ox_, oy_, oz_, max_ue2, min_rho, u, c_ue, c_rho);
+            Ue.x[n] = u[0]; Ue.y[n] = u[1]; Ue.z[n] = u[2];
+        } else {
+            const int src = neumann_source(g, i, j, k);
+            if (src >= 0) {
+                int si, sj, sk;
+                cell_ijk(g, src, si, sj, sk);
+                Real cj[3], u[3];
+                bool c0 = false, c1 = false;
+                node2cell_at(g, si, sj, sk, src, nodeJ, cj);
+                ue_at<Real, false>(src, cj, Ji, rho, inner, use_hall, ox_, oy_, oz_, max_ue2, min_rho, u, c0, c1);
                 Ue.x[n] = u[0]; Ue.y[n] = u[1]; Ue.z[n] = u[2];
             }
         }
     }
-    count_event(counters, NODE ? C_MAX_NODE_UE : C_MAX_CELL_UE, c_ue);
-    if (NODE) count_event(counters, C_MIN_NODE_RHO, c_rho);
+    count_event(counters, C_MAX_CELL_UE, c_ue);
 }
 
 // upwindNodeB (:1268-1339): inverse-distance weights of the 8 cells around
 // the node, seen from a point displaced by -Ue/|Ue|/2 (default (-1/2,0,0)).
 template <typename Real>
-__global__ __launch_bounds__(256) void k_upwind_B(Geom g, V3<Real> cellB, V3<Real> Ue, V3<Real> nodeB) {
-    RH_CELL_PROLOGUE
-    if (upper_missing(g, i, j, k)) return;
+__device__ __forceinline__ void upwind_B_at(const Geom& g, int i, int j, int k, const V3<Real>& cellB,
+                                            const Real (&ue3)[3], Real (&b)[3]) {
     Real xu = Real(-0.5), yu = Real(0), zu = Real(0);
-    const Real ue = sqrt(Ue.x[n] * Ue.x[n] + Ue.y[n] * Ue.y[n] + Ue.z[n] * Ue.z[n]);
+    const Real ue = sqrt(ue3[0] * ue3[0] + ue3[1] * ue3[1] + ue3[2] * ue3[2]);
     if (ue > Real(0)) {
         const Real a = Real(-0.5) / ue;
-        xu = Ue.x[n] * a; yu = Ue.y[n] * a; zu = Ue.z[n] * a;
+        xu = ue3[0] * a; yu = ue3[1] * a; zu = ue3[2] * a;
     }
     // reference order 111,112,121,211,122,221,212,222 (offsets 0 or +1)
     const int off[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}, {0, 1, 1}, {1, 1, 0}, {1, 0, 1}, {1, 1, 1}};
@@ -278,38 +367,60 @@ __global__ __launch_bounds__(256) void k_upwind_B(Geom g, V3<Real> cellB, V3<Rea
             by += w[q] * cellB.y[c[q]];
             bz += w[q] * cellB.z[c[q]];
         }
-        nodeB.x[n] = bx / wsum; nodeB.y[n] = by / wsum; nodeB.z[n] = bz / wsum;
+        b[0] = bx / wsum; b[1] = by / wsum; b[2] = bz / wsum;
     } else {
-        nodeB.x[n] = nodeB.y[n] = nodeB.z[n] = Real(0);
+        b[0] = b[1] = b[2] = Real(0);
     }
 }
 
 // calcNodeE (:1391-1446): E = -Ue x B + eta J, |E| <= Emax
 template <typename Real>
-__global__ __launch_bounds__(256) void k_node_E(Geom g, V3<Real> Ue, V3<Real> B, const Real* __restrict__ eta,
-                                               V3<Real> J, V3<Real> E, Real max_E2, unsigned long long* counters) {
-    RH_CELL_IJK
+__device__ __forceinline__ bool node_E_at(const Real (&u)[3], const Real (&b)[3], Real eta, Real jx, Real jy,
+                                          Real jz, Real max_E2, Real (&e)[3]) {
+    Real ex = -(u[1] * b[2] - u[2] * b[1]);
+    Real ey = -(u[2] * b[0] - u[0] * b[2]);
+    Real ez = -(u[0] * b[1] - u[1] * b[0]);
+    ex += eta * jx;
+    ey += eta * jy;
+    ez += eta * jz;
     bool c = false;
-    if (in_grid) {
-        if (!upper_missing(g, i, j, k)) {
-            Real ex = -(Ue.y[n] * B.z[n] - Ue.z[n] * B.y[n]);
-            Real ey = -(Ue.z[n] * B.x[n] - Ue.x[n] * B.z[n]);
-            Real ez = -(Ue.x[n] * B.y[n] - Ue.y[n] * B.x[n]);
-            ex += eta[n] * J.x[n];
-            ey += eta[n] * J.y[n];
-            ez += eta[n] * J.z[n];
-            if (max_E2 > Real(0)) {
-                const Real e2 = ex * ex + ey * ey + ez * ez;
-                if (e2 > max_E2) {
-                    const Real s = sqrt(max_E2 / e2);
-                    ex *= s; ey *= s; ez *= s;
-                    c = true;
-                }
-            }
-            E.x[n] = ex; E.y[n] = ey; E.z[n] = ez;
+    if (max_E2 > Real(0)) {
+        const Real e2 = ex * ex + ey * ey + ez * ez;
+        if (e2 > max_E2) {
+            const Real s = sqrt(max_E2 / e2);
+            ex *= s; ey *= s; ez *= s;
+            c = true;
         }
     }
-    count_event(counters, C_MAX_E, c);
+    e[0] = ex; e[1] = ey; e[2] = ez;
+    return c;
+}
+
+// calcNodeUe, upwindNodeB and calcNodeE (:300-352) in one launch: the three
+// run on the same nodes (cells with their +x, +y, +z neighbours) and each
+// reads only the node's own result of the one before (and cellB, nodeJ,
+// nodeJi, nodeRhoQi, nodeEta, which none of them writes).
+template <typename Real>
+__global__ __launch_bounds__(256) void k_node_Ue_B_E(Geom g, V3<Real> J, V3<Real> Ji, Real* __restrict__ rho,
+                                                    const uint8_t* __restrict__ inner, V3<Real> Ue, int use_hall,
+                                                    Real ox_, Real oy_, Real oz_, Real max_ue2, Real min_rho,
+                                                    V3<Real> cellB, V3<Real> nodeB, const Real* __restrict__ eta,
+                                                    V3<Real> E, Real max_E2, unsigned long long* counters) {
+    RH_CELL_IJK
+    bool c_ue = false, c_rho = false, c_e = false;
+    if (in_grid && !upper_missing(g, i, j, k)) {
+        Real u[3], b[3], e[3];
+        const Real jj[3] = {J.x[n], J.y[n], J.z[n]};
+        ue_at<Real, true>(n, jj, Ji, rho, inner, use_hall, ox_, oy_, oz_, max_ue2, min_rho, u, c_ue, c_rho);
+        Ue.x[n] = u[0]; Ue.y[n] = u[1]; Ue.z[n] = u[2];
+        upwind_B_at(g, i, j, k, cellB, u, b);
+        nodeB.x[n] = b[0]; nodeB.y[n] = b[1]; nodeB.z[n] = b[2];
+        c_e = node_E_at(u, b, eta[n], J.x[n], J.y[n], J.z[n], max_E2, e);
+        E.x[n] = e[0]; E.y[n] = e[1]; E.z[n] = e[2];
+    }
+    count_event(counters, C_MAX_NODE_UE, c_ue);
+    count_event(counters, C_MIN_NODE_RHO, c_rho);
+    count_event(counters, C_MAX_E, c_e);
 }
 
 // nodeAvg (:1115-1265).  The reference's gather buffer starts as
