@@ -128,7 +128,7 @@ def gpu_bench(args):
     names = ["push_fields", "push", "sort_slots", "sort_count", "scan", "sort_base", "move", "deposit",
              "deposit_dense", "fold", "boundary", "inject_slots", "inject_count", "inject_fill", "sort_still",
              "cell_B_bc", "neumann", "imf", "cell2node", "node_J", "node2cell", "cell_J_Ue", "node_Ue_B_E",
-             "node_avg", "face_curl", "cell_Ep", "fill", "diag"]
+             "node_avg", "face_curl_Ep", "fill", "diag"]
     ktimes = {}
     for nm in names:
         ms, n = ctx.kernel_time(nm)

@@ -733,15 +733,18 @@ public:
 
     // The blocks of the push fields are rewritten when the fields have
     // changed; both passes of every species then read the same blocks.
+    void alloc_push_fields() {
+        if (pf_tile.p) return;
+        pf_tile.alloc((size_t)g.ntiles * push_field_words<Real>(g));
+        // halo entries of cells outside a non-periodic grid stay zero
+        launch("fill", k_fill<uint32_t>, cdiv(pf_tile.n, 256), 256, 0, pf_tile.p, 0u, pf_tile.n);
+    }
+
     void prepare_push_fields() {
         if (!pf_dirty) return;
         for (auto& s_ : sp)
             if (s_->pushed) throw StateError("fields changed between propagate and wait_particle_sends");
-        if (!pf_tile.p) {
-            pf_tile.alloc((size_t)g.ntiles * push_field_words<Real>(g));
-            // halo entries of cells outside a non-periodic grid stay zero
-            launch("fill", k_fill<uint32_t>, cdiv(pf_tile.n, 256), 256, 0, pf_tile.p, 0u, pf_tile.n);
-        }
+        alloc_push_fields();
         const V3<Real> fb = v3(RH_FACE_B), ue = v3(RH_CELL_UE), ep = v3(RH_CELL_EP);
         PushFields<Real> pf{fb.x, fb.y, fb.z, ue.x, ue.y, ue.z, ep.x, ep.y, ep.z};
         launch("push_fields", k_push_fields<Real>, cell_grid(), cell_block(), 0, g, pf, (int)P.use_e_pressure,
@@ -851,11 +854,13 @@ public:
             launch("node_avg", k_node_avg<Real>, cb, bb, 0, g, old, v3(RH_NODE_E), (Real)P.gauss_coeffs[0],
                    (Real)P.gauss_coeffs[1], (Real)P.gauss_coeffs[2], (Real)P.gauss_coeffs[3]);
         }
-        launch("face_curl", k_face_curl<Real>, cb, bb, 0, g, v3(RH_NODE_E), v3(RH_FACE_B), (Real)P.dt);
-        if (P.use_e_pressure)
-            launch("cell_Ep", k_cell_Ep<Real>, cb, bb, 0, g, (const Real*)fld[RH_NODE_RHOQ].p,
-                   (const Real*)fld[RH_CELL_RHOQ].p, (const uint8_t*)flg[RH_INNER_CELLEP].p, v3(RH_CELL_EP),
-                   (Real)P.e_pressure_coeff, P.isothermal);
+        // faceCurl, calcCellEp and the next push's per-tile field blocks
+        alloc_push_fields();
+        launch("face_curl_Ep", k_face_curl_Ep<Real>, cb, bb, 0, g, v3(RH_NODE_E), v3(RH_FACE_B), (Real)P.dt,
+               (int)P.use_e_pressure, (const Real*)fld[RH_NODE_RHOQ].p, (const Real*)fld[RH_CELL_RHOQ].p,
+               (const uint8_t*)flg[RH_INNER_CELLEP].p, v3(RH_CELL_EP), (Real)P.e_pressure_coeff, P.isothermal,
+               v3(RH_CELL_UE), pf_tile.p);
+        pf_dirty = false;
     }
 
     void step(int n) override {

@@ -463,9 +463,8 @@ __global__ __launch_bounds__(256) void k_node_avg(Geom g, V3<Real> Eold, V3<Real
 
 // faceCurl, Faraday (:1587-1671): faceB -= dt curl(nodeE)
 template <typename Real>
-__global__ __launch_bounds__(256) void k_face_curl(Geom g, V3<Real> E, V3<Real> F, Real dt) {
-    RH_CELL_PROLOGUE
-    if (upper_missing(g, i, j, k)) return;
+__device__ __forceinline__ void face_curl_at(const Geom& g, int i, int j, int k, int n, const V3<Real>& E,
+                                             const V3<Real>& F, Real dt) {
     const bool xn = has_neg(i, g.per[0]), yn = has_neg(j, g.per[1]), zn = has_neg(k, g.per[2]);
     const bool doX = yn && zn, doY = xn && zn, doZ = xn && yn;
     const Real dx = (Real)g.dx;
@@ -490,10 +489,9 @@ __global__ __launch_bounds__(256) void k_face_curl(Geom g, V3<Real> E, V3<Real> 
 
 // calcCellEp (:1674-1737): Ep = -coeff grad(rho)/rho (isothermal) or -coeff grad(rho)
 template <typename Real>
-__global__ __launch_bounds__(256) void k_cell_Ep(Geom g, const Real* __restrict__ nrho, const Real* __restrict__ crho,
-                                                const uint8_t* __restrict__ inner, V3<Real> Ep, Real coeff, int isothermal) {
-    RH_CELL_PROLOGUE
-    if (!all_nbrs(g, i, j, k)) return;
+__device__ __forceinline__ void cell_Ep_at(const Geom& g, int i, int j, int k, int n, const Real* __restrict__ nrho,
+                                           const Real* __restrict__ crho, const uint8_t* __restrict__ inner,
+                                           const V3<Real>& Ep, Real coeff, int isothermal) {
     Ep.x[n] = Ep.y[n] = Ep.z[n] = Real(0);
     if (inner[n]) return;
     // node values at offsets (a,b,c) in {-1,0}^3
@@ -510,6 +508,25 @@ __global__ __launch_bounds__(256) void k_cell_Ep(Geom g, const Real* __restrict_
         Ep.x[n] = -coeff * gx; Ep.y[n] = -coeff * gy; Ep.z[n] = -coeff * gz;
     } else if (fabs(crho[n]) > Real(0)) {
         Ep.x[n] = -coeff * gx / crho[n]; Ep.y[n] = -coeff * gy / crho[n]; Ep.z[n] = -coeff * gz / crho[n];
+    }
+}
+
+// faceCurl (Faraday) and calcCellEp (:406-432), then, when out is not null,
+// the cell's entries of the per-tile push-field blocks (k_push_fields) from
+// its final faceB, cellUe and cellEp, in one launch: each stage reads only
+// arrays no stage writes (nodeE, node and cell rho, flags, cellUe) and the
+// cell's own results.
+template <typename Real>
+__global__ __launch_bounds__(256) void k_face_curl_Ep(Geom g, V3<Real> E, V3<Real> F, Real dt, int use_e_pressure,
+                                                     const Real* __restrict__ nrho, const Real* __restrict__ crho,
+                                                     const uint8_t* __restrict__ inner, V3<Real> Ep, Real coeff,
+                                                     int isothermal, V3<Real> Ue, uint32_t* __restrict__ out) {
+    RH_CELL_PROLOGUE
+    if (!upper_missing(g, i, j, k)) face_curl_at(g, i, j, k, n, E, F, dt);
+    if (use_e_pressure && all_nbrs(g, i, j, k)) cell_Ep_at(g, i, j, k, n, nrho, crho, inner, Ep, coeff, isothermal);
+    if (out) {
+        const PushFields<Real> pf{F.x, F.y, F.z, Ue.x, Ue.y, Ue.z, Ep.x, Ep.y, Ep.z};
+        push_fields_at(g, i, j, k, n, pf, use_e_pressure, out);
     }
 }
 

@@ -128,11 +128,8 @@ RH_HD uint32_t neighbour_mask(const Geom& g, int i, int j, int k) {
 // cells that do not exist are never written and keep the zero of the
 // allocation.
 template <typename Real>
-__global__ __launch_bounds__(256) void k_push_fields(Geom g, PushFields<Real> f, int use_e_pressure,
-                                                     uint32_t* __restrict__ out) {
-    const int i = blockIdx.x * 32 + threadIdx.x, j = blockIdx.y * 8 + threadIdx.y, k = blockIdx.z;
-    if (i >= g.nx || j >= g.ny) return;
-    const int n = cell_index(g, i, j, k);
+__device__ __forceinline__ void push_fields_at(const Geom& g, int i, int j, int k, int n, const PushFields<Real>& f,
+                                               int use_e_pressure, uint32_t* __restrict__ out) {
     const int hx = g.tx + 1, hy = g.ty + 1, nh = (g.tx + 1) * (g.ty + 1) * (g.tz + 1);
     const size_t nw = push_field_words<Real>(g);
     const int li = i & (g.tx - 1), lj = j & (g.ty - 1), lk = k & (g.tz - 1);
@@ -163,6 +160,14 @@ __global__ __launch_bounds__(256) void k_push_fields(Geom g, PushFields<Real> f,
                 const int h = (a ? 0 : li + 1) + hx * ((b ? 0 : lj + 1) + hy * (c ? 0 : lk + 1));
                 blk[h] = bx; blk[nh + h] = by; blk[2 * nh + h] = bz;
             }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void k_push_fields(Geom g, PushFields<Real> f, int use_e_pressure,
+                                                     uint32_t* __restrict__ out) {
+    const int i = blockIdx.x * 32 + threadIdx.x, j = blockIdx.y * 8 + threadIdx.y, k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    push_fields_at(g, i, j, k, cell_index(g, i, j, k), f, use_e_pressure, out);
 }
 
 // Injected particles waiting for their first push: per tile, the range of
