@@ -819,18 +819,21 @@ public:
         // face2Cell, Neumann on cellB, cellJi, cellRhoQi and the IMF on cellB
         launch("cell_B_bc", k_cell_B_bc<Real>, cb, bb, 0, g, v3(RH_FACE_B), v3(RH_CELL_B), v3(RH_CELL_JI),
                fld[RH_CELL_RHOQ].p, imf_cell, bx, by, bz);
+        // Neumann and IMF on faceB (they do not touch the cell arrays that
+        // cell2Node reads), then cell2Node of cellB, cellRhoQi, cellJi and
+        // calcNodeJ in one launch
+        launch("neumann", k_neumann_face<Real>, eb, 256, 0, g, (const int*)ext_cells.p, n_ext, v3(RH_FACE_B));
+        if (imf_face) launch("imf", k_imf<Real>, eb, 256, 0, g, (const int*)ext_cells.p, n_ext, v3(RH_FACE_B), 1, imf_face, bx, by, bz);
         {
+            // the order k_node_J reads: nodeB (3), nodeRhoQi (1), nodeJi (3)
             Cell2NodeSet<Real> set{};
             const int from[3] = {RH_CELL_B, RH_CELL_RHOQ, RH_CELL_JI}, to[3] = {RH_NODE_B, RH_NODE_RHOQ, RH_NODE_JI};
             const int dim[3] = {3, 1, 3};
             for (int q = 0; q < 3; ++q) { set.from[q] = fld[from[q]].p; set.to[q] = fld[to[q]].p; set.dim[q] = dim[q]; }
             set.n = 3;
-            launch("cell2node", k_cell2node_set<Real>, cb, bb, 0, g, set);
+            launch("node_J", k_node_J<Real>, cb, bb, 0, g, set, v3(RH_FACE_B), v3(RH_NODE_J), (Real)P.max_whistler,
+                   dcount.p);
         }
-        launch("neumann", k_neumann_face<Real>, eb, 256, 0, g, (const int*)ext_cells.p, n_ext, v3(RH_FACE_B));
-        if (imf_face) launch("imf", k_imf<Real>, eb, 256, 0, g, (const int*)ext_cells.p, n_ext, v3(RH_FACE_B), 1, imf_face, bx, by, bz);
-        launch("node_J", k_node_J<Real>, cb, bb, 0, g, v3(RH_FACE_B), v3(RH_NODE_B), (const Real*)fld[RH_NODE_RHOQ].p,
-               v3(RH_NODE_J), (Real)P.max_whistler, dcount.p);
         // node2Cell of nodeJ, calcCellUe, Neumann on cellUe
         launch("cell_J_Ue", k_cell_J_Ue<Real>, cb, bb, 0, g, v3(RH_NODE_J), v3(RH_CELL_J), v3(RH_CELL_JI),
                fld[RH_CELL_RHOQ].p, (const uint8_t*)flg[RH_INNER_FIELD].p, v3(RH_CELL_UE), P.use_hall,

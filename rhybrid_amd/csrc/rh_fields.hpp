@@ -58,8 +58,7 @@ __global__ __launch_bounds__(256) void k_cell2node(Geom g, const Real* __restric
     }
 }
 
-// cell2Node of several arrays in one launch (independent arrays: the same
-// arithmetic as one launch each)
+// cell2Node of several arrays in one launch (k_node_J)
 template <typename Real>
 struct Cell2NodeSet {
     const Real* from[3];
@@ -67,22 +66,6 @@ struct Cell2NodeSet {
     int dim[3];
     int n;
 };
-template <typename Real>
-__global__ __launch_bounds__(256) void k_cell2node_set(Geom g, Cell2NodeSet<Real> set) {
-    RH_CELL_PROLOGUE
-    if (upper_missing(g, i, j, k)) return;
-    const int c001 = nbr(g, i, j, k, 0, 0, 1), c010 = nbr(g, i, j, k, 0, 1, 0), c100 = nbr(g, i, j, k, 1, 0, 0);
-    const int c011 = nbr(g, i, j, k, 0, 1, 1), c110 = nbr(g, i, j, k, 1, 1, 0), c101 = nbr(g, i, j, k, 1, 0, 1);
-    const int c111 = nbr(g, i, j, k, 1, 1, 1);
-    for (int s = 0; s < set.n; ++s)
-        for (int l = 0; l < set.dim[s]; ++l) {
-            const Real* a = set.from[s] + (size_t)l * g.nc;
-            set.to[s][(size_t)l * g.nc + n] =
-                Real(0.125) * (a[n] + a[c001] + a[c010] + a[c100] + a[c011] + a[c110] + a[c101] + a[c111]);
-        }
-}
-
-
 // neumannCell (:443-655).  Walls, then edges, then corners overwrite one
 // another, so an exterior cell ends with the value of the neighbour lying
 // opposite to all of its missing directions; that neighbour is never an
@@ -224,34 +207,58 @@ __global__ __launch_bounds__(256) void k_cell_B_bc(Geom g, V3<Real> face, V3<Rea
 // calcNodeJ (:1449-1522): nodeJ = curl(faceB)/mu0 on edges averaged to the
 // node, divided by the whistler-limiter factor d.
 template <typename Real>
-__global__ __launch_bounds__(256) void k_node_J(Geom g, V3<Real> F, V3<Real> nodeB, const Real* __restrict__ nodeRho,
-                                               V3<Real> nodeJ, Real max_vw, unsigned long long* counters) {
+__device__ __forceinline__ bool node_J_at(const Geom& g, int i, int j, int k, int n, const V3<Real>& F, Real nbx,
+                                          Real nby, Real nbz, Real rho_n, Real max_vw, Real (&J)[3]) {
+    const int c100 = nbr(g, i, j, k, 1, 0, 0), c010 = nbr(g, i, j, k, 0, 1, 0), c001 = nbr(g, i, j, k, 0, 0, 1);
+    const int c110 = nbr(g, i, j, k, 1, 1, 0), c101 = nbr(g, i, j, k, 1, 0, 1), c011 = nbr(g, i, j, k, 0, 1, 1);
+    const Real ex1 = -F.z[n] + F.y[n] + F.z[c010] - F.y[c001];
+    const Real ex2 = -F.z[c100] + F.y[c100] + F.z[c110] - F.y[c101];
+    const Real ey1 = +F.z[n] + F.x[c001] - F.z[c100] - F.x[n];
+    const Real ey2 = +F.z[c010] + F.x[c011] - F.z[c110] - F.x[c010];
+    const Real ez1 = -F.y[n] + F.x[n] + F.y[c100] - F.x[c010];
+    const Real ez2 = -F.y[c001] + F.x[c001] + F.y[c101] - F.x[c011];
+    const Real e = Real(1.60217653e-19);
+    const Real mu0 = Real(4.0e-7 * 3.14159265358979323846);
+    Real d = Real(1), vw = Real(0);
+    bool limited = false;
+    const Real ne = rho_n / e;
+    const Real Btot = sqrt(nbx * nbx + nby * nby + nbz * nbz);
+    const Real dx = (Real)g.dx;
+    if (ne > Real(0) && dx > Real(0))
+        vw = Real(2) * Btot * Real(3.14159265358979323846) / (mu0 * ne * e * dx);
+    if (vw > max_vw && max_vw > Real(0)) { d = vw / max_vw; limited = true; }
+    const Real a = Real(0.5) / (dx * mu0 * d);
+    J[0] = (ex1 + ex2) * a;
+    J[1] = (ey1 + ey2) * a;
+    J[2] = (ez1 + ez2) * a;
+    return limited;
+}
+
+// cell2Node of cellB, cellRhoQi and cellJi, then calcNodeJ (:169-225) in one
+// launch: both run on the same nodes, and calcNodeJ reads the node's own
+// nodeB and nodeRhoQi (and faceB, after its Neumann and IMF copies, which
+// come first: they do not touch the cell arrays cell2Node reads).
+template <typename Real>
+__global__ __launch_bounds__(256) void k_node_J(Geom g, Cell2NodeSet<Real> set, V3<Real> F, V3<Real> nodeJ,
+                                               Real max_vw, unsigned long long* counters) {
     RH_CELL_IJK
     bool limited = false;
-    if (in_grid) {
-        if (!upper_missing(g, i, j, k)) {
-            const int c100 = nbr(g, i, j, k, 1, 0, 0), c010 = nbr(g, i, j, k, 0, 1, 0), c001 = nbr(g, i, j, k, 0, 0, 1);
-            const int c110 = nbr(g, i, j, k, 1, 1, 0), c101 = nbr(g, i, j, k, 1, 0, 1), c011 = nbr(g, i, j, k, 0, 1, 1);
-            const Real ex1 = -F.z[n] + F.y[n] + F.z[c010] - F.y[c001];
-            const Real ex2 = -F.z[c100] + F.y[c100] + F.z[c110] - F.y[c101];
-            const Real ey1 = +F.z[n] + F.x[c001] - F.z[c100] - F.x[n];
-            const Real ey2 = +F.z[c010] + F.x[c011] - F.z[c110] - F.x[c010];
-            const Real ez1 = -F.y[n] + F.x[n] + F.y[c100] - F.x[c010];
-            const Real ez2 = -F.y[c001] + F.x[c001] + F.y[c101] - F.x[c011];
-            const Real e = Real(1.60217653e-19);
-            const Real mu0 = Real(4.0e-7 * 3.14159265358979323846);
-            Real d = Real(1), vw = Real(0);
-            const Real ne = nodeRho[n] / e;
-            const Real Btot = sqrt(nodeB.x[n] * nodeB.x[n] + nodeB.y[n] * nodeB.y[n] + nodeB.z[n] * nodeB.z[n]);
-            const Real dx = (Real)g.dx;
-            if (ne > Real(0) && dx > Real(0))
-                vw = Real(2) * Btot * Real(3.14159265358979323846) / (mu0 * ne * e * dx);
-            if (vw > max_vw && max_vw > Real(0)) { d = vw / max_vw; limited = true; }
-            const Real a = Real(0.5) / (dx * mu0 * d);
-            nodeJ.x[n] = (ex1 + ex2) * a;
-            nodeJ.y[n] = (ey1 + ey2) * a;
-            nodeJ.z[n] = (ez1 + ez2) * a;
-        }
+    if (in_grid && !upper_missing(g, i, j, k)) {
+        const int c001 = nbr(g, i, j, k, 0, 0, 1), c010 = nbr(g, i, j, k, 0, 1, 0), c100 = nbr(g, i, j, k, 1, 0, 0);
+        const int c011 = nbr(g, i, j, k, 0, 1, 1), c110 = nbr(g, i, j, k, 1, 1, 0), c101 = nbr(g, i, j, k, 1, 0, 1);
+        const int c111 = nbr(g, i, j, k, 1, 1, 1);
+        Real v[7];
+        int q = 0;
+        for (int s = 0; s < set.n; ++s)
+            for (int l = 0; l < set.dim[s]; ++l, ++q) {
+                const Real* a = set.from[s] + (size_t)l * g.nc;
+                v[q] = Real(0.125) * (a[n] + a[c001] + a[c010] + a[c100] + a[c011] + a[c110] + a[c101] + a[c111]);
+                set.to[s][(size_t)l * g.nc + n] = v[q];
+            }
+        // set = {cellB -> nodeB (3), cellRhoQi -> nodeRhoQi (1), cellJi -> nodeJi (3)}
+        Real J[3];
+        limited = node_J_at(g, i, j, k, n, F, v[0], v[1], v[2], v[3], max_vw, J);
+        nodeJ.x[n] = J[0]; nodeJ.y[n] = J[1]; nodeJ.z[n] = J[2];
     }
     count_event(counters, C_MAX_VW, limited);
 }
