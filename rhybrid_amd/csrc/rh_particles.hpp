@@ -692,6 +692,8 @@ RH_HD int deposit_chunk(const Geom& g) {
 
 template <typename Real>
 struct alignas(4 * sizeof(Real)) Real4 { Real v[4]; };
+template <typename Real>
+struct alignas(2 * sizeof(Real)) Real2 { Real v[2]; };
 
 // The deposit of a crowded tile (k_deposit sends tiles with at least
 // dense_min particles here): eight lanes share a subcell and read eight
@@ -765,7 +767,7 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
                                                  const uint16_t* __restrict__ glist,
                                                  const uint8_t* __restrict__ gcount, Real* __restrict__ partial,
                                                  uint32_t dense_min) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     const int NS = g.tc * 8;
     const int OB = deposit_octant_block(g);  // octant block (padded)
     const int QS = 8 * OB;            // one (quantity, corner) plane
@@ -789,22 +791,30 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
         for (int c = 0; c < 8; ++c) acc[q][c] = Real(0);
     // The tile's particles go through LDS in chunks read with coalesced
     // loads; each thread then takes its subcell's particles of the chunk, in
-    // particle order (the same sums as reading them directly).
+    // particle order (the same sums as reading them directly).  A particle is
+    // staged as (x, y, z, vx) and (vy, vz): two LDS reads instead of six.
     const int CH = deposit_chunk<Real>(g);
-    Real* L = S;
+    Real4<Real>* L4 = reinterpret_cast<Real4<Real>*>(S);          // x, y, z, vx
+    Real2<Real>* L2 = reinterpret_cast<Real2<Real>*>(S + 4 * CH);  // vy, vz
     for (uint32_t cs = t0; cs < t1; cs += (uint32_t)CH) {
         const int n = (int)min((uint32_t)CH, t1 - cs);
         __syncthreads();  // the previous chunk is consumed
         for (int q = ls; q < n; q += blockDim.x) {
             const uint32_t p = cs + (uint32_t)q;
-            L[q] = A.x[p]; L[CH + q] = A.y[p]; L[2 * CH + q] = A.z[p];
-            L[3 * CH + q] = A.vx[p]; L[4 * CH + q] = A.vy[p]; L[5 * CH + q] = A.vz[p];
+            Real4<Real> r;
+            r.v[0] = A.x[p]; r.v[1] = A.y[p]; r.v[2] = A.z[p]; r.v[3] = A.vx[p];
+            Real2<Real> t;
+            t.v[0] = A.vy[p]; t.v[1] = A.vz[p];
+            L4[q] = r;
+            L2[q] = t;
         }
         __syncthreads();
         const uint32_t a = max(p0, cs), b = min(p1, cs + (uint32_t)n);
         for (uint32_t p = a; p < b; ++p) {
             const int q = (int)(p - cs);
-            cic_accumulate(acc, hx, hy, hz, L[q], L[CH + q], L[2 * CH + q], L[3 * CH + q], L[4 * CH + q], L[5 * CH + q]);
+            const Real4<Real> r = L4[q];
+            const Real2<Real> t = L2[q];
+            cic_accumulate(acc, hx, hy, hz, r.v[0], r.v[1], r.v[2], r.v[3], t.v[0], t.v[1]);
         }
     }
     __syncthreads();  // the staging is read; S may be written
