@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void k_neumann_cell(Geom g, const int* __restr
 // for each component the last applicable rule wins.
 struct FaceRule { uint8_t need; int8_t comp, a, b, c; };
 // need bits: 1 -x, 2 +x, 4 -y, 8 +y, 16 -z, 32 +z (neighbours that must be missing)
-__constant__ FaceRule c_face_rules[48] = {
+constexpr FaceRule kFaceRules[48] = {
     {1, 1, 1, 0, 0}, {1, 2, 1, 0, 0}, {8, 0, 0, -1, 0}, {8, 2, 0, -1, 0},
     {4, 0, 0, 1, 0}, {4, 2, 0, 1, 0}, {32, 0, 0, 0, -1}, {32, 1, 0, 0, -1},
     {16, 0, 0, 0, 1}, {16, 1, 0, 0, 1}, {2, 1, -1, 0, 0}, {2, 2, -1, 0, 0},
@@ -119,6 +119,27 @@ __constant__ FaceRule c_face_rules[48] = {
     {2 | 8 | 16, 2, -1, -1, 0},
 };
 
+// The rule that wins for component l of a cell with missing-neighbour mask
+// miss (the last applicable one), as the source offset (a, b, c) and a
+// found flag: 64 masks x 3 components, built at compile time from the rules.
+struct FaceSrc { int8_t a, b, c, found; };
+struct FaceSrcTable { FaceSrc t[64][3]; };
+constexpr FaceSrcTable make_face_src() {
+    FaceSrcTable T{};
+    for (int miss = 0; miss < 64; ++miss)
+        for (int l = 0; l < 3; ++l) {
+            T.t[miss][l] = FaceSrc{0, 0, 0, 0};
+            for (int r = 47; r >= 0; --r) {
+                const FaceRule R = kFaceRules[r];
+                if (R.comp != l || (miss & R.need) != R.need) continue;
+                T.t[miss][l] = FaceSrc{R.a, R.b, R.c, 1};
+                break;
+            }
+        }
+    return T;
+}
+__constant__ FaceSrcTable c_face_src = make_face_src();
+
 __device__ __forceinline__ int missing_mask(const Geom& g, int i, int j, int k) {
     int m = 0;
     if (!has_neg(i, g.per[0])) m |= 1;
@@ -139,13 +160,10 @@ __global__ __launch_bounds__(256) void k_neumann_face(Geom g, const int* __restr
     cell_ijk(g, n, i, j, k);
     const int miss = missing_mask(g, i, j, k);
     for (int l = 0; l < 3; ++l) {
-        for (int r = 47; r >= 0; --r) {
-            const FaceRule R = c_face_rules[r];
-            if (R.comp != l || (miss & R.need) != R.need) continue;
-            const int src = nbr(g, i, j, k, R.a, R.b, R.c);
-            if (src >= 0) F.c(l, n) = F.c(l, src);
-            break;
-        }
+        const FaceSrc R = c_face_src.t[miss][l];
+        if (!R.found) continue;
+        const int src = nbr(g, i, j, k, R.a, R.b, R.c);
+        if (src >= 0) F.c(l, n) = F.c(l, src);
     }
 }
 
