@@ -870,21 +870,60 @@ __global__ __launch_bounds__(256) void k_fold(Geom g, FoldSources<Real> src, dou
     if (i < g.nx && j < g.ny) {
         const int n = cell_index(g, i, j, k);
         int tx[6], ex[6], ty[6], ey[6], tz[6], ez[6];
-        const int mx = axis_candidates(i, g.nx, g.ltx, g.ntx, g.per[0], tx, ex);
-        const int my = axis_candidates(j, g.ny, g.lty, g.nty, g.per[1], ty, ey);
-        const int mz = axis_candidates(k, g.nz, g.ltz, g.ntz, g.per[2], tz, ez);
+        const bool fx = !g.per[0] || g.ntx >= 3, fy = !g.per[1] || g.nty >= 3, fz = !g.per[2] || g.ntz >= 3;
         double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-        for (int s = 0; s < src.n; ++s) {
-            double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
-            for (int c = 0; c < mz; ++c)
-                for (int b = 0; b < my; ++b)
-                    for (int a = 0; a < mx; ++a) {
-                        const int TT = tx[a] + g.ntx * (ty[b] + g.nty * tz[c]);
-                        const int e = ex[a] + g.ex * (ey[b] + g.ey * ez[c]);
-                        const Real4<Real> v = reinterpret_cast<const Real4<Real>*>(src.partial[s])[(size_t)TT * g.next + e];
-                        t0 += (double)v.v[0]; t1 += (double)v.v[1]; t2 += (double)v.v[2]; t3 += (double)v.v[3];
-                    }
-            s0 += src.wq[s] * t0; s1 += src.wq[s] * t1; s2 += src.wq[s] * t2; s3 += src.wq[s] * t3;
+        if (fx && fy && fz && g.tx >= 2 && g.ty >= 2 && g.tz >= 2) {
+            // at most two candidate tiles per axis: all eight loads of a
+            // species issued before its sum (the same order as below)
+            const int mx = axis_candidates_fast(i, g.ltx, g.ntx, g.per[0], tx, ex);
+            const int my = axis_candidates_fast(j, g.lty, g.nty, g.per[1], ty, ey);
+            const int mz = axis_candidates_fast(k, g.ltz, g.ntz, g.per[2], tz, ez);
+            for (int s = 0; s < src.n; ++s) {
+                const Real4<Real>* part = reinterpret_cast<const Real4<Real>*>(src.partial[s]);
+                Real4<Real> v[8];
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+#pragma unroll
+                        for (int a = 0; a < 2; ++a) {
+                            const int q = a + 2 * b + 4 * c;
+                            if (a < mx && b < my && c < mz) {
+                                const int TT = tx[a] + g.ntx * (ty[b] + g.nty * tz[c]);
+                                const int e = ex[a] + g.ex * (ey[b] + g.ey * ez[c]);
+                                v[q] = part[(size_t)TT * g.next + e];
+                            }
+                        }
+                double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+#pragma unroll
+                        for (int a = 0; a < 2; ++a) {
+                            const int q = a + 2 * b + 4 * c;
+                            if (!(a < mx && b < my && c < mz)) continue;
+                            t0 += (double)v[q].v[0]; t1 += (double)v[q].v[1];
+                            t2 += (double)v[q].v[2]; t3 += (double)v[q].v[3];
+                        }
+                s0 += src.wq[s] * t0; s1 += src.wq[s] * t1; s2 += src.wq[s] * t2; s3 += src.wq[s] * t3;
+            }
+        } else {
+            const int mx = axis_candidates(i, g.nx, g.ltx, g.ntx, g.per[0], tx, ex);
+            const int my = axis_candidates(j, g.ny, g.lty, g.nty, g.per[1], ty, ey);
+            const int mz = axis_candidates(k, g.nz, g.ltz, g.ntz, g.per[2], tz, ez);
+            for (int s = 0; s < src.n; ++s) {
+                double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+                for (int c = 0; c < mz; ++c)
+                    for (int b = 0; b < my; ++b)
+                        for (int a = 0; a < mx; ++a) {
+                            const int TT = tx[a] + g.ntx * (ty[b] + g.nty * tz[c]);
+                            const int e = ex[a] + g.ex * (ey[b] + g.ey * ez[c]);
+                            const Real4<Real> v = reinterpret_cast<const Real4<Real>*>(src.partial[s])[(size_t)TT * g.next + e];
+                            t0 += (double)v.v[0]; t1 += (double)v.v[1]; t2 += (double)v.v[2]; t3 += (double)v.v[3];
+                        }
+                s0 += src.wq[s] * t0; s1 += src.wq[s] * t1; s2 += src.wq[s] * t2; s3 += src.wq[s] * t3;
+            }
         }
         double r = s0 * inv_dV;
         if (r < min_rho) { r = min_rho; floored = true; }
