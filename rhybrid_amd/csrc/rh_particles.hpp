@@ -938,48 +938,51 @@ __global__ __launch_bounds__(256) void k_fold(Geom g, FoldSources<Real> src, dou
 // ParticleBoundaryCondHybrid::apply (src/particle_boundary_cond_hybrid.h:
 // 112-185): empty exterior cells, remove particles inside the obstacle in
 // innerFlagParticle cells.  Removed particles get x = -1; the next sort drops
-// them.  One wave per listed cell, its lanes striding through the cell's
-// particles (coalesced at any ppc).
-constexpr int BC_CELLS_PER_BLOCK = 4;
+// them.  16 lanes per listed cell (four cells a wave), striding through the
+// cell's particles.
+constexpr int BC_LANES = 16;
+constexpr int BC_CELLS_PER_BLOCK = 256 / BC_LANES;
 // The removed particles are counted for the logs (logCounterParticleEscape
 // / Impact and their kinetic energies, :121-122,153-154): per listed cell,
 // acc[4q..4q+3] += (escapes, sum v^2 of escapes, impacts, sum v^2 of
-// impacts), one wave per cell, summed in a fixed butterfly.
+// impacts), summed over the cell's lanes in a fixed butterfly.
 template <typename Real>
 __global__ __launch_bounds__(256) void k_boundary(Geom g, const int* __restrict__ cells, int ncells,
                                                  const uint32_t* __restrict__ sub, PArrays<Real> A,
                                                  double R2, double* __restrict__ acc) {
-    const int q = blockIdx.x * BC_CELLS_PER_BLOCK + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (q >= ncells) return;
-    const int n = cells[q];
-    int i, j, k;
-    cell_ijk(g, n, i, j, k);
-    const uint32_t kc = cell_key(g, i, j, k);
-    const uint32_t p0 = sub[(size_t)kc * 8], p1 = sub[(size_t)kc * 8 + 8];
-    const bool exterior = !all_nbrs(g, i, j, k);
-    const double xb = g.x0 + i * g.dx, yb = g.y0 + j * g.dx, zb = g.z0 + k * g.dx;
+    const int q = blockIdx.x * BC_CELLS_PER_BLOCK + (int)(threadIdx.x / BC_LANES);
+    const int lane = threadIdx.x & (BC_LANES - 1);
+    const bool valid = q < ncells;
     double c[4] = {0, 0, 0, 0};
-    for (uint32_t p = p0 + lane; p < p1; p += 64) {
-        if (A.x[p] < Real(0)) continue;
-        const double vx = A.vx[p], vy = A.vy[p], vz = A.vz[p];
-        const double v2 = vx * vx + vy * vy + vz * vz;
-        if (exterior) {
-            A.x[p] = Real(-1);
-            c[0] += 1.0; c[1] += v2;
-            continue;
-        }
-        const double x = xb + (double)A.x[p] * g.dx, y = yb + (double)A.y[p] * g.dx, z = zb + (double)A.z[p] * g.dx;
-        if (x * x + y * y + z * z < R2) {
-            A.x[p] = Real(-1);
-            c[2] += 1.0; c[3] += v2;
+    if (valid) {
+        const int n = cells[q];
+        int i, j, k;
+        cell_ijk(g, n, i, j, k);
+        const uint32_t kc = cell_key(g, i, j, k);
+        const uint32_t p0 = sub[(size_t)kc * 8], p1 = sub[(size_t)kc * 8 + 8];
+        const bool exterior = !all_nbrs(g, i, j, k);
+        const double xb = g.x0 + i * g.dx, yb = g.y0 + j * g.dx, zb = g.z0 + k * g.dx;
+        for (uint32_t p = p0 + lane; p < p1; p += BC_LANES) {
+            if (A.x[p] < Real(0)) continue;
+            const double vx = A.vx[p], vy = A.vy[p], vz = A.vz[p];
+            const double v2 = vx * vx + vy * vy + vz * vz;
+            if (exterior) {
+                A.x[p] = Real(-1);
+                c[0] += 1.0; c[1] += v2;
+                continue;
+            }
+            const double x = xb + (double)A.x[p] * g.dx, y = yb + (double)A.y[p] * g.dx, z = zb + (double)A.z[p] * g.dx;
+            if (x * x + y * y + z * z < R2) {
+                A.x[p] = Real(-1);
+                c[2] += 1.0; c[3] += v2;
+            }
         }
     }
     // most listed cells remove nothing in a step: then the wave is done
     if (!__ballot(c[0] != 0.0 || c[2] != 0.0)) return;
     for (int t = 0; t < 4; ++t)
-        for (int off = 32; off > 0; off >>= 1) c[t] += __shfl_xor(c[t], off, 64);
-    if (lane == 0)
+        for (int off = BC_LANES / 2; off > 0; off >>= 1) c[t] += __shfl_xor(c[t], off, 64);
+    if (valid && lane == 0)
         for (int t = 0; t < 4; ++t) acc[(size_t)q * 4 + t] += c[t];
 }
 
