@@ -210,6 +210,14 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
         if (c1 > c0) { a0 = sub[g.nkeys] + app.off[c0]; na = app.off[c1] - app.off[c0]; }
     }
     const uint32_t n0 = p1 - p0, ntot = n0 + na;
+    if (ntot == 0) {
+        // an empty tile (common for a species confined to part of the grid):
+        // nothing to push or move; pass 0 leaves zero slot counts
+        if (pass == 0)
+            for (int ec = tid; ec < g.next; ec += PM_THREADS)
+                *reinterpret_cast<uint4*>(cnt_tile + ((size_t)T * g.next + ec) * 8) = make_uint4(0u, 0u, 0u, 0u);
+        return;
+    }
     const uint32_t deadkey = (uint32_t)MAX_EXTK;
     // prefetch of the first chunk; an injected particle beyond the store's
     // capacity was never written (the injection counted the overflow)
@@ -774,14 +782,22 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
     Real* S = reinterpret_cast<Real*>(smem_raw);
     Real* R = S + 16 * QS;  // first-half results, next x 2
     const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
-    if (sub[(size_t)(T + 1) * NS] - sub[(size_t)T * NS] >= dense_min) {
+    const uint32_t t0 = sub[(size_t)T * NS], t1 = sub[(size_t)(T + 1) * NS];
+    if (t1 == t0) {
+        // an empty tile: zero ghost partials (the fold reads every tile's)
+        Real4<Real>* out = reinterpret_cast<Real4<Real>*>(partial) + (size_t)T * g.next;
+        Real4<Real> z;
+        z.v[0] = z.v[1] = z.v[2] = z.v[3] = Real(0);
+        for (int e = threadIdx.x; e < g.next; e += blockDim.x) out[e] = z;
+        return;
+    }
+    if (t1 - t0 >= dense_min) {
         deposit_dense_tile(g, A, sub, glist, gcount, partial, T, S, R);
         return;
     }
     const int ls = threadIdx.x;
     const uint32_t key = (uint32_t)(T * NS + ls);
     const uint32_t p0 = sub[key], p1 = sub[key + 1];
-    const uint32_t t0 = sub[(size_t)T * NS], t1 = sub[(size_t)(T + 1) * NS];
     const int o = ls & 7, cell = ls >> 3;
     const bool hx = o & 1, hy = (o >> 1) & 1, hz = (o >> 2) & 1;
     Real acc[4][8];
