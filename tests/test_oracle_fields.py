@@ -511,3 +511,121 @@ def test_propagate_B_keeps_div_B(oracle_mod):
     F = o.field("faceB").copy()
     assert np.abs(F - pr.fields["faceB"]).max() > 1e-12
     np.testing.assert_allclose(div_faces(g, F), d0, atol=1e-12 * np.abs(F).max())
+
+
+# ---------------------------------------------------------------- pins of paths the tests above leave open
+
+def test_upwind_node_B_diagonal_flow(oracle_mod):
+    """Ue along the body diagonal: the upwind point is d = -(1,1,1)/(2 sqrt 3)
+    and the cell centre s/2 (s in {-1,+1}^3) lies at distance
+    |s/2 - d| = sqrt(4 + 2 (s.1)/sqrt 3) / 2, so the four distance classes
+    are (sqrt 3 - 1)/2 (the cell straight upwind), sqrt(4 - 2/sqrt 3)/2,
+    sqrt(4 + 2/sqrt 3)/2 and (sqrt 3 + 1)/2.  One cell at a time carries B,
+    so each node's result is that cell's normalised weight; reversing Ue
+    swaps the classes.  Normalising Ue by |Ux|+|Uy|+|Uz| instead of its
+    length, or taking the point downwind, fails this."""
+    pr = make(4, 4, 4, periodic=(1, 1, 1))
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    r3 = np.sqrt(3.0)
+    dist = {-3: (r3 - 1) / 2, -1: np.sqrt(4 - 2 / r3) / 2, 1: np.sqrt(4 + 2 / r3) / 2, 3: (r3 + 1) / 2}
+    wsum = sum(1 / dist[s] * c for s, c in ((-3, 1), (-1, 3), (1, 3), (3, 1)))
+    node = 21                                  # node at the upper corner of cell (1,1,1)
+    i0, j0, k0 = 1, 1, 1
+    for sign in (+1, -1):
+        o.field("nodeUe")[:] = np.tile(sign * 3e5 * np.ones(3) / r3, g.nc)
+        for sx in (-1, 1):
+            for sy in (-1, 1):
+                for sz in (-1, 1):
+                    cb = np.zeros((g.nc, 3))
+                    c = (i0 + (sx > 0)) + 4 * ((j0 + (sy > 0)) + 4 * (k0 + (sz > 0)))
+                    cb[c] = (1.0, 2.0, -1.0)
+                    o.field("cellB")[:] = cb.reshape(-1)
+                    oracle_mod.lib().orc_upwind_node_B(o.raw())
+                    s = sign * (sx + sy + sz)
+                    want = (1 / dist[s]) / wsum * np.array([1.0, 2.0, -1.0])
+                    np.testing.assert_allclose(o.field("nodeB").reshape(-1, 3)[node], want, rtol=1e-13)
+
+
+def test_upwind_node_B_mirror_symmetry(oracle_mod):
+    """Swapping x and y (cells, B components and the flow in the x-y plane)
+    swaps the result: nodeB'(j,i,k) = P nodeB(i,j,k)."""
+    pr = make(5, 5, 4, periodic=(1, 1, 1))
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    rng = np.random.default_rng(41)
+    cb = rng.standard_normal((4, 5, 5, 3))            # [k][j][i][l]
+    a, b = 2.3e5, -0.7e5
+    o.field("cellB")[:] = cb.reshape(-1)
+    o.field("nodeUe")[:] = np.tile([a, b, 0.0], g.nc)
+    oracle_mod.lib().orc_upwind_node_B(o.raw())
+    nb = o.field("nodeB").reshape(4, 5, 5, 3).copy()
+    cbm = cb.transpose(0, 2, 1, 3)[..., [1, 0, 2]]     # cell (i,j,k) <- cell (j,i,k), Bx <-> By
+    o.field("cellB")[:] = np.ascontiguousarray(cbm).reshape(-1)
+    o.field("nodeUe")[:] = np.tile([b, a, 0.0], g.nc)
+    oracle_mod.lib().orc_upwind_node_B(o.raw())
+    nbm = o.field("nodeB").reshape(4, 5, 5, 3)
+    np.testing.assert_allclose(nbm, nb.transpose(0, 2, 1, 3)[..., [1, 0, 2]], rtol=1e-13, atol=1e-15)
+    # a flow with no y part weighs cells at +-y alike
+    o.field("cellB")[:] = cb.reshape(-1)
+    o.field("nodeUe")[:] = np.tile([a, 0.0, 0.3e5], g.nc)
+    oracle_mod.lib().orc_upwind_node_B(o.raw())
+    n1 = o.field("nodeB").reshape(4, 5, 5, 3).copy()
+    o.field("cellB")[:] = np.ascontiguousarray(cb[:, ::-1]).reshape(-1)   # j -> 4-j
+    oracle_mod.lib().orc_upwind_node_B(o.raw())
+    n2 = o.field("nodeB").reshape(4, 5, 5, 3)
+    # node (i,j,k) sits between cells j and j+1; mirrored it sits between 3-j and 4-j: node 3-j
+    np.testing.assert_allclose(n2[:, 3::-1], n1[:, 0:4], rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("where", ["wall", "edge", "corner"])
+def test_node_average_at_wall_nodes(oracle_mod, where):
+    """A uniform E at nodes of cells on the lower walls of an open box.  The
+    missing neighbours' entries count as zeros with their full weights, and
+    only the x component of a missing (-1,-1,-1) neighbour is recognised as
+    missing (the reference initialises its gather buffer with the sentinel in
+    element 0 only, src/hybrid_propagator.cpp:1130), which renormalises the x
+    component by 1 / (1 - C4).  Existing neighbours by distance (0, 1, sqrt 2,
+    sqrt 3): wall (i = 0) 1, 5, 8, 4; edge (i = j = 0) 1, 4, 5, 2; corner
+    (i = j = k = 0) 1, 3, 3, 1."""
+    C = P.gaussian_coeffs(0.9)
+    pr = make(5, 6, 7, use_gaussian=1, gauss_coeffs=C)
+    g = pr.grid
+    o = oracle_mod.Oracle(pr)
+    E0 = np.array([1.5, -2.0, 0.75])
+    E = np.tile(E0, g.nc)
+    out = np.zeros_like(E)
+    oracle_mod.lib().orc_node_avg(o.raw(), ptr(E), ptr(out))
+    cells = {"wall": (0, 2, 3), "edge": (0, 0, 3), "corner": (0, 0, 0)}
+    counts = {"wall": (1, 5, 8, 4), "edge": (1, 4, 5, 2), "corner": (1, 3, 3, 1)}
+    i, j, k = cells[where]
+    n = i + g.nx * (j + g.ny * k)
+    S = sum(c * w for c, w in zip(counts[where], C))
+    want = E0 * S
+    want[0] /= 1.0 - C[3]
+    np.testing.assert_allclose(out.reshape(-1, 3)[n], want, rtol=1e-13)
+    # an interior node of the same box keeps the uniform value
+    m = 2 + g.nx * (2 + g.ny * 2)
+    np.testing.assert_allclose(out.reshape(-1, 3)[m], E0, rtol=1e-13)
+
+
+@pytest.mark.parametrize("R_cells,field_cells,node_count,particle_cells", [(1.0, 8, 1, 32), (2.0, 32, 27, 88)])
+def test_inner_flags_count(R_cells, field_cells, node_count, particle_cells):
+    """Inner-boundary flags of a sphere at the centre of an even box
+    (src/user.cpp:1893-1912): cell centres sit on the half-integer lattice
+    and nodes on the integer lattice (in cells).  innerFlagField: centres with
+    r < R -- 8 for R = 1 (r^2 = 3/4), 32 for R = 2 (r^2 = 3/4, 11/4);
+    innerFlagNode: nodes with r < R -- the origin for R = 1, the 3^3 nodes
+    around it for R = 2; innerFlagParticle: (r - sqrt(3)/2)^2 < R^2, i.e.
+    r < R + sqrt(3)/2 -- r^2 in {3/4, 11/4} for R = 1 (32 cells) and
+    {3/4, 11/4, 19/4, 27/4} for R = 2 (8 + 24 + 24 + 32 = 88 cells)."""
+    n, dx = 10, 2e5
+    g = P.Grid(n, n, n, (0, 0, 0), dx, -0.5 * n * dx, -0.5 * n * dx, -0.5 * n * dx)
+    f = P.make_flags(g, R_field=R_cells * dx, R_particle=R_cells * dx, R_zero_pe=R_cells * dx)
+    assert f["innerFlagField"].sum() == field_cells
+    assert f["innerFlagNode"].sum() == node_count
+    assert f["innerFlagParticle"].sum() == particle_cells
+    assert f["innerFlagCellEp"].sum() == particle_cells
+    # symmetric under the box's mirror x -> -x (cell i -> n-1-i)
+    a = f["innerFlagParticle"].reshape(n, n, n)
+    assert np.array_equal(a, a[:, :, ::-1])

@@ -153,3 +153,40 @@ def test_migration_preserves_global_position(oracle_mod):
     d = np.abs(got - exp[o0])
     d = np.minimum(d, L - d)
     assert d.max() < 1e-6
+
+
+def test_gravity_kick(oracle_mod):
+    """With no fields the Boris rotation leaves v unchanged and the
+    gravitational kick is exactly -GMdt r / |r|^3 at the particle's global
+    position before the move (src/particle_propagator_boris_buneman.h:
+    137-145); the box origin is offset so that the global and cell-local
+    positions differ."""
+    g = P.Grid(4, 4, 4, (1, 1, 1), 1e5, -1.7e5, 0.6e5, -2.2e5)
+    GMdt = 3e13
+    p = P.Params(dt=0.01, use_gravity=1, GMdt=GMdt)
+    sp = P.Species("H+", P.QE, P.MP, 1.0)
+    v0 = np.array([2e4, -1e4, 5e3])
+    cell, r = 37, np.array([2.5e4, 7.5e4, 4.0e4])
+    pr = one_particle_problem(g, p, sp, cell, r, v0)
+    st = pushed(oracle_mod, pr)
+    i, j, k = cell % 4, (cell // 4) % 4, cell // 16
+    rg = np.array([g.x0 + i * g.dx, g.y0 + j * g.dx, g.z0 + k * g.dx]) + r
+    want = v0 - GMdt * rg / np.linalg.norm(rg) ** 3
+    assert np.allclose(st[3:6], want, rtol=1e-13, atol=0)
+    assert np.linalg.norm(want - v0) > 100.0  # the kick is there
+    # an exterior cell (not all neighbours) neither accelerates nor feels gravity
+    g2 = P.Grid(4, 4, 4, (0, 1, 1), 1e5, -1.7e5, 0.6e5, -2.2e5)
+    pr2 = one_particle_problem(g2, p, sp, 36, r, v0)   # i = 0: -x wall
+    assert np.array_equal(pushed(oracle_mod, pr2)[3:6], v0)
+
+
+def test_initial_flow_through_switches_acceleration_off(oracle_mod):
+    """Hybrid::useInitialFlowThrough: no particle is accelerated, whatever
+    the fields (src/particle_propagator_boris_buneman.h:66-68)."""
+    g, p, sp = box(use_e_pressure=1, initial_flow_through=1)
+    v = np.array([1e5, 2e4, -3e4])
+    r = np.array([2e4, 7e4, 1e4])
+    pr = one_particle_problem(g, p, sp, 30, r, v, B=(0, 0, 1e-6), Ue=(1e5, 0, 0), Ep=(1e-3, 0, 0))
+    st = pushed(oracle_mod, pr)
+    assert np.array_equal(st[3:6], v)
+    assert np.allclose(st[0:3], r + p.dt * v, rtol=0, atol=1e-9)
