@@ -28,6 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0      # MI355X published HBM3E bandwidth
+BENCH_TILE = (4, 4, 4)     # the tile shape the context picks for the headline box
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
 
 
@@ -51,14 +52,15 @@ def kernel_bytes(name: str, ctx, np_total: float, nspecies: int) -> float:
     fblock = 4.0 * (3 * nh + 7 * tc) * ntiles   # per-tile field blocks (FP32 + neighbourhood masks)
     if name == "push_fields":  # read 9 f32 fields per cell, write the per-tile blocks
         return 36.0 * nc + fblock
-    if name == "push":       # pass 0: read 6 f32 + local cell byte and the field blocks; write slot counts
-        return 25.0 * npp + fblock + 4.0 * extk * ntiles + 4.0 * ntiles
-    if name == "move":       # pass 1: read and write 6 f32 + local cell byte; field blocks; slot bases
-        return 50.0 * npp + fblock + 4.0 * extk * ntiles + 4.0 * ntiles
-    if name == "deposit":    # read 6 f32 per particle, subcell starts; read+write FP64 tile ghosts
-        return 24.0 * npp + 4.0 * nkeys + 64.0 * next_ * ntiles
-    if name == "fold":       # read FP64 tile ghosts once, write rho + J (f32)
-        return 32.0 * next_ * ntiles + 16.0 * nc
+    if name == "push":       # pass 0: read 6 f32 + local cell byte and the field blocks; write 16-bit slot counts
+        return 25.0 * npp + fblock + 2.0 * extk * ntiles + 8.0 * ntiles
+    if name == "move":       # pass 1: read and write 6 f32 + local cell byte; field blocks; 16-bit slot
+        # offsets and the 32-bit starts of the extended region's subcells
+        return 50.0 * npp + fblock + 6.0 * extk * ntiles + 8.0 * ntiles
+    if name == "deposit":    # read 6 f32 per particle and the subcell starts; write the f32 tile ghosts
+        return 24.0 * npp + 4.0 * nkeys + 16.0 * next_ * ntiles
+    if name == "fold":       # read the f32 tile ghosts once, write rho + J (f32)
+        return 16.0 * next_ * ntiles + 16.0 * nc
     if name == "sort_slots":  # read and write the 16-bit slot table, write the totals
         return 4.0 * extk * ntiles + 4.0 * nkeys
     return 0.0
@@ -66,7 +68,7 @@ def kernel_bytes(name: str, ctx, np_total: float, nspecies: int) -> float:
 
 PROFILE_KERNELS = {"push": "rh::k_push_move<float>_[pass_0:_push]", "move": "rh::k_push_move<float>_[pass_1:_move]",
                    "deposit": "rh::k_deposit<float>", "fold": "rh::k_fold<float>",
-                   "sort_slots": "rh::k_slot_offsets<true>"}
+                   "sort_slots": "rh::k_slot_offsets"}
 
 
 def measured_traffic(kernel: str):
@@ -86,18 +88,14 @@ def measured_traffic(kernel: str):
     return None
 
 
-def gpu_bench(args):
+def gpu_bench(args, pr):
     from rhybrid_amd.gpu import RH_FP32, Context
-    t0 = time.time()
-    pr = workload()
-    t_gen = time.time() - t0
     n0 = pr.n_particles
     t0 = time.time()
     ctx = Context(pr, precision=RH_FP32)
     t_up = time.time() - t0
-    print("# workload %s: %d cells, %d particles (gen %.1fs, upload %.1fs)" %
-          (pr.name, pr.grid.nc, n0, t_gen, t_up), file=sys.stderr, flush=True)
-    del pr
+    print("# workload %s: %d cells, %d particles (upload %.1fs)" %
+          (pr.name, pr.grid.nc, n0, t_up), file=sys.stderr, flush=True)
     ctx.step(args.warmup)
     ctx.synchronize()
     nspec = len(ctx.species)
@@ -244,59 +242,112 @@ def _rel_errors(x_gpu: np.ndarray, x_orc: np.ndarray, dim: int) -> dict:
             "max_rel": float((d[big] / nb[big]).max())}
 
 
-def cpu_baseline(threads: int):
+def host_cpu() -> dict:
+    """The host's CPU as the process sees it (nproc, the CPUs it may run
+    on, the model and socket count from /proc/cpuinfo)."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    try:
+        models, phys = set(), set()
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                models.add(line.split(":", 1)[1].strip())
+            elif line.startswith("physical id"):
+                phys.add(line.split(":", 1)[1].strip())
+        info["model"] = "; ".join(sorted(models))
+        info["sockets"] = len(phys) or None
+    except OSError:
+        pass
+    return info
+
+
+def cpu_baseline(threads: int, pr, steps: int = 4):
     """The FP64 oracle (the reference algorithm in the reference's data
-    layout): timed on a 64x48x48-cell box of the same workload (1/8 of the
-    particles) over 2 steps after 1 warm-up step, on `threads` cores and on
-    one core over 1 step.  As the checker, it also gives the matched-error
-    tables of the FP32 GPU path: on that obstacle box after its 3 steps, and
-    on the 32^3 periodic box (64 ppc, 2.1 M particles) after 100 steps, both
-    from the same upload."""
+    layout, one slab of cells per thread) on the headline workload itself:
+    `steps` full steps after one warm-up step on `threads` cores, and one
+    step of the 64x48x48 box of the same workload on one core.  As the
+    checker it also gives the matched-error tables of the FP32 GPU path from
+    the same upload: the headline box after the oracle's 1 + `steps` steps,
+    and the 32^3 periodic box (64 ppc, 2.1 M particles) after 100 steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from rhybrid_amd.gpu import RH_FP32, Context
     O.build()
-    pr = workload(scale=2)
     o = O.Oracle(pr, threads=threads)
     o.step(1)
     n = sum(o.num_particles(s) for s in range(o.nspecies))
+    print("# cpu baseline: %d particles, %d threads" % (n, threads), file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    steps = 2
     o.step(steps)
     dt = time.perf_counter() - t0
-    ctx = Context(pr, precision=RH_FP32)
+    ctx = Context(pr, precision=RH_FP32, tile=BENCH_TILE)
     ctx.step(1 + steps)
-    err_obstacle = {"workload": "64x48x48 obstacle box of the same workload, 3 steps"}
+    err_full = {"workload": "the headline box (128x96x96, ~71 M particles), %d full steps from the same upload, "
+                            "FP32 GPU vs FP64 oracle" % (1 + steps), "tile": list(BENCH_TILE)}
     for key, nm, dim in (("n", "cellRhoQi", 1), ("J", "cellJi", 3), ("E", "nodeE", 3), ("B", "faceB", 3)):
-        err_obstacle[key] = _rel_errors(ctx.field(nm), o.field(nm).copy(), dim)
+        err_full[key] = _rel_errors(ctx.field(nm), o.field(nm).copy(), dim)
+    err_full["particles_gpu"] = [ctx.num_particles(s) for s in range(len(pr.species))]
+    err_full["particles_oracle"] = [o.num_particles(s) for s in range(o.nspecies)]
     ctx.close()
     del o
-    o1 = O.Oracle(pr, threads=1)
+    small = workload(scale=2)
+    o1 = O.Oracle(small, threads=1)
     n1 = sum(o1.num_particles(s) for s in range(o1.nspecies))
     t0 = time.perf_counter()
     o1.step(1)
     dt1 = time.perf_counter() - t0
-    del o1, pr
+    del o1, small
     cpu = {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-           "sample": "FP64 oracle, 64x48x48-cell box of the same workload (%d particles), %d steps, "
-                     "full step incl. field update" % (n, steps),
-           "value_1core": n1 / dt1}
+           "sample": "FP64 oracle on the headline workload itself (%d particles), %d full steps incl. the field "
+                     "update after one warm-up step, %d OpenMP threads (one slab of cells each); value_1core: one "
+                     "step of the 64x48x48 box of the same workload on one core" % (n, steps, threads),
+           "value_1core": n1 / dt1, "host": host_cpu()}
     # matched error of the FP32 GPU path on the 32^3 periodic box
     from rhybrid_amd import problem as P
     err_steps = 100
     pr = P.config_periodic(n=32, ppc=64)
-    ctx = Context(pr, precision=RH_FP32)
+    ctx = Context(pr, precision=RH_FP32, tile=BENCH_TILE)
     ctx.step(err_steps)
     g = {nm: ctx.field(nm) for nm in ("cellRhoQi", "cellJi", "nodeE", "faceB")}
     ctx.close()
     o = O.Oracle(pr, threads=threads)
     o.step(err_steps)
     err = {"workload": "32^3 periodic box, 64 ppc H+, %d full steps, FP32 GPU vs FP64 oracle; rel_l2 = "
-                       "|X_gpu - X_oracle|_2 / |X_oracle|_2, max_rel over cells with |X| > 1e-3 max|X|" % err_steps}
+                       "|X_gpu - X_oracle|_2 / |X_oracle|_2, max_rel over cells with |X| > 1e-3 max|X|" % err_steps,
+           "tile": list(BENCH_TILE)}
     for key, nm, dim in (("n", "cellRhoQi", 1), ("J", "cellJi", 3), ("E", "nodeE", 3), ("B", "faceB", 3)):
         err[key] = _rel_errors(g[nm], o.field(nm).copy(), dim)
-    err["obstacle_box"] = err_obstacle
+    err["headline_box"] = err_full
     return cpu, err
+
+
+def error_curve(pr, steps: int = 500, every: int = 50):
+    """FP32 against FP64 on the headline box over `steps` full steps, both
+    on the GPU from the same upload (the FP64 instantiation runs the same
+    kernels in double and matches the FP64 oracle to 1e-9 in the tests): the
+    relative errors of n, J, E and B every `every` steps."""
+    from rhybrid_amd.gpu import RH_FP32, RH_FP64, Context
+    c32 = Context(pr, precision=RH_FP32, tile=BENCH_TILE)
+    c64 = Context(pr, precision=RH_FP64, tile=BENCH_TILE)
+    rows = []
+    for done in range(every, steps + 1, every):
+        c32.step(every)
+        c64.step(every)
+        row = {"step": done}
+        for key, nm, dim in (("n", "cellRhoQi", 1), ("J", "cellJi", 3), ("E", "nodeE", 3), ("B", "faceB", 3)):
+            row[key] = _rel_errors(c32.field(nm), c64.field(nm), dim)
+        row["particles_fp32"] = [c32.num_particles(s) for s in range(len(pr.species))]
+        row["particles_fp64"] = [c64.num_particles(s) for s in range(len(pr.species))]
+        rows.append(row)
+        print("# error curve step %d: n %.2e J %.2e E %.2e B %.2e" % (done, row["n"]["rel_l2"], row["J"]["rel_l2"],
+              row["E"]["rel_l2"], row["B"]["rel_l2"]), file=sys.stderr, flush=True)
+    c32.close()
+    c64.close()
+    return {"workload": "the headline box, FP32 vs FP64 GPU paths from the same upload, %d full steps" % steps,
+            "tile": list(BENCH_TILE), "rows": rows}
 
 
 def exosphere_bench(steps: int = 10, warmup: int = 2, prefill: int = 600, tune_shapes: int = 4):
@@ -382,6 +433,7 @@ def main():
     ap.add_argument("--no-in-loop", action="store_true", help="skip the denoiser's in-loop comparison runs")
     ap.add_argument("--only-denoiser", action="store_true", help="the denoiser section alone (its kernels' profile)")
     ap.add_argument("--no-exosphere", action="store_true", help="skip the uneven-load exosphere box")
+    ap.add_argument("--no-error-curve", action="store_true", help="skip the 500-step FP32-vs-FP64 error curve")
     ap.add_argument("--calibration-copy-gib", type=float, default=0.0,
                     help="first copy this many GiB device to device (a traffic counter calibration dispatch)")
     args = ap.parse_args()
@@ -394,7 +446,10 @@ def main():
         return
     if args.calibration_copy_gib > 0:
         calibration_copy(args.calibration_copy_gib)
-    res = gpu_bench(args)
+    t0 = time.time()
+    pr = workload()
+    print("# workload generated in %.1fs" % (time.time() - t0), file=sys.stderr, flush=True)
+    res = gpu_bench(args, pr)
     dn = None if args.no_denoiser else denoiser_bench()
     if dn is not None and not args.no_in_loop:
         from rhybrid_amd import denoise_loop
@@ -407,8 +462,11 @@ def main():
             exo = {"error": repr(e)}
     cpu, err = None, None
     if not args.no_cpu_baseline:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-        cpu, err = cpu_baseline(threads)
+        # the process's CPU share (OMP_NUM_THREADS where the host sets it)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        cpu, err = cpu_baseline(threads, pr)
+    curve = None if args.no_error_curve else error_curve(pr)
+    del pr
     line = {
         "metric": "particle-steps/s (push+deposit) & step ms at matched n/J error vs FP64 oracle",
         "value": res["value"],
@@ -431,6 +489,7 @@ def main():
         "roofline": res["roofline"],
         "cpu_baseline": cpu,
         "error_vs_oracle": err,
+        "error_curve_fp32_vs_fp64": curve,
         "denoiser": dn,
         "exosphere_box": exo,
         "push_deposit": res["push_deposit"],

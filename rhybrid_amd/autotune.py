@@ -2,8 +2,9 @@
 
 The tile shape is fixed when a context is created (rh_grid_desc.tile).  The
 autotuner creates one context per candidate shape -- power-of-two sides that
-divide the grid, at most 64 cells per tile for FP32 (32 for FP64, the
-deposit's LDS), enough tiles to fill the chip -- runs a few full steps of the
+divide the grid, at most 64 cells per tile (the deposit runs one thread per
+subcell; the context also rejects shapes whose LDS exceeds the device's),
+enough tiles to fill the chip -- runs a few full steps of the
 caller's problem on each and keeps the fastest.  Timing is the wall clock of
 the library's own steps after one warm-up step, HIP-synchronised.
 """
@@ -23,7 +24,7 @@ def candidates(grid: P.Grid, precision: int = RH_FP32, min_tiles: int = 2048,
                max_cells: Optional[int] = None) -> List[Tile]:
     """Tile shapes worth timing on `grid`."""
     if max_cells is None:
-        max_cells = 64 if precision == RH_FP32 else 32
+        max_cells = 64
     sides = (1, 2, 4, 8, 16)
     out = []
     for tx, ty, tz in itertools.product(sides, sides, sides):

@@ -270,7 +270,18 @@ public:
         g.nc = g.nx * g.ny * g.nz;
         for (int d = 0; d < 3; ++d) g.per[d] = gd->periodic[d] ? 1 : 0;
         g.dx = gd->dx; g.x0 = gd->x0; g.y0 = gd->y0; g.z0 = gd->z0;
+        {
+            int dev = 0, lds = 0;
+            RH_HIP(hipGetDevice(&dev));
+            RH_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+            lds_limit = (size_t)lds;
+        }
         choose_tiles(gd->tile);
+        // dynamic LDS above the default 64 KiB (FP64 tiles of 64 cells)
+        RH_HIP(hipFuncSetAttribute((const void*)k_deposit<Real>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)deposit_lds_bytes<Real>(g)));
+        RH_HIP(hipFuncSetAttribute((const void*)k_push_move<Real>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)push_move_lds_bytes<Real>(g)));
         RH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         for (int f = 0; f < RH_N_FIELDS; ++f) {
             fld[f].alloc((size_t)g.nc * FIELD_DIM[f]);
@@ -331,17 +342,32 @@ public:
         if (stream) (void)hipStreamDestroy(stream);
     }
 
+    size_t lds_limit = 65536;  // the device's LDS per workgroup (queried at creation)
+
+    // LDS of the largest per-tile kernels for a tile shape
+    size_t tile_lds_bytes(const int* t) const {
+        Geom h = g;
+        h.tx = t[0]; h.ty = t[1]; h.tz = t[2];
+        h.tc = t[0] * t[1] * t[2];
+        h.ex = t[0] + 2; h.ey = t[1] + 2; h.ez = t[2] + 2;
+        h.next = h.ex * h.ey * h.ez;
+        h.extk = h.next * 8;
+        return std::max(deposit_lds_bytes<Real>(h), push_move_lds_bytes<Real>(h));
+    }
+
     void choose_tiles(const int32_t* req) {
         int t[3] = {req ? req[0] : 0, req ? req[1] : 0, req ? req[2] : 0};
         const int dims[3] = {g.nx, g.ny, g.nz};
-        const int max_cells = sizeof(Real) == 4 ? 64 : 32;  // deposit LDS <= 64 KiB
+        // the deposit runs one thread per subcell (at most 512) and both
+        // per-tile kernels keep their tile in LDS
+        const int max_cells = 64;
         if (t[0] <= 0 || t[1] <= 0 || t[2] <= 0) {
             for (int d = 0; d < 3; ++d) {
                 t[d] = 4;
                 while (t[d] > 1 && dims[d] % t[d]) t[d] /= 2;
             }
             auto ntiles = [&]() { return (dims[0] / t[0]) * (dims[1] / t[1]) * (dims[2] / t[2]); };
-            while (t[0] * t[1] * t[2] > max_cells) {
+            while (t[0] * t[1] * t[2] > max_cells || (t[0] * t[1] * t[2] > 1 && tile_lds_bytes(t) > lds_limit)) {
                 int d = 0;
                 for (int q = 1; q < 3; ++q) if (t[q] > t[d]) d = q;
                 t[d] /= 2;
@@ -358,7 +384,10 @@ public:
         for (int d = 0; d < 3; ++d)
             if (t[d] < 1 || (t[d] & (t[d] - 1)) || dims[d] % t[d])
                 throw ArgError("tile sides must be powers of two that divide the grid");
-        if (t[0] * t[1] * t[2] > max_cells) throw ArgError("tile too large for this precision");
+        if (t[0] * t[1] * t[2] > max_cells) throw ArgError("tile has more than 64 cells");
+        if (tile_lds_bytes(t) > lds_limit)
+            throw ArgError("tile needs " + std::to_string(tile_lds_bytes(t)) + " bytes of LDS, the device has " +
+                           std::to_string(lds_limit));
         g.tx = t[0]; g.ty = t[1]; g.tz = t[2];
         g.ltx = __builtin_ctz(g.tx); g.lty = __builtin_ctz(g.ty); g.ltz = __builtin_ctz(g.tz);
         g.tc = g.tx * g.ty * g.tz;
@@ -882,19 +911,14 @@ public:
         check_overflow();
     }
 
-    // The sort and the injection count overflows of a particle store in
-    // slot 0 of their counter; read it so that the steps that lost particles
-    // report it.
+    // The sort and the injection count overflows of a particle store into
+    // the overflow counter; its copies in every counter slot are summed, so
+    // that the steps that lost particles report it whichever slot a kernel
+    // used.
     void check_overflow() {
-        unsigned long long h = 0;
-        for (size_t b = 0; b <= sp.size(); ++b) {
-            unsigned long long x = 0;
-            const unsigned long long* src = b == 0 ? dcount.p : sp[b - 1]->pcount.p;
-            RH_HIP(hipMemcpyAsync(&x, src + C_RANK_OVERFLOW, sizeof(x), hipMemcpyDeviceToHost, stream));
-            RH_HIP(hipStreamSynchronize(stream));
-            h += x;
-        }
-        if (h)
+        unsigned long long h[C_N];
+        counter_totals(h);
+        if (h[C_RANK_OVERFLOW])
             throw CapError("particle sort overflow (a subcell holds more than 65535 particles of one species, "
                            "or the particle store is full)");
     }

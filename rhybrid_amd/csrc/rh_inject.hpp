@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void k_inject_slots(Geom g, InjParams ip, int 
     // integer sums: the same total in any order
     const unsigned long long b = __ballot(live);
     if ((threadIdx.x & 63) == 0 && b) atomicAdd(total, (unsigned long long)__popcll(b));
-    count_event(counters, C_RANK_OVERFLOW, overflow);
+    if (overflow) atomicAdd(counters + C_RANK_OVERFLOW, 1ull);  // slot 0, as every overflow site
     __shared__ double wsum[4];
     for (int off = 32; off > 0; off >>= 1) v2 += __shfl_xor(v2, off, 64);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v2;

@@ -128,21 +128,21 @@ RH_HD uint32_t neighbour_mask(const Geom& g, int i, int j, int k) {
 // cells that do not exist are never written and keep the zero of the
 // allocation.
 template <typename Real>
-__device__ __forceinline__ void push_fields_at(const Geom& g, int i, int j, int k, int n, const PushFields<Real>& f,
-                                               int use_e_pressure, uint32_t* __restrict__ out) {
+__device__ __forceinline__ void push_fields_store(const Geom& g, int i, int j, int k, Real bx, Real by, Real bz,
+                                                  Real uex, Real uey, Real uez, Real epx, Real epy, Real epz,
+                                                  int use_e_pressure, uint32_t* __restrict__ out) {
     const int hx = g.tx + 1, hy = g.ty + 1, nh = (g.tx + 1) * (g.ty + 1) * (g.tz + 1);
     const size_t nw = push_field_words<Real>(g);
     const int li = i & (g.tx - 1), lj = j & (g.ty - 1), lk = k & (g.tz - 1);
     const int lc = li + (lj << g.ltx) + (lk << (g.ltx + g.lty));
-    const Real bx = f.fbx[n], by = f.fby[n], bz = f.fbz[n];
     {
         Real* blk = reinterpret_cast<Real*>(out + (size_t)tile_of(g, i, j, k) * nw);
         const int h = (li + 1) + hx * ((lj + 1) + hy * (lk + 1));
         blk[h] = bx; blk[nh + h] = by; blk[2 * nh + h] = bz;
         Real* sue = blk + 3 * nh;
         Real* sep = sue + 3 * g.tc;
-        sue[lc] = f.uex[n]; sue[g.tc + lc] = f.uey[n]; sue[2 * g.tc + lc] = f.uez[n];
-        if (use_e_pressure) { sep[lc] = f.epx[n]; sep[g.tc + lc] = f.epy[n]; sep[2 * g.tc + lc] = f.epz[n]; }
+        sue[lc] = uex; sue[g.tc + lc] = uey; sue[2 * g.tc + lc] = uez;
+        if (use_e_pressure) { sep[lc] = epx; sep[g.tc + lc] = epy; sep[2 * g.tc + lc] = epz; }
         else { sep[lc] = sep[g.tc + lc] = sep[2 * g.tc + lc] = Real(0); }
         reinterpret_cast<uint32_t*>(sep + 3 * g.tc)[lc] = neighbour_mask(g, i, j, k);
     }
@@ -160,6 +160,14 @@ __device__ __forceinline__ void push_fields_at(const Geom& g, int i, int j, int 
                 const int h = (a ? 0 : li + 1) + hx * ((b ? 0 : lj + 1) + hy * (c ? 0 : lk + 1));
                 blk[h] = bx; blk[nh + h] = by; blk[2 * nh + h] = bz;
             }
+}
+
+template <typename Real>
+__device__ __forceinline__ void push_fields_at(const Geom& g, int i, int j, int k, int n, const PushFields<Real>& f,
+                                               int use_e_pressure, uint32_t* __restrict__ out) {
+    push_fields_store(g, i, j, k, f.fbx[n], f.fby[n], f.fbz[n], f.uex[n], f.uey[n], f.uez[n],
+                      use_e_pressure ? f.epx[n] : Real(0), use_e_pressure ? f.epy[n] : Real(0),
+                      use_e_pressure ? f.epz[n] : Real(0), use_e_pressure, out);
 }
 
 template <typename Real>

@@ -197,3 +197,15 @@ def test_injector_argument_errors(gpu_mod):
     ctx.inject(0)
     assert ctx.num_particles(0) == 0
     assert math.isfinite(ctx.diagnostics(0)[0])
+
+
+def test_store_overflow_in_a_later_block_is_reported(gpu_mod):
+    """The first place past the store's capacity lies in the second block of
+    the injection kernel (22 x 22 wall cells x 8 places, capacity 300): the
+    step that loses particles returns RH_ERR_CAPACITY."""
+    pr = wall_box((-3e5, 0, 0), ny=24, nz=24, capacity=300)
+    print("case overflow in block 1", flush=True)
+    ctx = gpu_mod.Context(pr, precision=gpu_mod.RH_FP32)
+    with pytest.raises(gpu_mod.RhError) as e:
+        ctx.step(1)
+    assert e.value.code == gpu_mod.RH_ERR_CAPACITY

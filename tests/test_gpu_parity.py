@@ -209,3 +209,45 @@ def test_error_paths(gpu_mod):
     with pytest.raises(gpu_mod.RhError) as e:
         gpu_mod.Context(pr, precision=gpu_mod.RH_FP32, tile=(3, 4, 4))
     assert e.value.code == gpu_mod.RH_ERR_ARG
+
+
+@pytest.mark.parametrize("case", ["gravity", "flow_through", "emax", "whistler", "vi_clamp"])
+def test_rare_branches_fp64(gpu_mod, oracle_mod, case):
+    """Branches the workloads seldom reach, driven until they fire: the
+    gravitational kick (src/particle_propagator_boris_buneman.h:137-145),
+    useInitialFlowThrough (:66-68), the |E| clamp of calcNodeE
+    (src/hybrid_propagator.cpp:1391-1446), the whistler limiter of calcNodeJ
+    (:1449-1522) and the ion speed clamp (:146-157).  FP64 against the
+    oracle after 6 full steps; the constraint counters agree."""
+    pr = P.config_obstacle(nx=16, ny=12, nz=12, ppc=8, R_cells=3, seed=51, inflow=True)
+    for sp in pr.species:
+        sp.capacity = 3 * pr.particles[0].n
+    p = pr.params
+    counter = None
+    if case == "gravity":
+        p.use_gravity, p.GMdt = 1, 1e16           # |dv| = 1e4 m/s at 1000 km
+    elif case == "flow_through":
+        p.initial_flow_through = 1
+    elif case == "emax":
+        p.max_E2, counter = (0.8e-3) ** 2, "max_E"  # |U x B| = 1.3e-3 V/m upstream
+    elif case == "whistler":
+        p.max_whistler, counter = 0.8e5, "max_vw"   # upstream grid whistler speed ~1e5 m/s
+    elif case == "vi_clamp":
+        p.max_ion_speed2, counter = (4.6e5) ** 2, "max_vi"
+    ctx = make_ctx(gpu_mod, pr, gpu_mod.RH_FP64)
+    ora = oracle_mod.Oracle(pr)
+    for step in range(6):
+        ctx.step(1)
+        ora.step(1)
+        for s in range(2):
+            assert ctx.num_particles(s) == ora.num_particles(s), (step, s)
+    for name in FIELDS:
+        b = ora.field(name)
+        if np.linalg.norm(b) > 0:
+            assert rel(ctx.field(name), b) < 1e-9, (name, rel(ctx.field(name), b))
+    cg, co = ctx.counters(), ora.counters()
+    print("counters gpu %s oracle %s" % (cg, co), flush=True)
+    for k in co:
+        assert cg[k] == co[k], (k, cg[k], co[k])
+    if counter is not None:
+        assert co[counter] > 0

@@ -1,0 +1,147 @@
+"""The production tile shapes against the FP64 oracle, through the C ABI.
+
+The context picks 4x4x4 tiles for large grids, and the autotuner times the
+most compact 64-cell shapes (2x4x8, 2x8x4, 4x2x8) on the uneven-load box;
+small grids fall back to 2x2x2 tiles.  These cases pass the tile shape
+explicitly, so the 512-thread deposit (its one-lane-per-subcell and its
+crowded-tile paths), the 11-bit ranking over 1,728 slots and the LDS sizes of
+64-cell tiles all run against the oracle:
+  * FP32 on an obstacle box with inflow, 20 full steps, at every production
+    shape: n, J and faceB within 1e-5 (global relative L2) of the FP64
+    oracle, particle counts within 2 per species;
+  * FP32 at 96 ppc (every tile takes the crowded-tile deposit path);
+  * FP64 at 4x4x4 (LDS above 64 KiB): the oracle to 1e-9 after 8 steps,
+    identical particle counts;
+  * a reduced uneven-load box (device uniform fill of H+, O+ photoions
+    accumulated near an absorbing sphere, so that many tiles hold no O+ and a
+    few are crowded) stepped against the oracle at every autotune shape."""
+import numpy as np
+import pytest
+
+from rhybrid_amd import problem as P
+
+pytestmark = pytest.mark.gpu
+
+_ORACLE = {}
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def oracle_after(oracle_mod, key, pr, steps, prepare=None):
+    """The oracle's fields and particle counts after `steps` full steps
+    (computed once per key: the oracle is deterministic)."""
+    if key not in _ORACLE:
+        ora = oracle_mod.Oracle(pr)
+        if prepare is not None:
+            prepare(ora)
+        ora.step(steps)
+        _ORACLE[key] = ({n: ora.field(n).copy() for n in ("cellRhoQi", "cellJi", "faceB", "nodeE")},
+                        [ora.num_particles(s) for s in range(ora.nspecies)])
+        del ora
+    return _ORACLE[key]
+
+
+def run_ctx(gpu_mod, pr, prec, tile, steps, prepare=None):
+    print("case %s precision=%s tile=%s np=%d steps=%d" % (pr.name, "fp64" if prec else "fp32", tile,
+                                                         pr.n_particles, steps), flush=True)
+    ctx = gpu_mod.Context(pr, precision=prec, tile=tile)
+    lay = ctx.layout()
+    assert (lay["tx"], lay["ty"], lay["tz"]) == tuple(tile)
+    if prepare is not None:
+        prepare(ctx)
+    ctx.step(steps)
+    out = ({n: ctx.field(n) for n in ("cellRhoQi", "cellJi", "faceB", "nodeE")},
+           [ctx.num_particles(s) for s in range(len(pr.species))])
+    ctx.close()
+    return out
+
+
+def obstacle_inflow(ppc, n=32, seed=41):
+    pr = P.config_obstacle(nx=n, ny=n, nz=n, ppc=ppc, R_cells=n / 8.0, seed=seed, inflow=True)
+    for sp in pr.species:
+        sp.capacity = 2 * pr.particles[0].n
+    return pr
+
+
+@pytest.mark.parametrize("tile", [(4, 4, 4), (2, 4, 8), (2, 8, 4), (4, 2, 8)])
+def test_production_tiles_fp32_vs_oracle(gpu_mod, oracle_mod, tile):
+    pr = obstacle_inflow(ppc=8)
+    steps = 20
+    fo, no = oracle_after(oracle_mod, "obstacle32_ppc8", pr, steps)
+    fg, ng = run_ctx(gpu_mod, pr, gpu_mod.RH_FP32, tile, steps)
+    for name in ("cellRhoQi", "cellJi", "faceB"):
+        r = rel(fg[name], fo[name])
+        print("tile %s: %s rel L2 = %.3e" % (tile, name, r), flush=True)
+        assert r < 1e-5, (name, r)
+    for s in range(2):
+        assert abs(ng[s] - no[s]) <= 2, (s, ng[s], no[s])
+
+
+def test_crowded_tiles_fp32_vs_oracle(gpu_mod, oracle_mod):
+    """96 ppc: 6,144 particles per 4x4x4 tile of a species, above the 5,120
+    at which a tile takes the eight-lanes-per-subcell deposit."""
+    pr = obstacle_inflow(ppc=96, n=16, seed=42)
+    steps = 10
+    fo, no = oracle_after(oracle_mod, "obstacle16_ppc96", pr, steps)
+    fg, ng = run_ctx(gpu_mod, pr, gpu_mod.RH_FP32, (4, 4, 4), steps)
+    for name in ("cellRhoQi", "cellJi", "faceB"):
+        r = rel(fg[name], fo[name])
+        print("crowded: %s rel L2 = %.3e" % (name, r), flush=True)
+        assert r < 1e-5, (name, r)
+    for s in range(2):
+        assert abs(ng[s] - no[s]) <= 2, (s, ng[s], no[s])
+
+
+def test_fp64_64cell_tiles_vs_oracle(gpu_mod, oracle_mod):
+    pr = obstacle_inflow(ppc=8, n=16, seed=43)
+    steps = 8
+    fo, no = oracle_after(oracle_mod, "obstacle16_ppc8", pr, steps)
+    fg, ng = run_ctx(gpu_mod, pr, gpu_mod.RH_FP64, (4, 4, 4), steps)
+    for name in fo:
+        r = rel(fg[name], fo[name])
+        print("fp64 4x4x4: %s rel L2 = %.3e" % (name, r), flush=True)
+        assert r < 1e-9, (name, r)
+    assert ng == no
+
+
+def exosphere_small(seed=44):
+    """Reduced uneven-load box: 64x48x48 cells, H+ 24 ppc filled at timestep
+    1 and entering through the +x wall, O+ photoions of a corona around an
+    absorbing sphere of 6 cells (scale height 2.5 cells)."""
+    pr = P.config_exosphere(nx=64, ny=48, nz=48, ppc=24, R_cells=6.0, H_cells=2.5, seed=seed, sub=2)
+    pr.species[0].capacity = int(1.4 * 24 * pr.grid.nc)
+    pr.species[1].capacity = 600000
+    return pr
+
+
+PREFILL = 150
+
+
+def exo_prepare(pr):
+    def prep(sim):
+        fill, own = pr.meta["fill"][0], pr.injectors
+        for s, inj, steps in ((0, fill, (1,)), (1, own[1], range(1, PREFILL + 1))):
+            sim.set_injector(s, inj)
+            for t in steps:
+                sim.set_timestep(t)
+                sim.inject(s)
+            sim.set_injector(s, own[s])
+        sim.set_timestep(0)
+    return prep
+
+
+@pytest.mark.parametrize("tile", [(4, 4, 4), (2, 4, 8), (2, 8, 4), (4, 2, 8)])
+def test_uneven_load_box_vs_oracle(gpu_mod, oracle_mod, tile):
+    pr = exosphere_small()
+    steps = 10
+    fo, no = oracle_after(oracle_mod, "exosphere64x48x48", pr, steps, exo_prepare(pr))
+    assert no[1] > 50000  # the O+ cloud is there
+    fg, ng = run_ctx(gpu_mod, pr, gpu_mod.RH_FP32, tile, steps, exo_prepare(pr))
+    for name in ("cellRhoQi", "cellJi", "faceB"):
+        r = rel(fg[name], fo[name])
+        print("exosphere tile %s: %s rel L2 = %.3e" % (tile, name, r), flush=True)
+        assert r < 1e-5, (name, r)
+    for s in range(2):
+        assert abs(ng[s] - no[s]) <= 2, (s, ng[s], no[s])
