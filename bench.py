@@ -63,6 +63,10 @@ def kernel_bytes(name: str, ctx, np_total: float, nspecies: int) -> float:
         return 16.0 * next_ * ntiles + 16.0 * nc
     if name == "sort_slots":  # read and write the 16-bit slot table, write the totals
         return 4.0 * extk * ntiles + 4.0 * nkeys
+    if name == "field_update":  # fused propagateB: read faceB, rho, Ji, eta (32 B), flags (3 B); write faceB,
+        # cellUe, cellEp (36 B) and the per-tile push-field blocks (the last step of a call also writes the
+        # reference's other arrays; not counted)
+        return 71.0 * nc + fblock
     return 0.0
 
 
@@ -126,7 +130,7 @@ def gpu_bench(args, pr):
     names = ["push_fields", "push", "sort_slots", "sort_count", "scan", "sort_base", "move", "deposit",
              "deposit_dense", "fold", "boundary", "inject_slots", "inject_count", "inject_fill", "sort_still",
              "cell_B_bc", "neumann", "imf", "cell2node", "node_J", "node2cell", "cell_J_Ue", "node_Ue_B_E",
-             "node_avg", "face_curl_Ep", "fill", "diag"]
+             "node_avg", "face_curl_Ep", "field_update", "fill", "diag"]
     ktimes = {}
     for nm in names:
         ms, n = ctx.kernel_time(nm)
@@ -230,6 +234,40 @@ def denoiser_bench(patches: int = 8, reps: int = 10):
             "roofline": {"bound": "mfma", "kernel": dom, "achieved": ach, "peak": BF16_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(ach / BF16_PEAK_TFLOPS, 4)},
             "kernels": kern}
+
+
+def denoiser_forward_volume(shape=(256, 192, 192), reps: int = 5):
+    """The denoiser's forward pass on one volume of the uneven-load box's
+    size (the in-loop call), per-layer times from HIP events."""
+    import torch
+    from rhybrid_amd import denoise as D
+    X, Y, Z = shape
+    V = X * Y * Z
+    g = torch.Generator(device="cuda").manual_seed(12)
+    x = torch.randn((1, Z, Y, X, 7), device="cuda", generator=g)
+    d = D.Denoiser(V, scales=[1.0] * 7, weights=D.init_weights(seed=0))
+    d.forward(x)
+    d.synchronize()
+    d.reset_timing()
+    d.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        d.forward(x)
+    d.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    d.set_timing(False)
+    flop_unit = 2.0 * V * 27
+    kern = {}
+    for nm, u in (("pack_input", 0), ("conv1", 7 * 32), ("conv2", 1024), ("conv3", 1024), ("conv4", 128)):
+        ms, n = d.kernel_time(nm)
+        if n:
+            us = ms * 1e3 / n
+            kern[nm] = {"us_avg": round(us, 1)}
+            if u:
+                kern[nm]["TFLOPs"] = round(u * flop_unit / (us * 1e-6) / 1e12, 1)
+    d.close()
+    return {"volume": list(shape), "voxels": V, "ms_forward": t * 1e3,
+            "forward_tflops": D.forward_flops(V) / t / 1e12, "ns_per_voxel": t / V * 1e9, "kernels": kern}
 
 
 def _rel_errors(x_gpu: np.ndarray, x_orc: np.ndarray, dim: int) -> dict:
@@ -442,7 +480,7 @@ def main():
               "the halo and migration exchange between GPUs is not built).", file=sys.stderr)
         sys.exit(2)
     if args.only_denoiser:
-        print(json.dumps({"denoiser": denoiser_bench()}), flush=True)
+        print(json.dumps({"denoiser": denoiser_bench(), "forward_volume": denoiser_forward_volume()}), flush=True)
         return
     if args.calibration_copy_gib > 0:
         calibration_copy(args.calibration_copy_gib)
@@ -451,6 +489,8 @@ def main():
     print("# workload generated in %.1fs" % (time.time() - t0), file=sys.stderr, flush=True)
     res = gpu_bench(args, pr)
     dn = None if args.no_denoiser else denoiser_bench()
+    if dn is not None:
+        dn["forward_volume"] = denoiser_forward_volume()
     if dn is not None and not args.no_in_loop:
         from rhybrid_amd import denoise_loop
         dn["in_loop"] = denoise_loop.compare()

@@ -123,6 +123,7 @@ public:
     virtual void log_write(double t) = 0;
     virtual void write_vlsv(const char* path, double t, int64_t timestep, const char* const* vars, int nvars,
                             const char* const* pops) = 0;
+    virtual void set_temporal_averages(bool on) = 0;
 };
 
 // Boltzmann constant of vth = sqrt(kB T / m) (CODATA 2002, the same table as
@@ -202,6 +203,7 @@ public:
         DevBuf<uint32_t> sub, sub_new;
         DevBuf<uint16_t> cnt_tile;
         DevBuf<Real> partial;  // deposit: per-tile ghost partials (ntiles x next x 4)
+        DevBuf<double> ave_n, ave_v;  // temporal averages: running charge and current sums (FP64)
         bool deposited = false;
         bool pushed = false;  // pass 0 done, pass 1 pending
         PushParams pp{};      // parameters of the pending push
@@ -243,6 +245,7 @@ public:
     hipStream_t stream = nullptr;
     DevBuf<Real> fld[RH_N_FIELDS];
     DevBuf<Real> tmpv;            // old nodeE for the gaussian smoothing
+    DevBuf<Real> faceB_alt;       // the fused field update's second faceB buffer
     DevBuf<uint8_t> flg[RH_N_FLAGS];
     DevBuf<int> ext_cells, bc_cells;
     int n_ext = 0, n_bc = 0;
@@ -288,6 +291,7 @@ public:
             fill(fld[f].p, Real(0), fld[f].n);
         }
         tmpv.alloc((size_t)g.nc * 3);
+        faceB_alt.alloc((size_t)g.nc * 3);
         for (int f = 0; f < RH_N_FLAGS; ++f) {
             flg[f].alloc(g.nc);
             RH_HIP(hipMemsetAsync(flg[f].p, 0, g.nc, stream));
@@ -500,6 +504,7 @@ public:
         s->bc_acc.alloc((size_t)std::max(n_bc, 1) * 4);
         fill(s->bc_acc.p, 0.0, s->bc_acc.n);
         ensure_capacity(*s, (size_t)std::max<int64_t>(d->capacity, 0));
+        ensure_averages(*s);
         sp.push_back(std::move(s));
         RH_HIP(hipStreamSynchronize(stream));
         return (int)sp.size() - 1;
@@ -754,6 +759,41 @@ public:
         }
         launch("fold", k_fold<Real>, cell_grid(), cell_block(), 0, g, src, 1.0 / (g.dx * g.dx * g.dx), P.min_rho,
                fld[RH_CELL_RHOQ].p, ji.x, ji.y, ji.z, dcount.p);
+        if (ave_on)
+            for (auto& s_ : sp)
+                if (s_->deposited)
+                    launch("average", k_average_moments<Real>, cell_grid(), cell_block(), 0, g,
+                           (const Real*)s_->partial.p, s_->d.w * s_->d.q, s_->ave_n.p, s_->ave_v.p);
+    }
+
+    // ---- temporal averages (WRITE_GRID_TEMPORAL_AVERAGES) -------------------
+
+    bool ave_on = false;
+    DevBuf<double> aveB;    // running sum of cellB, SoA
+    int64_t ave_count = 0;  // Hybrid::gridTemporalAverageCounter
+
+    void set_temporal_averages(bool on) override {
+        for (auto& sp_ : sp)
+            if (sp_->pushed) throw StateError("temporal averages switched between propagate and wait_particle_sends");
+        ave_on = on;
+        ave_count = 0;
+        if (!on) {
+            aveB.free();
+            for (auto& sp_ : sp) { sp_->ave_n.free(); sp_->ave_v.free(); }
+            return;
+        }
+        aveB.alloc((size_t)g.nc * 3);
+        fill(aveB.p, 0.0, aveB.n);
+        for (auto& sp_ : sp) ensure_averages(*sp_);
+        RH_HIP(hipStreamSynchronize(stream));
+    }
+
+    void ensure_averages(Species& S_) {
+        if (!ave_on || S_.ave_n.p) return;
+        S_.ave_n.alloc(g.nc);
+        S_.ave_v.alloc((size_t)g.nc * 3);
+        fill(S_.ave_n.p, 0.0, S_.ave_n.n);
+        fill(S_.ave_v.p, 0.0, S_.ave_v.n);
     }
 
     DevBuf<float> dn_in, dn_out, dn_filt;  // denoiser volumes, filtered density (allocated on first use)
@@ -834,9 +874,76 @@ public:
         launch("cell2node", k_cell2node<Real>, cell_grid(), cell_block(), 0, g, (const Real*)fld[from].p, fld[to].p, dim);
     }
 
-    void propagate_B() override {
+    // The fused update (k_field_update) runs without E smoothing and when no
+    // tile of it is one cell thick along a non-periodic axis (an exterior
+    // cell's Neumann source then lies in its own tile).
+    bool fused_field_update() const {
+        if (P.n_linear_smoothings > 0 || P.use_gaussian) return false;
+        const int dims[3] = {g.nx, g.ny, g.nz};
+        const int ft[3] = {FieldTile<Real>::X, FieldTile<Real>::Y, FieldTile<Real>::Z};
+        for (int d = 0; d < 3; ++d)
+            if (!g.per[d] && dims[d] % ft[d] == 1) return false;
+        return true;
+    }
+
+    void propagate_B() override { propagate_B_impl(true); }
+
+    // full: every grid array of the reference gets its new value; otherwise
+    // (the steps of rh_step before its last) only faceB, cellUe, cellEp, the
+    // push-field blocks and, for an attached denoiser, cellB: the arrays the
+    // next step reads.
+    void propagate_B_impl(bool full) {
         for (auto& sp_ : sp)
             if (sp_->pushed) throw StateError("propagate_B between propagate and wait_particle_sends");
+        if (fused_field_update()) {
+            pf_dirty = true;
+            alloc_push_fields();
+            FieldUpdateArgs<Real> a{};
+            a.faceB_in = v3(RH_FACE_B);
+            a.faceB_out = {faceB_alt.p, faceB_alt.p + g.nc, faceB_alt.p + 2 * (size_t)g.nc};
+            a.rho = fld[RH_CELL_RHOQ].p;
+            a.Ji = v3(RH_CELL_JI);
+            a.eta = fld[RH_NODE_ETA].p;
+            a.inner_field = flg[RH_INNER_FIELD].p;
+            a.inner_node = flg[RH_INNER_NODE].p;
+            a.inner_cellEp = flg[RH_INNER_CELLEP].p;
+            a.cellUe = v3(RH_CELL_UE);
+            a.cellEp = v3(RH_CELL_EP);
+            a.cellB = v3(RH_CELL_B);
+            a.cellJ = v3(RH_CELL_J);
+            a.nodeB = v3(RH_NODE_B);
+            a.nodeJi = v3(RH_NODE_JI);
+            a.nodeJ = v3(RH_NODE_J);
+            a.nodeUe = v3(RH_NODE_UE);
+            a.nodeE = v3(RH_NODE_E);
+            a.nodeRho = fld[RH_NODE_RHOQ].p;
+            a.push_blocks = pf_tile.p;
+            a.counters = dcount.p;
+            a.dt = (Real)P.dt;
+            a.e_coeff = (Real)P.e_pressure_coeff;
+            a.max_ue2 = (Real)P.max_electron_speed2;
+            a.min_rho = (Real)P.min_rho;
+            a.max_vw = (Real)P.max_whistler;
+            a.max_E2 = (Real)P.max_E2;
+            for (int l = 0; l < 3; ++l) { a.oue[l] = (Real)P.obstacle_ue[l]; a.imf[l] = (Real)P.imf[l]; }
+            a.imf_cell = P.imf_cell[0] | P.imf_cell[1] << 1 | P.imf_cell[2] << 2 | P.imf_cell[3] << 3 |
+                         P.imf_cell[4] << 4 | P.imf_cell[5] << 5;
+            a.imf_face = P.imf_face[0] | P.imf_face[1] << 1 | P.imf_face[2] << 2 | P.imf_face[3] << 3 |
+                         P.imf_face[4] << 4 | P.imf_face[5] << 5;
+            a.use_hall = P.use_hall;
+            a.use_e_pressure = P.use_e_pressure;
+            a.isothermal = P.isothermal;
+            a.full = full;
+            a.write_cellB = dn_attached != nullptr;
+            a.aveB = ave_on ? aveB.p : nullptr;
+            if (ave_on) ++ave_count;
+            const dim3 grid(cdiv(g.nx, FieldTile<Real>::X), cdiv(g.ny, FieldTile<Real>::Y), cdiv(g.nz, FieldTile<Real>::Z));
+            if (full) launch("field_update", k_field_update<Real, true>, grid, 256, 0, g, a);
+            else launch("field_update", k_field_update<Real, false>, grid, 256, 0, g, a);
+            std::swap(fld[RH_FACE_B].p, faceB_alt.p);
+            pf_dirty = false;
+            return;
+        }
         pf_dirty = true;  // faceB, cellUe and cellEp change
         const dim3 cb = cell_grid(), bb = cell_block();
         const unsigned eb = cdiv(n_ext, 256);
@@ -848,6 +955,11 @@ public:
         // face2Cell, Neumann on cellB, cellJi, cellRhoQi and the IMF on cellB
         launch("cell_B_bc", k_cell_B_bc<Real>, cb, bb, 0, g, v3(RH_FACE_B), v3(RH_CELL_B), v3(RH_CELL_JI),
                fld[RH_CELL_RHOQ].p, imf_cell, bx, by, bz);
+        if (ave_on) {
+            launch("average", k_average_cellB<Real>, cdiv((size_t)g.nc * 3, 256), 256, 0, g.nc,
+                   (const Real*)fld[RH_CELL_B].p, aveB.p);
+            ++ave_count;
+        }
         // Neumann and IMF on faceB (they do not touch the cell arrays that
         // cell2Node reads), then cell2Node of cellB, cellRhoQi, cellJi and
         // calcNodeJ in one launch
@@ -906,7 +1018,9 @@ public:
             for (size_t s = 0; s < sp.size(); ++s) apply_bcs((int)s);
             for (size_t s = 0; s < sp.size(); ++s) inject((int)s);
             if (dn_attached) denoise_on_stream(dn_attached, dn_bound);
-            propagate_B();
+            // only the last step's update writes the arrays nothing in
+            // between reads
+            propagate_B_impl(it == n - 1);
         }
         check_overflow();
     }
@@ -1335,48 +1449,24 @@ public:
         static const char* known[] = {"faceB", "cellRhoQi", "cellB", "cellJ", "cellUe", "cellJi", "cellEp",
                                       "nodeRhoQi", "nodeE", "nodeB", "nodeJ", "nodeUe", "nodeJi", "nodeEta",
                                       "innerFlagField", "innerFlagNode", "innerFlagParticle", "innerFlagCellEp",
-                                      "ne", "cellDivB", "cellNPles", "n", "v", "T"};
+                                      "ne", "cellDivB", "cellNPles", "n", "v", "T", "cellBAverage", "n_ave",
+                                      "v_ave"};
         for (const auto& kv : want) {
             bool ok = false;
             for (const char* k : known) ok = ok || kv.first == k;
             if (!ok) throw ArgError("unknown VLSV variable " + kv.first);
         }
-        if ((want.count("n") || want.count("v") || want.count("T")) && !pops)
-            throw ArgError("population names are required for n, v and T");
+        if ((want.count("n") || want.count("v") || want.count("T") || want.count("n_ave") || want.count("v_ave")) &&
+            !pops)
+            throw ArgError("population names are required for n, v, T, n_ave and v_ave");
+        if ((want.count("cellBAverage") || want.count("n_ave") || want.count("v_ave")) && !ave_on)
+            throw ArgError("temporal averages are off (rh_set_temporal_averages)");
         for (auto& sp_ : sp) flush_injected(*sp_);
         const uint64_t nc = (uint64_t)g.nc;
         VlsvWriter W(path);
-        auto param = [&](const char* name, const void* v, const char* type, uint64_t size) {
-            W.array("PARAMETER", {{"name", name}}, v, 1, 1, type, size);
-        };
-        const double xmax = g.x0 + g.nx * g.dx, ymax = g.y0 + g.ny * g.dx, zmax = g.z0 + g.nz * g.dx;
-        const uint64_t ts = (uint64_t)timestep, cx = (uint64_t)g.nx, cy = (uint64_t)g.ny, cz = (uint64_t)g.nz;
-        param("time", &t, "float", 8);
-        param("t", &t, "float", 8);
-        param("timestep", &ts, "uint", 8);
-        param("dt", &P.dt, "float", 8);
-        param("xmin", &g.x0, "float", 8); param("ymin", &g.y0, "float", 8); param("zmin", &g.z0, "float", 8);
-        param("xmax", &xmax, "float", 8); param("ymax", &ymax, "float", 8); param("zmax", &zmax, "float", 8);
-        param("xcells_ini", &cx, "uint", 8); param("ycells_ini", &cy, "uint", 8); param("zcells_ini", &cz, "uint", 8);
-        const std::map<std::string, std::string> mesh{{"mesh", "SpatialGrid"}};
-        const uint64_t bbox[6] = {cx, cy, cz, 1, 1, 1};
-        W.array("MESH_BBOX", mesh, bbox, 6, 1, "uint", 8);
-        const double o3[3] = {g.x0, g.y0, g.z0};
-        const int dims[3] = {g.nx, g.ny, g.nz};
-        const char* crd_tags[3] = {"MESH_NODE_CRDS_X", "MESH_NODE_CRDS_Y", "MESH_NODE_CRDS_Z"};
-        for (int d = 0; d < 3; ++d) {
-            std::vector<double> c((size_t)dims[d] + 1);
-            for (int q = 0; q <= dims[d]; ++q) c[q] = o3[d] + q * g.dx;
-            W.array(crd_tags[d], mesh, c.data(), c.size(), 1, "float", 8);
-        }
-        std::vector<uint64_t> ids(nc);
-        for (uint64_t n = 0; n < nc; ++n) ids[n] = n;
-        W.array("MESH", {{"name", "SpatialGrid"}, {"type", "multi_ucd"}}, ids.data(), nc, 1, "uint", 8);
-        W.array("VARIABLE", {{"name", "CellID"}, {"mesh", "SpatialGrid"}, {"type", "celldata"}}, ids.data(), nc, 1,
-                "uint", 8);
+        vlsv_write_grid(W, VlsvGrid{g.nx, g.ny, g.nz, g.dx, g.x0, g.y0, g.z0}, t, timestep, P.dt);
         auto var = [&](const std::string& name, const void* data, uint64_t vec, const char* type, uint64_t size) {
-            W.array("VARIABLE", {{"name", name}, {"mesh", "SpatialGrid"}, {"type", "celldata"}}, data, nc, vec, type,
-                    size);
+            vlsv_write_cell_var(W, nc, name, data, vec, type, size);
         };
         std::vector<double> h;
         auto field_var = [&](const char* name, int id) {
@@ -1476,7 +1566,63 @@ public:
                 if (want.count("v")) var("v_" + name, vel.data(), 3, "float", 8);
             }
         }
+        if (ave_on) write_averages(want, pops, var);
         W.close();
+    }
+
+    // The temporal averages of UserDataOP::writeData (src/operator_userdata.
+    // cpp:156-232): cellBAverage = sum / count; per output population
+    // n_<pop>_ave = sum of the members' charge / (q dV) / count (q of the
+    // first member) and v_<pop>_ave = sum of current / sum of charge; the
+    // sums that are written restart, and so does the count.
+    template <typename VarFn>
+    void write_averages(const std::map<std::string, bool>& want, const char* const* pops, VarFn& var) {
+        const uint64_t nc = (uint64_t)g.nc;
+        const double inv = ave_count > 0 ? 1.0 / (double)ave_count : 0.0;
+        if (want.count("cellBAverage")) {
+            std::vector<double> h(nc * 3), out(nc * 3);
+            RH_HIP(hipMemcpyAsync(h.data(), aveB.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+            RH_HIP(hipStreamSynchronize(stream));
+            for (uint64_t n = 0; n < nc; ++n)
+                for (int l = 0; l < 3; ++l) out[3 * n + l] = h[(size_t)l * nc + n] * inv;
+            var("cellBAverage", out.data(), 3, "float", 8);
+            fill(aveB.p, 0.0, aveB.n);
+        }
+        if (want.count("n_ave") || want.count("v_ave")) {
+            std::vector<std::string> order;
+            for (size_t s = 0; s < sp.size(); ++s) {
+                if (!pops[s]) throw ArgError("null population name");
+                if (std::find(order.begin(), order.end(), std::string(pops[s])) == order.end()) order.push_back(pops[s]);
+            }
+            const double dV = g.dx * g.dx * g.dx;
+            for (const auto& name : order) {
+                std::vector<double> nsum(nc, 0.0), vsum(nc * 3, 0.0), hn(nc), hv(nc * 3);
+                double q0 = 0.0;
+                for (size_t s = 0; s < sp.size(); ++s) {
+                    if (name != pops[s]) continue;
+                    if (q0 == 0.0) q0 = sp[s]->d.q;
+                    RH_HIP(hipMemcpyAsync(hn.data(), sp[s]->ave_n.p, nc * sizeof(double), hipMemcpyDeviceToHost, stream));
+                    RH_HIP(hipMemcpyAsync(hv.data(), sp[s]->ave_v.p, hv.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                          stream));
+                    RH_HIP(hipStreamSynchronize(stream));
+                    for (uint64_t n = 0; n < nc; ++n) {
+                        nsum[n] += hn[n];
+                        for (int l = 0; l < 3; ++l) vsum[3 * n + l] += hv[(size_t)l * nc + n];
+                    }
+                    fill(sp[s]->ave_n.p, 0.0, sp[s]->ave_n.n);
+                    fill(sp[s]->ave_v.p, 0.0, sp[s]->ave_v.n);
+                }
+                std::vector<double> dens(nc), vel(nc * 3);
+                for (uint64_t n = 0; n < nc; ++n) {
+                    dens[n] = nsum[n] / (q0 * dV) * inv;
+                    for (int l = 0; l < 3; ++l) vel[3 * n + l] = nsum[n] > 0 ? vsum[3 * n + l] / nsum[n] : 0.0;
+                }
+                if (want.count("n_ave")) var("n_" + name + "_ave", dens.data(), 1, "float", 8);
+                if (want.count("v_ave")) var("v_" + name + "_ave", vel.data(), 3, "float", 8);
+            }
+        }
+        ave_count = 0;
+        RH_HIP(hipStreamSynchronize(stream));
     }
 
     int64_t injected(int s) override {
@@ -1748,6 +1894,9 @@ int rh_write_vlsv(rh_ctx* ctx, const char* filename, double t, int64_t timestep,
                   int32_t nvariables, const char* const* population_names) {
     if (nvariables < 0) return RH_ERR_ARG;
     return guarded(ctx, [&](rh::Ctx& c) { c.write_vlsv(filename, t, timestep, variables, nvariables, population_names); });
+}
+int rh_set_temporal_averages(rh_ctx* ctx, int32_t enable) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.set_temporal_averages(enable != 0); });
 }
 int rh_layout_info(rh_ctx* ctx, int64_t out[6]) {
     if (!out) return RH_ERR_ARG;

@@ -555,4 +555,374 @@ __global__ __launch_bounds__(256) void k_face_curl_Ep(Geom g, V3<Real> E, V3<Rea
     }
 }
 
+// ---- the fused field update ------------------------------------------------
+//
+// propagateB without E smoothing (src/hybrid_propagator.cpp:42-440, steps
+// 1-8 and 14-15) in one launch over field tiles of FT cells, with every
+// stencil input staged in LDS:
+//   faceB of the tile grown by two cells below and one above (after the
+//   exterior cells' neumannFace and setIMFFace copies), cellRhoQi, cellJi
+//   and cellB of the tile grown by one cell (exterior cells: neumannCell
+//   copies of their source and setIMFCell; interior cellB by face2Cell),
+//   then the nodes of the tile grown by one node below (cell2Node of cellB,
+//   cellRhoQi and cellJi, calcNodeJ, calcNodeUe, upwindNodeB, calcNodeE) in
+//   LDS, then for the tile's own cells node2Cell of nodeJ with calcCellUe
+//   (exterior cells: their source's value), calcCellEp and faceCurl.  A
+//   node of the halo is computed by every tile that holds it, and counted
+//   only by the tile that owns it.  faceB is read from one buffer and
+//   written to another (tiles read their neighbours' old faces as halo).
+//   The arithmetic of every stage is the same as in the kernels above.
+// The exterior cells' Neumann sources lie in the same tile when no tile is
+// one cell thick along a non-periodic axis (the host checks n % FT != 1).
+template <typename Real> struct FieldTile;
+template <> struct FieldTile<float> { static constexpr int X = 8, Y = 8, Z = 8; };
+template <> struct FieldTile<double> { static constexpr int X = 8, Y = 4, Z = 4; };
+
+template <typename Real>
+struct FieldUpdateArgs {
+    V3<Real> faceB_in, faceB_out;
+    Real* rho;                // cellRhoQi (exterior cells: their Neumann copies, full update)
+    V3<Real> Ji;              // cellJi (idem)
+    const Real* eta;          // nodeEta
+    const uint8_t* inner_field;
+    const uint8_t* inner_node;
+    const uint8_t* inner_cellEp;
+    V3<Real> cellUe, cellEp;
+    // the reference's other arrays (full update; cellB also when write_cellB)
+    V3<Real> cellB, cellJ, nodeB, nodeJi, nodeJ, nodeUe, nodeE;
+    Real* nodeRho;
+    uint32_t* push_blocks;    // the next push's per-tile field blocks (null: none)
+    double* aveB;             // cellAverageB += cellB (temporal averages; null: off)
+    unsigned long long* counters;
+    Real dt, e_coeff, max_ue2, min_rho, max_vw, max_E2;
+    Real oue[3];              // fieldObstacleUe
+    Real imf[3];
+    int imf_cell, imf_face;
+    int use_hall, use_e_pressure, isothermal;
+    int full, write_cellB;
+};
+
+// calcCellUe / calcNodeUe on values (ue_at above, NODE clamps r)
+template <typename Real, bool NODE>
+__device__ __forceinline__ void ue_core(const Real (&jj)[3], const Real (&ji)[3], Real& r, bool inner, int use_hall,
+                                        const Real (&oue)[3], Real max_ue2, Real min_rho, Real (&u)[3], bool& c_ue,
+                                        bool& c_rho) {
+    if (inner) {
+        u[0] = oue[0]; u[1] = oue[1]; u[2] = oue[2];
+        return;
+    }
+    if (NODE && r < min_rho) { r = min_rho; c_rho = true; }
+    for (int l = 0; l < 3; ++l) {
+        if (fabs(r) > Real(0)) u[l] = use_hall ? -(jj[l] - ji[l]) / r : ji[l] / r;
+        else u[l] = Real(0);
+    }
+    const Real u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    if (u2 > max_ue2) {
+        const Real norm = sqrt(max_ue2 / u2);
+        u[0] *= norm; u[1] *= norm; u[2] *= norm;
+        c_ue = true;
+    }
+}
+
+// setIMFCell / setIMFFace on a cell's values (imf_at above)
+template <typename Real>
+__device__ __forceinline__ void imf_values(const Geom& g, int i, int j, int k, int face, int flags, const Real (&b)[3],
+                                           Real (&v)[3]) {
+    const int miss = missing_mask(g, i, j, k);
+    const int wall_bit[6] = {2, 1, 8, 4, 32, 16};
+    for (int w = 0; w < 6; ++w) {
+        if (!((flags >> w) & 1) || !(miss & wall_bit[w])) continue;
+        const int axis = w / 2;
+        if (!face || axis != 0) v[0] = b[0];
+        if (!face || axis != 1) v[1] = b[1];
+        if (!face || axis != 2) v[2] = b[2];
+    }
+}
+
+template <typename Real, bool FULL>
+__global__ __launch_bounds__(256) void k_field_update(Geom g, FieldUpdateArgs<Real> a) {
+    constexpr int TX = FieldTile<Real>::X, TY = FieldTile<Real>::Y, TZ = FieldTile<Real>::Z;
+    constexpr int X2 = TX + 3, Y2 = TY + 3, Z2 = TZ + 3, N2 = X2 * Y2 * Z2;  // faceB: cells -2..T
+    constexpr int X1 = TX + 2, Y1 = TY + 2, Z1 = TZ + 2, N1 = X1 * Y1 * Z1;  // cells -1..T
+    constexpr int XN = TX + 1, YN = TY + 1, ZN = TZ + 1, NN = XN * YN * ZN;  // nodes -1..T-1
+    __shared__ Real sF[3][N2];
+    __shared__ Real sB[3][N1], sJi[3][N1], sR[N1];
+    __shared__ Real nJ[3][NN], nE[3][NN], nR[NN];
+    const int tid = threadIdx.x;
+    const int ox = blockIdx.x * TX, oy = blockIdx.y * TY, oz = blockIdx.z * TZ;
+    const V3<Real>& F = a.faceB_in;
+
+    // 1. faceB (exterior cells: neumannFace, then setIMFFace), cellRhoQi,
+    //    cellJi and the exterior cells' cellB (neumannCell of the source's
+    //    face2Cell, then setIMFCell)
+    for (int e = tid; e < N2; e += blockDim.x) {
+        const int lx = e % X2, ly = (e / X2) % Y2, lz = e / (X2 * Y2);
+        const int i = wrap_axis(ox - 2 + lx, g.nx, g.per[0]), j = wrap_axis(oy - 2 + ly, g.ny, g.per[1]),
+                  k = wrap_axis(oz - 2 + lz, g.nz, g.per[2]);
+        const bool in1 = lx >= 1 && ly >= 1 && lz >= 1;
+        const int e1 = in1 ? (lx - 1) + X1 * ((ly - 1) + Y1 * (lz - 1)) : 0;
+        if (i < 0 || j < 0 || k < 0) {
+            sF[0][e] = sF[1][e] = sF[2][e] = Real(0);
+            if (in1) {
+                sB[0][e1] = sB[1][e1] = sB[2][e1] = Real(0);
+                sJi[0][e1] = sJi[1][e1] = sJi[2][e1] = Real(0);
+                sR[e1] = Real(0);
+            }
+            continue;
+        }
+        const int n = cell_index(g, i, j, k);
+        if (all_nbrs(g, i, j, k)) {
+            sF[0][e] = F.x[n]; sF[1][e] = F.y[n]; sF[2][e] = F.z[n];
+            if (in1) {
+                sJi[0][e1] = a.Ji.x[n]; sJi[1][e1] = a.Ji.y[n]; sJi[2][e1] = a.Ji.z[n];
+                sR[e1] = a.rho[n];
+            }
+            continue;
+        }
+        const int miss = missing_mask(g, i, j, k);
+        Real f[3];
+        for (int l = 0; l < 3; ++l) {
+            const FaceSrc R = c_face_src.t[miss][l];
+            const int src = R.found ? nbr(g, i, j, k, R.a, R.b, R.c) : -1;
+            f[l] = F.c(l, src >= 0 ? src : n);
+        }
+        if (a.imf_face) imf_values(g, i, j, k, 1, a.imf_face, a.imf, f);
+        sF[0][e] = f[0]; sF[1][e] = f[1]; sF[2][e] = f[2];
+        if (in1) {
+            const int src = neumann_source(g, i, j, k);
+            int si, sj, sk;
+            cell_ijk(g, src, si, sj, sk);
+            Real c[3];
+            face2cell_at(g, si, sj, sk, src, F, c[0], c[1], c[2]);
+            if (a.imf_cell) imf_values(g, i, j, k, 0, a.imf_cell, a.imf, c);
+            sB[0][e1] = c[0]; sB[1][e1] = c[1]; sB[2][e1] = c[2];
+            sJi[0][e1] = a.Ji.x[src]; sJi[1][e1] = a.Ji.y[src]; sJi[2][e1] = a.Ji.z[src];
+            sR[e1] = a.rho[src];
+        }
+    }
+    __syncthreads();
+    // 2. face2Cell of the interior cells (their lower faces are never
+    //    changed by the exterior copies: see k_cell_B_bc)
+    for (int e = tid; e < N1; e += blockDim.x) {
+        const int lx = e % X1, ly = (e / X1) % Y1, lz = e / (X1 * Y1);
+        const int i = wrap_axis(ox - 1 + lx, g.nx, g.per[0]), j = wrap_axis(oy - 1 + ly, g.ny, g.per[1]),
+                  k = wrap_axis(oz - 1 + lz, g.nz, g.per[2]);
+        if (i < 0 || j < 0 || k < 0 || !all_nbrs(g, i, j, k)) continue;
+        const int f = (lx + 1) + X2 * ((ly + 1) + Y2 * (lz + 1));
+        sB[0][e] = Real(0.5) * (sF[0][f] + sF[0][f - 1]);
+        sB[1][e] = Real(0.5) * (sF[1][f] + sF[1][f - X2]);
+        sB[2][e] = Real(0.5) * (sF[2][f] + sF[2][f - X2 * Y2]);
+    }
+    __syncthreads();
+    // 3. the nodes: cell2Node, calcNodeJ, calcNodeUe, upwindNodeB, calcNodeE
+    for (int q = tid; q < NN; q += blockDim.x) {
+        const int qx = q % XN, qy = (q / XN) % YN, qz = q / (XN * YN);
+        const int i = wrap_axis(ox - 1 + qx, g.nx, g.per[0]), j = wrap_axis(oy - 1 + qy, g.ny, g.per[1]),
+                  k = wrap_axis(oz - 1 + qz, g.nz, g.per[2]);
+        const bool ok = i >= 0 && j >= 0 && k >= 0 && !upper_missing(g, i, j, k);
+        // the node of one of the tile's own cells (a partial tile's halo may
+        // wrap onto cells of another tile along a periodic axis)
+        const bool own = ok && qx >= 1 && qy >= 1 && qz >= 1 && ox - 1 + qx < g.nx && oy - 1 + qy < g.ny &&
+                         oz - 1 + qz < g.nz;
+        bool c_vw = false, c_ue = false, c_rho = false, c_e = false;
+        if (ok) {
+            const int n = cell_index(g, i, j, k);
+            const int c = qx + X1 * (qy + Y1 * qz);  // the node's cell in the cell region
+            const int c001 = c + X1 * Y1, c010 = c + X1, c100 = c + 1;
+            const int c011 = c010 + X1 * Y1, c110 = c010 + 1, c101 = c001 + 1, c111 = c011 + 1;
+            auto avg = [&](const Real* v) {
+                return Real(0.125) * (v[c] + v[c001] + v[c010] + v[c100] + v[c011] + v[c110] + v[c101] + v[c111]);
+            };
+            const Real bx = avg(sB[0]), by = avg(sB[1]), bz = avg(sB[2]);
+            Real r = avg(sR);
+            const Real ji[3] = {avg(sJi[0]), avg(sJi[1]), avg(sJi[2])};
+            // calcNodeJ from faceB of the cells n, +x, +y, +z, +xy, +xz, +yz
+            const int f = (qx + 1) + X2 * ((qy + 1) + Y2 * (qz + 1));
+            const int f100 = f + 1, f010 = f + X2, f001 = f + X2 * Y2;
+            const int f110 = f010 + 1, f101 = f001 + 1, f011 = f001 + X2;
+            const Real ex1 = -sF[2][f] + sF[1][f] + sF[2][f010] - sF[1][f001];
+            const Real ex2 = -sF[2][f100] + sF[1][f100] + sF[2][f110] - sF[1][f101];
+            const Real ey1 = +sF[2][f] + sF[0][f001] - sF[2][f100] - sF[0][f];
+            const Real ey2 = +sF[2][f010] + sF[0][f011] - sF[2][f110] - sF[0][f010];
+            const Real ez1 = -sF[1][f] + sF[0][f] + sF[1][f100] - sF[0][f010];
+            const Real ez2 = -sF[1][f001] + sF[0][f001] + sF[1][f101] - sF[0][f011];
+            const Real qe = Real(1.60217653e-19);
+            const Real mu0 = Real(4.0e-7 * 3.14159265358979323846);
+            Real d = Real(1), vw = Real(0);
+            const Real ne = r / qe;
+            const Real Btot = sqrt(bx * bx + by * by + bz * bz);
+            const Real dx = (Real)g.dx;
+            if (ne > Real(0) && dx > Real(0)) vw = Real(2) * Btot * Real(3.14159265358979323846) / (mu0 * ne * qe * dx);
+            if (vw > a.max_vw && a.max_vw > Real(0)) { d = vw / a.max_vw; c_vw = true; }
+            const Real af = Real(0.5) / (dx * mu0 * d);
+            const Real J[3] = {(ex1 + ex2) * af, (ey1 + ey2) * af, (ez1 + ez2) * af};
+            // calcNodeUe (clamps the node's rho)
+            Real u[3];
+            ue_core<Real, true>(J, ji, r, a.inner_node[n] != 0, a.use_hall, a.oue, a.max_ue2, a.min_rho, u, c_ue, c_rho);
+            // upwindNodeB: cells in the reference's order 111,112,121,211,122,221,212,222
+            Real xu = Real(-0.5), yu = Real(0), zu = Real(0);
+            const Real ue = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            if (ue > Real(0)) {
+                const Real am = Real(-0.5) / ue;
+                xu = u[0] * am; yu = u[1] * am; zu = u[2] * am;
+            }
+            const int cc[8] = {c, c001, c010, c100, c011, c110, c101, c111};
+            const int off[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}, {0, 1, 1}, {1, 1, 0}, {1, 0, 1}, {1, 1, 1}};
+            Real w[8], wsum = Real(0);
+            for (int t = 0; t < 8; ++t) {
+                const Real xc = off[t][0] ? Real(0.5) : Real(-0.5);
+                const Real yc = off[t][1] ? Real(0.5) : Real(-0.5);
+                const Real zc = off[t][2] ? Real(0.5) : Real(-0.5);
+                w[t] = Real(1) / sqrt((xc - xu) * (xc - xu) + (yc - yu) * (yc - yu) + (zc - zu) * (zc - zu));
+                wsum += w[t];
+            }
+            Real b[3];
+            if (wsum > Real(0)) {
+                Real b0 = 0, b1 = 0, b2 = 0;
+                for (int t = 0; t < 8; ++t) {
+                    b0 += w[t] * sB[0][cc[t]];
+                    b1 += w[t] * sB[1][cc[t]];
+                    b2 += w[t] * sB[2][cc[t]];
+                }
+                b[0] = b0 / wsum; b[1] = b1 / wsum; b[2] = b2 / wsum;
+            } else {
+                b[0] = b[1] = b[2] = Real(0);
+            }
+            Real E[3];
+            c_e = node_E_at(u, b, a.eta[n], J[0], J[1], J[2], a.max_E2, E);
+            for (int l = 0; l < 3; ++l) { nJ[l][q] = J[l]; nE[l][q] = E[l]; }
+            nR[q] = r;
+            if (FULL && own) {
+                a.nodeB.x[n] = b[0]; a.nodeB.y[n] = b[1]; a.nodeB.z[n] = b[2];
+                a.nodeRho[n] = r;
+                a.nodeJi.x[n] = ji[0]; a.nodeJi.y[n] = ji[1]; a.nodeJi.z[n] = ji[2];
+                a.nodeJ.x[n] = J[0]; a.nodeJ.y[n] = J[1]; a.nodeJ.z[n] = J[2];
+                a.nodeUe.x[n] = u[0]; a.nodeUe.y[n] = u[1]; a.nodeUe.z[n] = u[2];
+                a.nodeE.x[n] = E[0]; a.nodeE.y[n] = E[1]; a.nodeE.z[n] = E[2];
+            }
+        }
+        count_event(a.counters, C_MAX_VW, c_vw && own);
+        count_event(a.counters, C_MAX_NODE_UE, c_ue && own);
+        count_event(a.counters, C_MIN_NODE_RHO, c_rho && own);
+        count_event(a.counters, C_MAX_E, c_e && own);
+    }
+    __syncthreads();
+    // 4. the tile's cells: node2Cell + calcCellUe (exterior cells: the
+    //    source's value), calcCellEp, faceCurl, the push-field blocks
+    for (int t = tid; t < TX * TY * TZ; t += blockDim.x) {
+        const int rx = t % TX, ry = (t / TX) % TY, rz = t / (TX * TY);
+        const int i = ox + rx, j = oy + ry, k = oz + rz;
+        bool c_ue = false;
+        if (i < g.nx && j < g.ny && k < g.nz) {
+            const int n = cell_index(g, i, j, k);
+            const int c1 = (rx + 1) + X1 * ((ry + 1) + Y1 * (rz + 1));
+            const bool interior = all_nbrs(g, i, j, k);
+            // node2Cell of nodeJ at the cell whose node sits at node index qn
+            auto cellJ_at = [&](int qn, Real (&cj)[3]) {
+                const int m100 = qn - 1, m010 = qn - XN, m001 = qn - XN * YN;
+                const int m110 = m010 - 1, m011 = m001 - XN, m101 = m001 - 1, m111 = m011 - 1;
+                for (int l = 0; l < 3; ++l) {
+                    const Real* v = nJ[l];
+                    cj[l] = Real(0.125) * (v[qn] + v[m100] + v[m110] + v[m010] + v[m001] + v[m011] + v[m101] + v[m111]);
+                }
+            };
+            const int qn = (rx + 1) + XN * ((ry + 1) + YN * (rz + 1));
+            Real u[3], cj[3] = {Real(0), Real(0), Real(0)};
+            bool c_rho = false;
+            if (interior) {
+                cellJ_at(qn, cj);
+                Real r = sR[c1];
+                const Real ji[3] = {sJi[0][c1], sJi[1][c1], sJi[2][c1]};
+                ue_core<Real, false>(cj, ji, r, a.inner_field[n] != 0, a.use_hall, a.oue, a.max_ue2, a.min_rho, u, c_ue,
+                                     c_rho);
+            } else {
+                const int src = neumann_source(g, i, j, k);
+                int si, sj, sk;
+                cell_ijk(g, src, si, sj, sk);
+                const int dxs = si - i, dys = sj - j, dzs = sk - k;  // -1, 0 or +1 (no wrap: exterior)
+                Real cs[3];
+                cellJ_at(qn + dxs + XN * (dys + YN * dzs), cs);
+                const int cs1 = c1 + dxs + X1 * (dys + Y1 * dzs);
+                Real r = sR[cs1];
+                const Real ji[3] = {sJi[0][cs1], sJi[1][cs1], sJi[2][cs1]};
+                bool c0 = false;
+                ue_core<Real, false>(cs, ji, r, a.inner_field[src] != 0, a.use_hall, a.oue, a.max_ue2, a.min_rho, u, c0,
+                                     c_rho);
+            }
+            a.cellUe.x[n] = u[0]; a.cellUe.y[n] = u[1]; a.cellUe.z[n] = u[2];
+            // calcCellEp from the clamped nodeRhoQi
+            Real ep[3] = {Real(0), Real(0), Real(0)};
+            if (a.use_e_pressure) {
+                if (interior) {
+                    if (!a.inner_cellEp[n]) {
+                        auto N = [&](int ax, int ay, int az) { return nR[qn + ax + XN * (ay + YN * az)]; };
+                        const Real xP = Real(0.25) * (N(0, 0, 0) + N(0, 0, -1) + N(0, -1, -1) + N(0, -1, 0));
+                        const Real xN = Real(0.25) * (N(-1, 0, 0) + N(-1, 0, -1) + N(-1, -1, -1) + N(-1, -1, 0));
+                        const Real yP = Real(0.25) * (N(0, 0, 0) + N(0, 0, -1) + N(-1, 0, -1) + N(-1, 0, 0));
+                        const Real yN = Real(0.25) * (N(0, -1, 0) + N(0, -1, -1) + N(-1, -1, -1) + N(-1, -1, 0));
+                        const Real zP = Real(0.25) * (N(0, 0, 0) + N(0, -1, 0) + N(-1, -1, 0) + N(-1, 0, 0));
+                        const Real zN = Real(0.25) * (N(0, 0, -1) + N(0, -1, -1) + N(-1, -1, -1) + N(-1, 0, -1));
+                        const Real dx = (Real)g.dx;
+                        const Real gx = (xP - xN) / dx, gy = (yP - yN) / dx, gz = (zP - zN) / dx;
+                        const Real crho = sR[c1];
+                        if (!a.isothermal) {
+                            ep[0] = -a.e_coeff * gx; ep[1] = -a.e_coeff * gy; ep[2] = -a.e_coeff * gz;
+                        } else if (fabs(crho) > Real(0)) {
+                            ep[0] = -a.e_coeff * gx / crho; ep[1] = -a.e_coeff * gy / crho; ep[2] = -a.e_coeff * gz / crho;
+                        }
+                    }
+                    a.cellEp.x[n] = ep[0]; a.cellEp.y[n] = ep[1]; a.cellEp.z[n] = ep[2];
+                } else {
+                    // cells the reference's calcCellEp skips keep their values
+                    ep[0] = a.cellEp.x[n]; ep[1] = a.cellEp.y[n]; ep[2] = a.cellEp.z[n];
+                }
+            }
+            // faceCurl (Faraday) on the faces after their exterior copies
+            const int f = (rx + 2) + X2 * ((ry + 2) + Y2 * (rz + 2));
+            Real fb[3] = {sF[0][f], sF[1][f], sF[2][f]};
+            if (!upper_missing(g, i, j, k)) {
+                const bool xn = has_neg(i, g.per[0]), yn = has_neg(j, g.per[1]), zn = has_neg(k, g.per[2]);
+                const Real dx = (Real)g.dx;
+                auto E = [&](int l, int ax, int ay, int az) { return nE[l][qn + ax + XN * (ay + YN * az)]; };
+                if (yn && zn) {
+                    const Real curl = Real(0.5) * (+E(2, 0, -1, 0) + E(2, 0, -1, -1) - E(1, 0, -1, -1) - E(1, 0, 0, -1) -
+                                                   E(2, 0, 0, -1) - E(2, 0, 0, 0) + E(1, 0, 0, 0) + E(1, 0, -1, 0)) / dx;
+                    fb[0] += a.dt * curl;
+                }
+                if (xn && zn) {
+                    const Real curl = Real(0.5) * (-E(0, -1, 0, 0) - E(0, 0, 0, 0) + E(2, 0, 0, 0) + E(2, 0, 0, -1) +
+                                                   E(0, 0, 0, -1) + E(0, -1, 0, -1) - E(2, -1, 0, -1) - E(2, -1, 0, 0)) / dx;
+                    fb[1] += a.dt * curl;
+                }
+                if (xn && yn) {
+                    const Real curl = Real(0.5) * (-E(0, -1, -1, 0) - E(0, 0, -1, 0) - E(1, 0, -1, 0) - E(1, 0, 0, 0) +
+                                                   E(0, 0, 0, 0) + E(0, -1, 0, 0) + E(1, -1, 0, 0) + E(1, -1, -1, 0)) / dx;
+                    fb[2] += a.dt * curl;
+                }
+            }
+            a.faceB_out.x[n] = fb[0]; a.faceB_out.y[n] = fb[1]; a.faceB_out.z[n] = fb[2];
+            if (FULL || a.write_cellB) {
+                a.cellB.x[n] = sB[0][c1]; a.cellB.y[n] = sB[1][c1]; a.cellB.z[n] = sB[2][c1];
+            }
+            if (a.aveB) {
+                a.aveB[n] += (double)sB[0][c1];
+                a.aveB[(size_t)g.nc + n] += (double)sB[1][c1];
+                a.aveB[2 * (size_t)g.nc + n] += (double)sB[2][c1];
+            }
+            if (FULL) {
+                if (interior) {
+                    a.cellJ.x[n] = cj[0]; a.cellJ.y[n] = cj[1]; a.cellJ.z[n] = cj[2];
+                } else {
+                    a.rho[n] = sR[c1];
+                    a.Ji.x[n] = sJi[0][c1]; a.Ji.y[n] = sJi[1][c1]; a.Ji.z[n] = sJi[2][c1];
+                }
+            }
+            if (a.push_blocks)
+                push_fields_store(g, i, j, k, fb[0], fb[1], fb[2], u[0], u[1], u[2], ep[0], ep[1], ep[2], a.use_e_pressure,
+                                  a.push_blocks);
+        }
+        count_event(a.counters, C_MAX_CELL_UE, c_ue);
+    }
+}
+
 }  // namespace rh

@@ -959,6 +959,42 @@ __global__ __launch_bounds__(256) void k_fold(Geom g, FoldSources<Real> src, dou
     count_event(counters, C_MIN_CELL_RHO, floored);
 }
 
+// WRITE_GRID_TEMPORAL_AVERAGES of the accumulator (src/particle_accumulator.
+// cpp:148-151, 287-300): one species' charge and current sums of this step
+// (its tile partials folded in k_fold's fixed order, times w q, before /dV
+// and the floor) added to its running FP64 sums nave[n], vave[l*nc + n].
+template <typename Real>
+__global__ __launch_bounds__(256) void k_average_moments(Geom g, const Real* __restrict__ partial, double wq,
+                                                        double* __restrict__ nave, double* __restrict__ vave) {
+    const int i = blockIdx.x * 32 + threadIdx.x, j = blockIdx.y * 8 + threadIdx.y, k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int n = cell_index(g, i, j, k);
+    int tx[6], ex[6], ty[6], ey[6], tz[6], ez[6];
+    const int mx = axis_candidates(i, g.nx, g.ltx, g.ntx, g.per[0], tx, ex);
+    const int my = axis_candidates(j, g.ny, g.lty, g.nty, g.per[1], ty, ey);
+    const int mz = axis_candidates(k, g.nz, g.ltz, g.ntz, g.per[2], tz, ez);
+    double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+    for (int c = 0; c < mz; ++c)
+        for (int b = 0; b < my; ++b)
+            for (int a = 0; a < mx; ++a) {
+                const int TT = tx[a] + g.ntx * (ty[b] + g.nty * tz[c]);
+                const int e = ex[a] + g.ex * (ey[b] + g.ey * ez[c]);
+                const Real4<Real> v = reinterpret_cast<const Real4<Real>*>(partial)[(size_t)TT * g.next + e];
+                t0 += (double)v.v[0]; t1 += (double)v.v[1]; t2 += (double)v.v[2]; t3 += (double)v.v[3];
+            }
+    nave[n] += wq * t0;
+    vave[n] += wq * t1;
+    vave[(size_t)g.nc + n] += wq * t2;
+    vave[2 * (size_t)g.nc + n] += wq * t3;
+}
+
+// cellAverageB += cellB (src/hybrid_propagator.cpp:157-167), FP64 sums
+template <typename Real>
+__global__ __launch_bounds__(256) void k_average_cellB(int nc, const Real* __restrict__ b, double* __restrict__ ave) {
+    const size_t q = blockIdx.x * (size_t)256 + threadIdx.x;
+    if (q < 3 * (size_t)nc) ave[q] += (double)b[q];
+}
+
 // ParticleBoundaryCondHybrid::apply (src/particle_boundary_cond_hybrid.h:
 // 112-185): empty exterior cells, remove particles inside the obstacle in
 // innerFlagParticle cells.  Removed particles get x = -1; the next sort drops

@@ -251,3 +251,56 @@ def test_rare_branches_fp64(gpu_mod, oracle_mod, case):
         assert cg[k] == co[k], (k, cg[k], co[k])
     if counter is not None:
         assert co[counter] > 0
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-11), (0, 2e-5)])
+@pytest.mark.parametrize("shape", [(12, 10, 8), (20, 12, 10), (11, 14, 15), "periodic10"])
+def test_fused_field_update_parity(gpu_mod, oracle_mod, prec, tol, shape):
+    """propagateB without E smoothing runs as one LDS-tiled launch
+    (k_field_update); grids with partial field tiles (FP32 tiles 8x8x8, FP64
+    8x4x4), resistivity, an obstacle, IMF walls and Neumann walls, and a
+    periodic box whose partial tiles wrap, against the oracle, every array
+    and the constraint counters."""
+    if shape == "periodic10":
+        base = P.config_periodic(n=10, ppc=8, seed=6)
+        nx = 10
+    else:
+        nx, ny, nz = shape
+        base = P.config_obstacle(nx=nx, ny=ny, nz=nz, ppc=8, R_cells=2.0, seed=6)
+        base.params.imf_cell = (1, 1, 0, 0, 1, 0)
+        base.params.imf_face = (1, 0, 0, 1, 0, 0)
+    rng = np.random.default_rng(nx)
+    base.fields["nodeEta"][:] = rng.uniform(0, 2e3, base.grid.nc)
+    pr = P.perturb(base, seed=3)
+    ctx = make_ctx(gpu_mod, pr, prec)
+    ora = oracle_mod.Oracle(pr)
+    ctx.deposit()
+    ora.deposit()
+    ctx.propagate_B()
+    ora.propagate_B()
+    for name in FIELDS:
+        a, b = ctx.field(name), ora.field(name)
+        if np.linalg.norm(b) == 0:
+            assert np.linalg.norm(a) == 0, name
+            continue
+        assert rel(a, b) < tol, (name, rel(a, b))
+    cg, co = ctx.counters(), ora.counters()
+    for k in co:
+        assert cg[k] == co[k], (k, cg[k], co[k])
+
+
+def test_lean_steps_equal_full_steps(gpu_mod):
+    """rh_step(n) writes the arrays nothing reads in between only in its last
+    step: n steps in one call and n calls of one step give the same bits in
+    every array."""
+    pr = P.config_obstacle(nx=16, ny=16, nz=16, ppc=8, R_cells=3, seed=13, inflow=True)
+    for sp in pr.species:
+        sp.capacity = 3 * pr.particles[0].n
+    a = make_ctx(gpu_mod, pr, gpu_mod.RH_FP32, (4, 4, 4))
+    b = make_ctx(gpu_mod, pr, gpu_mod.RH_FP32, (4, 4, 4))
+    a.step(5)
+    for _ in range(5):
+        b.step(1)
+    for name in FIELDS:
+        assert np.array_equal(a.field(name), b.field(name)), name
+    assert a.counters() == b.counters()
