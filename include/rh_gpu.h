@@ -153,8 +153,12 @@ int rh_apply_boundary_conditions(rh_ctx* ctx, int32_t species);
 int rh_propagate_B(rh_ctx* ctx);
 /* propagate() (src/user.cpp:118-134) without logging, nsteps times: each
  * step advances the timestep by one, then push, migration, deposit, particle
- * boundary conditions, injection and field update.  Synchronises at the end;
- * returns RH_ERR_CAPACITY when a particle store overflowed in these steps. */
+ * boundary conditions, injection and field update.  Every grid array holds
+ * the reference's values when the call returns; the field updates of the
+ * steps before the last write only the arrays a step reads (faceB, cellUe,
+ * cellEp; cellB too with a denoiser attached), the last writes all.
+ * Synchronises at the end; returns RH_ERR_CAPACITY when a particle store
+ * overflowed in these steps. */
 int rh_step(rh_ctx* ctx, int32_t nsteps);
 
 /* ---- injection (src/particle_injector.cpp) ----------------------------- */
@@ -275,9 +279,19 @@ int rh_log_write(rh_ctx* ctx, double t);
  * population, species with one name are combined).  The file carries the
  * mesh "SpatialGrid" (MESH_BBOX, MESH_NODE_CRDS_X/Y/Z, CellID = cell index)
  * and the parameters time, timestep, dt and the box extent, as the
- * reference's reader (tools/rhybridpy.py) expects. */
+ * reference's reader (tools/rhybridpy.py) expects.  With temporal averages
+ * on (rh_set_temporal_averages), "cellBAverage", "n_ave" and "v_ave" write
+ * the averages since the previous file (cellBAverage, n_<pop>_ave,
+ * v_<pop>_ave, src/operator_userdata.cpp:156-232) and restart them. */
 int rh_write_vlsv(rh_ctx* ctx, const char* filename, double t, int64_t timestep, const char* const* variables,
                   int32_t nvariables, const char* const* population_names);
+/* WRITE_GRID_TEMPORAL_AVERAGES (src/Makefile:6, on in the reference's
+ * default build): with enable, every field update adds cellB to a running
+ * FP64 sum (src/hybrid_propagator.cpp:157-167) and every accumulation adds
+ * each species' charge and current (before /dV) to its own
+ * (src/particle_accumulator.cpp:148-151, 287-300); the sums and their count
+ * start at zero.  Off (the default here) frees them. */
+int rh_set_temporal_averages(rh_ctx* ctx, int32_t enable);
 
 /* ---- measurement -------------------------------------------------------- */
 

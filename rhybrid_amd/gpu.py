@@ -38,7 +38,7 @@ EXPORTS = ["rh_create", "rh_destroy", "rh_last_error", "rh_add_species", "rh_upl
            "rh_reset_timing",
            "rh_layout_info", "rh_denoise_moments", "rh_set_denoiser", "rh_set_injector",
            "rh_inject_particles", "rh_set_timestep", "rh_get_timestep", "rh_injected", "rh_log_values",
-           "rh_log_begin", "rh_log_open", "rh_log_write", "rh_write_vlsv"]
+           "rh_log_begin", "rh_log_open", "rh_log_write", "rh_write_vlsv", "rh_set_temporal_averages"]
 
 
 class GridDesc(C.Structure):
@@ -133,6 +133,7 @@ def load_library(path: str = LIB_PATH):
     L.rh_log_open.argtypes = [vp, C.c_char_p, C.POINTER(C.c_char_p), i32, i32, C.c_double]
     L.rh_log_write.argtypes = [vp, C.c_double]
     L.rh_write_vlsv.argtypes = [vp, C.c_char_p, C.c_double, i64, C.POINTER(C.c_char_p), i32, C.POINTER(C.c_char_p)]
+    L.rh_set_temporal_averages.argtypes = [vp, i32]
     for name in EXPORTS:
         if name not in ("rh_last_error",):
             getattr(L, name).restype = C.c_int
@@ -335,6 +336,12 @@ class Context:
         self._check(load_library().rh_log_write(self._h, float(t)))
 
     # -- output ------------------------------------------------------------------
+    def set_temporal_averages(self, on: bool) -> None:
+        """WRITE_GRID_TEMPORAL_AVERAGES: running sums of cellB and of each
+        species' charge and current, written (and restarted) by write_vlsv's
+        "cellBAverage", "n_ave" and "v_ave"."""
+        self._check(load_library().rh_set_temporal_averages(self._h, int(on)))
+
     def write_vlsv(self, filename: str, t: float, timestep: int, variables, populations=None) -> None:
         """VLSV file of the selected cell variables (UserDataOP::writeData)."""
         pops = populations or [sp.name for sp in self.problem.species]

@@ -4,33 +4,13 @@ through the C ABI, read back the way the reference's reader reads it
 footer whose elements give each array's offset, type and shape).  Values are
 checked against the context's own arrays and against numpy on the
 downloaded particles."""
-import struct
-import xml.etree.ElementTree as ET
-
 import numpy as np
 import pytest
 
 from rhybrid_amd import problem as P
+from vlsv_read import read_vlsv
 
 pytestmark = pytest.mark.gpu
-
-DT = {("float", 8): "<f8", ("float", 4): "<f4", ("uint", 8): "<u8", ("uint", 4): "<u4", ("uint", 1): "<u1",
-      ("int", 8): "<i8", ("int", 4): "<i4"}
-
-
-def read_vlsv(path):
-    raw = open(path, "rb").read()
-    (off,) = struct.unpack("=Q", raw[8:16])
-    root = ET.fromstring(raw[off:])
-    out = {}
-    for el in root:
-        a = el.attrib
-        n, v = int(a["arraysize"]), int(a["vectorsize"])
-        data = np.frombuffer(raw, dtype=DT[(a["datatype"], int(a["datasize"]))], count=n * v, offset=int(el.text))
-        if v > 1:
-            data = data.reshape(n, v)
-        out[(el.tag, a.get("name", ""))] = (data, a)
-    return out
 
 
 def test_vlsv_file(gpu_mod, tmp_path):
@@ -105,3 +85,48 @@ def test_vlsv_population_groups_and_errors(gpu_mod, tmp_path):
     with pytest.raises(gpu_mod.RhError) as e:
         ctx.write_vlsv(path, 0.0, 0, ["faceJ"])
     assert e.value.code == gpu_mod.RH_ERR_ARG
+
+
+def test_vlsv_temporal_averages(gpu_mod, tmp_path):
+    """WRITE_GRID_TEMPORAL_AVERAGES: cellBAverage is the mean of cellB over
+    the field updates since the previous file, n_<pop>_ave the mean charge
+    density of the population's species over q dV and v_<pop>_ave their
+    summed current over their summed charge (src/operator_userdata.cpp:
+    156-232); writing restarts the sums.  Checked against the per-step
+    arrays in FP64, two species combined into one population, on the cells
+    the rho floor and the Neumann copies leave alone."""
+    pr = P.config_obstacle(nx=16, ny=12, nz=12, ppc=8, R_cells=3, seed=24, inflow=True)
+    for sp in pr.species:
+        sp.capacity = 3 * pr.particles[0].n
+    print("case temporal averages", flush=True)
+    ctx = gpu_mod.Context(pr, precision=gpu_mod.RH_FP64)
+    with pytest.raises(gpu_mod.RhError):
+        ctx.write_vlsv(str(tmp_path / "off.vlsv"), 0.0, 0, ["cellBAverage"], populations=["ions", "ions"])
+    ctx.set_temporal_averages(True)
+    inside = pr.grid.all_neighbours()
+    q0, dV = pr.species[0].q, pr.grid.dV
+    for window, steps in enumerate((4, 3)):
+        B, rho, J = [], [], []
+        for _ in range(steps):
+            ctx.step(1)
+            B.append(ctx.field("cellB").reshape(-1, 3))
+            rho.append(ctx.field("cellRhoQi"))
+            J.append(ctx.field("cellJi").reshape(-1, 3))
+        path = str(tmp_path / ("ave%d.vlsv" % window))
+        ctx.write_vlsv(path, 0.0, steps, ["cellBAverage", "n_ave", "v_ave"], populations=["ions", "ions"])
+        f = read_vlsv(path)
+        np.testing.assert_allclose(f[("VARIABLE", "cellBAverage")][0], np.mean(B, axis=0), rtol=1e-12, atol=1e-25)
+        ok = inside & (np.min(rho, axis=0) > 1.0001 * pr.params.min_rho)
+        assert ok.sum() > 0.5 * inside.sum()
+        n_ave = f[("VARIABLE", "n_ions_ave")][0]
+        np.testing.assert_allclose(n_ave[ok], (np.mean(rho, axis=0) / q0)[ok], rtol=1e-11)
+        v_ave = f[("VARIABLE", "v_ions_ave")][0]
+        want = np.sum(J, axis=0) / np.sum(rho, axis=0)[:, None]
+        np.testing.assert_allclose(v_ave[ok], want[ok], rtol=1e-9, atol=1e-6)
+    # an immediate second file: the sums restarted at the previous one
+    path = str(tmp_path / "empty.vlsv")
+    ctx.write_vlsv(path, 0.0, 0, ["cellBAverage", "n_ave"], populations=["ions", "ions"])
+    f = read_vlsv(path)
+    assert not f[("VARIABLE", "cellBAverage")][0].any()
+    assert not f[("VARIABLE", "n_ions_ave")][0].any()
+    ctx.close()
