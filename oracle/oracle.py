@@ -14,7 +14,9 @@ from typing import Optional
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "liboracle.so")
+# RH_ORACLE_SANITIZED=1: the ASan/UBSan build (the caller preloads the ASan runtime)
+_SAN = os.environ.get("RH_ORACLE_SANITIZED") == "1"
+_LIB = os.path.join(_HERE, "liboracle_san.so" if _SAN else "liboracle.so")
 
 N_FIELDS = 14
 N_FLAGS = 4
@@ -63,7 +65,7 @@ class OrcSpecies(C.Structure):
 def build(force: bool = False) -> str:
     src = os.path.join(_HERE, "rh_oracle.c")
     if force or not os.path.exists(_LIB) or os.path.getmtime(_LIB) < os.path.getmtime(src):
-        subprocess.check_call(["make", "-s", "-B" if force else "-s", "-C", _HERE, "liboracle.so"])
+        subprocess.check_call(["make", "-s", "-B" if force else "-s", "-C", _HERE, os.path.basename(_LIB)])
     return _LIB
 
 
