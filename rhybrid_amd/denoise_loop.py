@@ -118,3 +118,95 @@ def main():
 
 if __name__ == "__main__":
     main()
+
+
+def _checkpointed_run(pr, precision, steps, every, dn=None, bound=0.5, log=None, name=""):
+    """Run `steps` full steps of a device-filled problem, `every` steps a
+    checkpoint of n and J (float32, host); the time per step excludes the
+    checkpoints' downloads."""
+    from .denoise_train import filled_context
+    ctx = filled_context(pr, precision=precision)
+    if dn is not None:
+        ctx.set_denoiser(dn, bound)
+    ctx.synchronize()
+    cps, t_steps, done = [], 0.0, 0
+    while done < steps:
+        k = min(every, steps - done)
+        t0 = time.perf_counter()
+        ctx.step(k)
+        ctx.synchronize()
+        t_steps += time.perf_counter() - t0
+        done += k
+        cps.append((done, ctx.field("cellRhoQi").astype(np.float32),
+                    ctx.field("cellJi").reshape(-1, 3).astype(np.float32)))
+        if log is not None:
+            log("%s: step %d, %.3f ms/step" % (name, done, t_steps / done * 1e3))
+    out = {"ms_per_step": t_steps / steps * 1e3, "checkpoints": cps,
+           "particles": [ctx.num_particles(s) for s in range(len(pr.species))],
+           "fallbacks": ctx.counters()["denoise_fallback"]}
+    if dn is not None:
+        ctx.set_denoiser(None)
+    ctx.close()
+    return out
+
+
+def compare_uneven(dn, nx=256, ny=192, nz=192, R_cells=12.0, H_cells=5.0, steps=2000, every=250,
+                   ppcs=(8, 64, 256), bound=0.5, seed=23, log=None):
+    """The denoiser in the loop on the uneven-load geometry (config_exosphere:
+    H+ solar wind at ppc, O+ photoions in proportion): plain FP32 runs at each
+    of `ppcs` and a denoised FP32 run at the lowest, from the same seeded
+    distribution (device fill), against the highest-ppc run as the judge;
+    n and J errors over the plasma cells every `every` steps, time per step,
+    fallback fraction."""
+    make = lambda ppc: P.config_exosphere(nx=nx, ny=ny, nz=nz, ppc=ppc, R_cells=R_cells, H_cells=H_cells,
+                                          seed=seed, o_per_ppc=True)
+    runs = {}
+    judge_ppc, low = max(ppcs), min(ppcs)
+    for ppc in sorted(ppcs, reverse=True):
+        runs["fp32_%d" % ppc] = _checkpointed_run(make(ppc), RH_FP32, steps, every, log=log, name="fp32_%d" % ppc)
+    runs["fp32_%d_denoised" % low] = _checkpointed_run(make(low), RH_FP32, steps, every, dn=dn, bound=bound,
+                                                       log=log, name="denoised")
+    g = make(1)
+    plasma = g.grid.all_neighbours() & (g.flags["innerFlagField"] == 0)
+    judge = runs["fp32_%d" % judge_ppc]["checkpoints"]
+    out = {"geometry": "%dx%dx%d cells, exospheric O+ around an absorbing sphere of %g cells, H+ solar wind; "
+                       "%d steps" % (nx, ny, nz, R_cells, steps),
+           "judge": "plain FP32 at %d ppc" % judge_ppc, "fallback_bound": bound, "runs": {}}
+    for name, r in runs.items():
+        rows = []
+        for (st, rho, J), (_, rho_j, J_j) in zip(r["checkpoints"], judge):
+            en = np.linalg.norm((rho - rho_j)[plasma]) / np.linalg.norm(rho_j[plasma])
+            eJ = np.linalg.norm((J - J_j)[plasma]) / np.linalg.norm(J_j[plasma])
+            rows.append({"step": st, "n_error": float(en), "J_error": float(eJ)})
+        entry = {"ms_per_step": round(r["ms_per_step"], 4), "errors": rows, "particles": r["particles"]}
+        if "denoised" in name:
+            entry["fallback_fraction"] = r["fallbacks"] / float(steps * g.grid.nc)
+        out["runs"][name] = entry
+    d = out["runs"]["fp32_%d_denoised" % low]
+    p64 = out["runs"].get("fp32_64")
+    if p64 is not None:
+        last_d, last_p = d["errors"][-1], p64["errors"][-1]
+        out["denoised_vs_plain_64"] = {
+            "time_ratio": p64["ms_per_step"] / d["ms_per_step"],
+            "n_error_ratio_last": last_d["n_error"] / last_p["n_error"],
+            "J_error_ratio_last": last_d["J_error"] / last_p["J_error"],
+            "n_error_ratio_mean": float(np.mean([a["n_error"] / b["n_error"] for a, b in zip(d["errors"], p64["errors"])])),
+            "J_error_ratio_mean": float(np.mean([a["J_error"] / b["J_error"] for a, b in zip(d["errors"], p64["errors"])]))}
+    return out
+
+
+def judge_precision(nx=64, ny=48, nz=48, R_cells=6.0, H_cells=2.5, ppc=256, steps=100, seed=23):
+    """The judge's precision: FP32 against FP64 at the judge's ppc on a
+    sub-box of the uneven-load geometry, from the same device fill."""
+    from .denoise_train import filled_context
+    pr = P.config_exosphere(nx=nx, ny=ny, nz=nz, ppc=ppc, R_cells=R_cells, H_cells=H_cells, seed=seed,
+                            o_per_ppc=True)
+    res = {}
+    for prec in (RH_FP32, RH_FP64):
+        ctx = filled_context(pr, precision=prec)
+        ctx.step(steps)
+        res[prec] = (ctx.field("cellRhoQi"), ctx.field("cellJi"), ctx.field("faceB"), ctx.field("nodeE"))
+        ctx.close()
+    names = ("n", "J", "B", "E")
+    return {"box": "%dx%dx%d, %d ppc, %d steps" % (nx, ny, nz, ppc, steps),
+            **{k: float(np.linalg.norm(a - b) / np.linalg.norm(b)) for k, a, b in zip(names, res[RH_FP32], res[RH_FP64])}}

@@ -309,6 +309,39 @@ def config_obstacle(nx: int = 128, ny: int = 96, nz: int = 96, ppc: int = 32,
                    injectors=injectors)
 
 
+def interior_box(grid: Grid) -> Tuple[float, float, float, float, float, float]:
+    """The extent holding the centres of the cells with all 26 neighbours (a
+    periodic axis: all of it)."""
+    out = []
+    for d, (n, o) in enumerate(((grid.nx, grid.x0), (grid.ny, grid.y0), (grid.nz, grid.z0))):
+        if grid.periodic[d]:
+            out += [o - grid.dx, o + (n + 1) * grid.dx]
+        else:
+            out += [o + grid.dx, o + (n - 1) * grid.dx]
+    return tuple(out)
+
+
+def config_obstacle_filled(nx: int = 128, ny: int = 96, nz: int = 96, ppc: int = 32, seed: int = 6461674,
+                           R_cells: float = 10.0, **kw) -> Problem:
+    """config_obstacle with inflow whose uniform preload is done on the device
+    (InjectorUniform at timestep 1, meta["fill"], one injector per species,
+    seeded by `seed`) instead of uploaded: for runs at hundreds of particles
+    per cell.  The particles inside the obstacle are removed by the first
+    step's boundary conditions."""
+    base = config_obstacle(nx=nx, ny=ny, nz=nz, ppc=1, seed=seed, R_cells=R_cells, inflow=True, **kw)
+    g, U, T = base.grid, base.meta["U"], base.meta["T"]
+    n_sw, he = base.meta["density"], kw.get("he_fraction", 0.04)
+    dens = (n_sw * (1 - he), n_sw * he)
+    species = [dataclasses.replace(sp, w=d * g.dV / ppc, capacity=int(1.25 * ppc * g.nc) + 100000)
+               for sp, d in zip(base.species, dens)]
+    fill = [uniform_injector(U, d, T, ppc, seed=seed + 11 + q, box=interior_box(g)) for q, d in enumerate(dens)]
+    injectors = [solar_wind_injector(U, d, T, ppc, seed=seed + 1 + q) for q, d in enumerate(dens)]
+    empty = [Particles(np.zeros(0, dtype=np.int64), np.zeros((0, 7))) for _ in species]
+    meta = dict(base.meta, ppc=ppc, fill=fill)
+    return Problem("obstacle%dx%dx%d_ppc%d_filled" % (nx, ny, nz, ppc), g, base.params, species, base.fields,
+                   base.flags, empty, steps=base.steps, meta=meta, injectors=injectors)
+
+
 def uniform_injector(velocity, density: float, T: float, ppc: float, seed: int, box=ALL_SPACE) -> Injector:
     """InjectorUniform (src/particle_injector.cpp:159-345): at timestep 1,
     probRound(ppc) particles in every cell whose centre is in the box; the
@@ -362,7 +395,7 @@ def exosphere_injector(grid: Grid, neutrals: np.ndarray, T: float, macro_per_dt:
 def config_exosphere(nx: int = 256, ny: int = 192, nz: int = 192, ppc: int = 24, dx: float = 200e3,
                      dt: float = 0.04, R_cells: float = 12.0, H_cells: float = 5.0, n0: float = 1e12,
                      ionization_rate: float = 1e-6, o_relative: float = 4.0, n_sw: float = 3e6,
-                     seed: int = 6461674, steps: int = 500, sub: int = 2) -> Problem:
+                     seed: int = 6461674, steps: int = 500, sub: int = 2, o_per_ppc: bool = False) -> Problem:
     """Uneven-load box: solar-wind H+ (ppc per cell, 450 km/s, T=1e5 K, IMF
     on the +x wall) around an absorbing sphere of R_cells cells with an
     exponential oxygen corona (n0 at r0 = R, scale height H_cells cells,
@@ -371,7 +404,9 @@ def config_exosphere(nx: int = 256, ny: int = 192, nz: int = 192, ppc: int = 24,
     (InjectorExosphere's macroparticles_per_cell_relative x the upstream
     ratio, src/particle_injector.cpp:1680-1688).  Particles start empty: H+
     is filled on the device (meta["fill"], InjectorUniform at timestep 1),
-    then enters through the +x wall; O+ enters by photoionisation."""
+    then enters through the +x wall; O+ enters by photoionisation (with
+    o_per_ppc, in proportion to the H+ ppc, so that runs at different ppc
+    carry the same relative noise in both species)."""
     grid = Grid(nx, ny, nz, (0, 0, 0), dx, -0.5 * nx * dx, -0.5 * ny * dx, -0.5 * nz * dx)
     T, Te = 1e5, 1e5
     U = (-450e3, 0.0, 0.0)
@@ -381,6 +416,8 @@ def config_exosphere(nx: int = 256, ny: int = 192, nz: int = 192, ppc: int = 24,
     spH = Species("H+", QE, MP, n_sw * grid.dV / ppc, R2_obstacle=R * R, capacity=int(1.3 * ppc * grid.nc))
     # upstream ratio: solar-wind macroparticles per step over ppc (:698-701)
     macro_per_dt = o_relative * (abs(U[0]) * dt / dx) * (ny - 2) * (nz - 2)
+    if o_per_ppc:  # O+ macroparticles in proportion to the H+ ppc (24 ppc: unchanged)
+        macro_per_dt *= ppc / 24.0
     neut = exosphere_neutrals(grid, n0, R, H, R_exobase=R, sub=sub)
     total_rate = ionization_rate * neut.sum()
     spO = Species("O+", QE, 16 * MP, total_rate * dt / macro_per_dt, R2_obstacle=R * R,
