@@ -38,15 +38,17 @@ def workload(scale: int = 1, seed: int = 6461674):
                              R_cells=10.0 / scale, seed=seed, inflow=True)
 
 
-def kernel_bytes(name: str, ctx, np_total: float, nspecies: int) -> float:
+def kernel_bytes(name: str, ctx, np_total: float, nspecies: int, npp: float = None) -> float:
     """Algorithmic HBM bytes of one launch of a kernel (FP32 store: 6 floats
-    per particle), for the mean species population."""
+    per particle), for the mean species population (or `npp` particles of
+    the launch's species)."""
     lay = ctx.layout()
     nc, ntiles = lay["ncells"], lay["ntiles"]
     next_ = (lay["tx"] + 2) * (lay["ty"] + 2) * (lay["tz"] + 2)
     extk = next_ * 8
     nkeys = ntiles * lay["tx"] * lay["ty"] * lay["tz"] * 8
-    npp = np_total / nspecies  # particles per species launch (mean)
+    if npp is None:
+        npp = np_total / nspecies  # particles per species launch (mean)
     nh = (lay["tx"] + 1) * (lay["ty"] + 1) * (lay["tz"] + 1)
     tc = lay["tx"] * lay["ty"] * lay["tz"]
     fblock = 4.0 * (3 * nh + 7 * tc) * ntiles   # per-tile field blocks (FP32 + neighbourhood masks)
@@ -422,17 +424,39 @@ def exosphere_bench(steps: int = 10, warmup: int = 2, prefill: int = 600, tune_s
     ctx.synchronize()
     wall = time.perf_counter() - t0
     n1 = [ctx.num_particles(s) for s in range(2)]
-    # kernel breakdown from further steps with every launch timed
+    # kernel breakdown per species from further steps with every launch timed
+    nb = min(steps, 5)
     ctx.reset_timing()
     ctx.set_timing(True)
-    ctx.step(min(steps, 5))
+    ctx.step(nb)
     ctx.synchronize()
     ctx.set_timing(False)
     kt = {}
-    for nm in ("push", "move", "deposit", "sort_slots", "inject_count", "inject_fill", "inject_slots", "boundary"):
+    per_species = {}
+    for s, sp in enumerate(("H+", "O+")):
+        npp = 0.5 * (n0[s] + n1[s])
+        row = {}
+        for nm in ("push", "sort_slots", "move", "deposit", "boundary", "inject_count", "inject_fill",
+                   "inject_slots"):
+            ms, n = ctx.kernel_time("%s:%d" % (nm, s))
+            if n:
+                b = kernel_bytes(nm, ctx, 0.0, 1, npp=npp)
+                avg = ms / 1e3 / n
+                row[nm] = {"us_avg": round(avg * 1e6, 2), "GBps": round(b / avg / 1e9, 1) if b > 0 else None,
+                           "ms_per_step": round(ms / nb, 4)}
+        per_species[sp] = {"particles": npp, "kernels": row}
+    for nm in ("fold", "field_update", "push_fields"):
         ms, n = ctx.kernel_time(nm)
         if n:
-            kt[nm] = round(ms / n * 1e3, 2)
+            kt[nm] = {"us_avg": round(ms / n * 1e3, 2), "ms_per_step": round(ms / nb, 4)}
+    # the dominant kernel (largest time per step over both species) against HBM
+    dom_sp, dom_nm = max(((sp, nm) for sp in per_species for nm in per_species[sp]["kernels"]),
+                         key=lambda k: per_species[k[0]]["kernels"][k[1]]["ms_per_step"])
+    dk = per_species[dom_sp]["kernels"][dom_nm]
+    dom_bytes = kernel_bytes(dom_nm, ctx, 0.0, 1, npp=per_species[dom_sp]["particles"])
+    roof = {"bound": "hbm", "kernel": "%s (%s)" % (dom_nm, dom_sp), "achieved": dk["GBps"],
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dk["GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+            "algorithmic_bytes": dom_bytes}
     ctx.close()
     np_mean = 0.5 * (sum(n0) + sum(n1))
     occ = nO_cells[nO_cells > 0]
@@ -445,7 +469,9 @@ def exosphere_bench(steps: int = 10, warmup: int = 2, prefill: int = 600, tune_s
             "O+_ppc": {"max": int(occ.max()) if occ.size else 0, "mean_occupied": float(occ.mean()) if occ.size else 0.0,
                        "cells_occupied": int(occ.size), "prefill_steps": prefill},
             "O+_production_per_s": pr.meta["O_total_rate"],
-            "kernel_us_avg": kt, "seconds_setup": round(t_gen, 1), "seconds_fill": round(t_fill, 2)}
+            "roofline": roof, "per_species": per_species, "grid_kernels": kt,
+            "kernels_from": "%d further steps with every launch timed" % nb,
+            "seconds_setup": round(t_gen, 1), "seconds_fill": round(t_fill, 2)}
 
 
 def calibration_copy(gib: float) -> None:

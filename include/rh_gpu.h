@@ -298,7 +298,10 @@ int rh_set_temporal_averages(rh_ctx* ctx, int32_t enable);
 /* Per-kernel timing with HIP events on the context's stream (the profile::
  * regions of src/hybrid_propagator.cpp:33-38).  When enabled, every launch is
  * bracketed by events; rh_kernel_times returns, for the kernel name, the
- * accumulated milliseconds and the number of launches since the last reset. */
+ * accumulated milliseconds and the number of launches since the last reset.
+ * A launch made for one species (push, sort, move, deposit, boundary
+ * conditions, injection) also counts under "<name>:<species id>", e.g.
+ * "move:1". */
 int rh_set_timing(rh_ctx* ctx, int32_t enable);
 /* Restrict the timing to the launches of one kernel name (NULL or "": every
  * launch).  Each timed launch adds two event records to the stream, which

@@ -77,7 +77,11 @@ struct DevBuf {
 };
 
 struct Timer {
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    struct Pending {
+        hipEvent_t e0, e1;
+        int species;  // the species the launch worked on, -1 for none
+    };
+    std::vector<Pending> pending;
     double ms = 0;
     int64_t launches = 0;
 };
@@ -341,7 +345,7 @@ public:
         if (stream) (void)hipStreamSynchronize(stream);
         if (dn_attached) (void)rh_dn_set_stream(dn_attached, nullptr);  // before the stream goes
         for (auto& t : timers)
-            for (auto& pe : t.second.pending) { (void)hipEventDestroy(pe.first); (void)hipEventDestroy(pe.second); }
+            for (auto& pe : t.second.pending) { (void)hipEventDestroy(pe.e0); (void)hipEventDestroy(pe.e1); }
         for (auto e : event_pool) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -432,19 +436,41 @@ public:
         return e;
     }
 
+    // A launch made on behalf of one species is also counted under
+    // "<name>:<species>" (e.g. "move:1"), so that per-species times can be
+    // read where species differ in size.
     void resolve_timers() {
         RH_HIP(hipStreamSynchronize(stream));
+        std::map<std::string, Timer> per_species;
         for (auto& kv : timers) {
             for (auto& pe : kv.second.pending) {
                 float ms = 0;
-                RH_HIP(hipEventElapsedTime(&ms, pe.first, pe.second));
+                RH_HIP(hipEventElapsedTime(&ms, pe.e0, pe.e1));
                 kv.second.ms += ms;
-                event_pool.push_back(pe.first);
-                event_pool.push_back(pe.second);
+                if (pe.species >= 0) {
+                    Timer& t = per_species[kv.first + ":" + std::to_string(pe.species)];
+                    t.ms += ms;
+                    t.launches++;
+                }
+                event_pool.push_back(pe.e0);
+                event_pool.push_back(pe.e1);
             }
             kv.second.pending.clear();
         }
+        for (auto& kv : per_species) {
+            Timer& t = timers[kv.first];
+            t.ms += kv.second.ms;
+            t.launches += kv.second.launches;
+        }
     }
+
+    int timing_species = -1;  // the species the operator being run works on
+    struct SpeciesScope {
+        int& slot;
+        int prev;
+        SpeciesScope(int& s, int species) : slot(s), prev(s) { slot = species; }
+        ~SpeciesScope() { slot = prev; }
+    };
 
     template <typename K, typename... Args>
     void launch(const char* name, K kernel, dim3 grid, dim3 block, size_t shmem, Args... args) {
@@ -461,7 +487,7 @@ public:
         if (timed) {
             RH_HIP(hipEventRecord(e1, stream));
             Timer& t = timers[name];
-            t.pending.emplace_back(e0, e1);
+            t.pending.push_back({e0, e1, timing_species});
             t.launches++;
             if (t.pending.size() > 4096) resolve_timers();
         }
@@ -475,7 +501,8 @@ public:
     }
 
     static unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
-    dim3 tile_grid() const { return dim3(g.ntx, g.nty, g.ntz); }
+    // per-tile launches: one workgroup per tile in XCD-aware order (xcd_tile)
+    dim3 tile_grid() const { return dim3(N_XCD * cdiv((size_t)g.ntiles, N_XCD)); }
     dim3 cell_grid() const { return dim3(cdiv(g.nx, 32), cdiv(g.ny, 8), g.nz); }
     static dim3 cell_block() { return dim3(32, 8, 1); }
 
@@ -656,6 +683,7 @@ public:
     // ---- the operators ------------------------------------------------------
 
     void propagate(int s) override {
+        SpeciesScope species_scope(timing_species, s);
         Species& S_ = S(s);
         if (S_.pushed) throw StateError("propagate called twice without wait_particle_sends");
         PushParams pp;
@@ -708,6 +736,7 @@ public:
     }
 
     void wait_particle_sends(int s) override {
+        SpeciesScope species_scope(timing_species, s);
         Species& S_ = S(s);
         if (!S_.pushed) throw StateError("wait_particle_sends without a preceding propagate");
         finish_sort(S_, "move");
@@ -732,6 +761,7 @@ public:
     }
 
     void accumulate(int s) override {
+        SpeciesScope species_scope(timing_species, s);
         Species& S_ = S(s);
         if (!S_.d.accumulate) return;
         if (S_.pushed) throw StateError("accumulate between propagate and wait_particle_sends");
@@ -858,6 +888,7 @@ public:
     }
 
     void apply_bcs(int s) override {
+        SpeciesScope species_scope(timing_species, s);
         Species& S_ = S(s);
         if (S_.pushed) throw StateError("boundary conditions between propagate and wait_particle_sends");
         flush_injected(S_);
@@ -1181,6 +1212,7 @@ public:
     // ParticleInjectorBase::inject for the current timestep (nothing at
     // timestep <= 0, src/particle_injector.cpp:544): count, scan, fill.
     void inject(int s) override {
+        SpeciesScope species_scope(timing_species, s);
         Species& S_ = S(s);
         if (S_.pushed) throw StateError("inject between propagate and wait_particle_sends");
         if (S_.inj.kind == INJ_NONE || S_.inj_ncell == 0 || tstep <= 0) return;

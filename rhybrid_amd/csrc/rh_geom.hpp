@@ -100,6 +100,25 @@ RH_HD void local_ijk(const Geom& g, int lc, int& li, int& lj, int& lk) {
     lk = lc >> (g.ltx + g.lty);
 }
 
+// XCD-aware tile order.  The hardware deals the workgroups of a launch to the
+// eight XCDs (each with its own L2) in turn, workgroup b to XCD b % 8.  Per-
+// tile launches use a 1D grid of 8 * ceil(ntiles / 8) workgroups and give XCD
+// x the contiguous tile range [x * chunk, (x + 1) * chunk) in order, so that
+// tiles that are neighbours in memory -- whose particle ranges share cache
+// lines at their ends, written by both in the move -- run on one XCD, close
+// in time, instead of on two.  A result >= ntiles is no tile.
+constexpr int N_XCD = 8;
+RH_HD int xcd_tile(const Geom& g, int b) {
+    const int chunk = (g.ntiles + N_XCD - 1) / N_XCD;
+    return (b % N_XCD) * chunk + b / N_XCD;
+}
+
+RH_HD void tile_coords(const Geom& g, int T, int& bx, int& by, int& bz) {
+    bx = T % g.ntx;
+    by = (T / g.ntx) % g.nty;
+    bz = T / (g.ntx * g.nty);
+}
+
 RH_HD void tile_origin(const Geom& g, int T, int& ox, int& oy, int& oz) {
     ox = (T % g.ntx) * g.tx;
     oy = ((T / g.ntx) % g.nty) * g.ty;

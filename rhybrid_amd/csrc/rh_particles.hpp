@@ -206,9 +206,12 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
     Real* sue = fl + 3 * nh;   // cellUe, cellEp of the tile's cells
     Real* sep = sue + 3 * g.tc;
     uint32_t* nmask = reinterpret_cast<uint32_t*>(sep + 3 * g.tc);  // neighbourhood of each tile cell
-    const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
+    const int T = xcd_tile(g, (int)blockIdx.x);
+    if (T >= g.ntiles) return;
+    int bx, by, bz;
+    tile_coords(g, T, bx, by, bz);
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    const int ox = blockIdx.x << g.ltx, oy = blockIdx.y << g.lty, oz = blockIdx.z << g.ltz;
+    const int ox = bx << g.ltx, oy = by << g.lty, oz = bz << g.ltz;
     // the tile's particle range first: its loads overlap the set-up below
     const uint32_t p0 = sub[(size_t)T * NS], p1 = sub[(size_t)(T + 1) * NS];
     // the tile's injected particles (past the store's end) follow its own
@@ -250,9 +253,9 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
         // running counter then grows by the slot's particles in order
         for (int ec = tid; ec < g.next; ec += PM_THREADS) {
             const int lx = ec % g.ex - 1, ly = (ec / g.ex) % g.ey - 1, lz = ec / (g.ex * g.ey) - 1;
-            int tX = (int)blockIdx.x + (lx < 0 ? -1 : (lx >= g.tx ? 1 : 0));
-            int tY = (int)blockIdx.y + (ly < 0 ? -1 : (ly >= g.ty ? 1 : 0));
-            int tZ = (int)blockIdx.z + (lz < 0 ? -1 : (lz >= g.tz ? 1 : 0));
+            int tX = bx + (lx < 0 ? -1 : (lx >= g.tx ? 1 : 0));
+            int tY = by + (ly < 0 ? -1 : (ly >= g.ty ? 1 : 0));
+            int tZ = bz + (lz < 0 ? -1 : (lz >= g.tz ? 1 : 0));
             // (outside a non-periodic grid the slot is never used; the wrap
             // keeps the index in range)
             tX += tX < 0 ? g.ntx : (tX >= g.ntx ? -g.ntx : 0);
@@ -789,7 +792,8 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
     const int QS = 8 * OB;            // one (quantity, corner) plane
     Real* S = reinterpret_cast<Real*>(smem_raw);
     Real* R = S + 16 * QS;  // first-half results, next x 2
-    const int T = blockIdx.x + g.ntx * (blockIdx.y + g.nty * blockIdx.z);
+    const int T = xcd_tile(g, (int)blockIdx.x);
+    if (T >= g.ntiles) return;
     const uint32_t t0 = sub[(size_t)T * NS], t1 = sub[(size_t)(T + 1) * NS];
     if (t1 == t0) {
         // an empty tile: zero ghost partials (the fold reads every tile's)

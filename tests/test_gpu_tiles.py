@@ -14,7 +14,16 @@ crowded-tile paths), the 11-bit ranking over 1,728 slots and the LDS sizes of
     identical particle counts;
   * a reduced uneven-load box (device uniform fill of H+, O+ photoions
     accumulated near an absorbing sphere, so that many tiles hold no O+ and a
-    few are crowded) stepped against the oracle at every autotune shape."""
+    few are crowded) stepped against the oracle at every autotune shape: FP64
+    to 1e-9 with identical particle counts; FP32 within 1e-5 plus the deposit
+    of two particles.  A particle that ends a step within FP32 rounding
+    (~1e-7 dx) of a face of the exterior layer or of the obstacle can cross it
+    a step earlier or later in FP32 than in FP64, and the reference deletes a
+    particle after depositing it in the exterior cell: for one step its
+    deposit is in one side's moments and not in the other's, whatever the
+    final counts (seen here at the +z wall).  One particle of weight w and
+    charge q moves rho by at most q w / dV in L2 (its eight weights have L2
+    norm <= 1) and J by that times its speed."""
 import numpy as np
 import pytest
 
@@ -138,10 +147,27 @@ def test_uneven_load_box_vs_oracle(gpu_mod, oracle_mod, tile):
     steps = 10
     fo, no = oracle_after(oracle_mod, "exosphere64x48x48", pr, steps, exo_prepare(pr))
     assert no[1] > 50000  # the O+ cloud is there
+    f64, n64 = run_ctx(gpu_mod, pr, gpu_mod.RH_FP64, tile, steps, exo_prepare(pr))
     fg, ng = run_ctx(gpu_mod, pr, gpu_mod.RH_FP32, tile, steps, exo_prepare(pr))
+    print("exosphere tile %s: particles oracle %s fp64 %s fp32 %s" % (tile, no, n64, ng), flush=True)
+    r64 = {name: rel(f64[name], fo[name]) for name in fo}
+    r32 = {name: rel(fg[name], fo[name]) for name in fo}
+    # two particles whose deletion falls in another step, plus any count
+    # difference: per particle q w / dV (rho), times the largest speed (J)
+    dV = pr.grid.dx ** 3
+    vmax = np.sqrt(pr.params.max_ion_speed2)  # the ion speed clamp (and a bound for injected speeds here)
+    qw = max(abs(sp.q * sp.w) for sp in pr.species)
+    drho = (2 + sum(abs(ng[s] - no[s]) for s in range(2))) * qw / dV
+    allow = {"cellRhoQi": 1e-5 + drho / np.linalg.norm(fo["cellRhoQi"]),
+             "cellJi": 1e-5 + drho * vmax / np.linalg.norm(fo["cellJi"]),
+             "faceB": 1e-5, "nodeE": None}
+    for name in fo:
+        print("exosphere tile %s: %s rel L2 fp64 %.3e fp32 %.3e (allowed %s)"
+              % (tile, name, r64[name], r32[name], allow[name]), flush=True)
+    for name in fo:
+        assert r64[name] < 1e-9, ("fp64", name, r64[name])
+    assert n64 == no
     for name in ("cellRhoQi", "cellJi", "faceB"):
-        r = rel(fg[name], fo[name])
-        print("exosphere tile %s: %s rel L2 = %.3e" % (tile, name, r), flush=True)
-        assert r < 1e-5, (name, r)
+        assert r32[name] < allow[name], ("fp32", name, r32[name], allow[name])
     for s in range(2):
         assert abs(ng[s] - no[s]) <= 2, (s, ng[s], no[s])
