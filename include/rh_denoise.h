@@ -51,6 +51,21 @@ int rh_dn_set_scales(rh_denoiser* d, const float scale[7]);
 /* y = denoise(x) on device volumes */
 int rh_dn_forward(rh_denoiser* d, const float* x, float* y, int P, int X, int Y, int Z);
 
+/* rh_dn_forward without allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The same kernels
+ * and the same result as rh_dn_forward, but the packed input and the
+ * activations live in one static device array of the library (36 MiB on
+ * each GPU it runs on, held until the process ends) instead of the
+ * denoiser's buffers, so P*X*Y*Z may exceed the capacity given at creation
+ * up to RH_DN_STATIC_MAX_VOXELS (RH_DN_ERR_CAPACITY above it).  The call
+ * makes no hipMalloc, hipFree or event: it records no kernel times.  Every
+ * call on one GPU shares that GPU's array, so calls that overlap on one GPU
+ * (two denoisers on different streams) race; calls ordered on one stream do
+ * not.  Nothing in the array needs clearing between calls. */
+enum { RH_DN_STATIC_MAX_VOXELS = 262144 };
+int rh_dn_forward_static(rh_denoiser* d, const float* x, float* y, int P, int X, int Y, int Z);
+
 /* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the mean squared error
  * between the normalised output and target/scale over the 4 output channels;
  * loss (before the step) is written to *loss_host. */

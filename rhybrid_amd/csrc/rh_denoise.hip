@@ -33,6 +33,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -478,6 +479,38 @@ unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
 }  // namespace
 
+// Scratch of rh_dn_forward_static: the packed input volume (8 BF16 channels
+// per voxel) and two 32-channel BF16 activation volumes, for at most
+// RH_DN_STATIC_MAX_VOXELS voxels (144 B per voxel, 36 MiB).  Every GPU the
+// library runs on holds its own copy until the process ends.  In host code
+// the name is only a stand-in for the symbol: the device address is what
+// hipGetSymbolAddress returns with that device current (scratch_on).
+constexpr size_t STATIC_X8_BYTES = (size_t)RH_DN_STATIC_MAX_VOXELS * 8 * sizeof(bf16);
+constexpr size_t STATIC_ACT_BYTES = (size_t)RH_DN_STATIC_MAX_VOXELS * 32 * sizeof(bf16);
+static __device__ __align__(256) unsigned char scratch[STATIC_X8_BYTES + 2 * STATIC_ACT_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void* scratch_dev[MAX_DEVICES] = {};
+
+// The address of the scratch on device `dev`, which must be the current
+// device: looked up once per device and kept.
+void* scratch_on(int dev) {
+    if (dev < 0 || dev >= MAX_DEVICES) throw ArgErr{"device index out of range for the static scratch"};
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void* p = nullptr;
+        DN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw HipErr{"hipGetSymbolAddress gave no address for the static scratch"};
+        scratch_dev[dev] = p;
+    }
+    return scratch_dev[dev];
+}
+
+}  // namespace
+
 struct rh_denoiser {
     hipStream_t stream = nullptr;      // where the work is ordered
     hipStream_t own_stream = nullptr;  // the denoiser's own stream
@@ -492,22 +525,26 @@ struct rh_denoiser {
     Buf<double> lpart;
     int adam_step = 0;
     int wgrad_blocks = 512;  // persistent workgroups (two rounds of one per CU)
+    int device = 0;                  // the device current at creation (own_stream's)
+    unsigned char* static_scratch = nullptr;  // that device's copy of `scratch`
     // timing
     bool timing = false;
+    bool no_events = false;  // rh_dn_forward_static: launches record no events
     std::map<std::string, std::pair<double, int64_t>> times;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
 
     template <typename K, typename... Args>
     void launch(const char* name, K kernel, dim3 grid, dim3 block, Args... args) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (timing) {
+        const bool timed = timing && !no_events;
+        if (timed) {
             DN_HIP(hipEventCreate(&e0));
             DN_HIP(hipEventCreate(&e1));
             DN_HIP(hipEventRecord(e0, stream));
         }
         hipLaunchKernelGGL(kernel, grid, block, 0, stream, args...);
         DN_HIP(hipGetLastError());
-        if (timing) {
+        if (timed) {
             DN_HIP(hipEventRecord(e1, stream));
             pending.push_back({name, {e0, e1}});
         }
@@ -581,9 +618,12 @@ struct rh_denoiser {
         pk(3, 1, 8, 2, bw[3]);
     }
 
-    void check_dims(int P, int X, int Y, int Z) const {
+    static void check_shape(int P, int X, int Y, int Z) {
         if (P <= 0 || X <= 0 || Y <= 0 || Z <= 0) throw ArgErr{"volume dimensions must be positive"};
         if (X % 16 || Y % 4 || Z % 4) throw ArgErr{"X must be a multiple of 16, Y and Z multiples of 4"};
+    }
+    void check_dims(int P, int X, int Y, int Z) const {
+        check_shape(P, X, Y, Z);
         if ((int64_t)P * X * Y * Z > max_voxels) throw CapErr{"P*X*Y*Z exceeds the voxel capacity given at creation"};
     }
 
@@ -605,15 +645,23 @@ struct rh_denoiser {
         launch(name, k_conv<CIN, NT, EP>, dim3(X / TX, Y / TY, P * (Z / TZ)), dim3(256), a);
     }
 
-    void forward(const float* x, float* yo, int P, int X, int Y, int Z) {
+    // The forward pass with the packed input in bx8 and the activations of
+    // layers 1-3 in b1, b2, b3 (b3 may be b1: layer 3 reads only b2).
+    void forward_with(const float* x, float* yo, int P, int X, int Y, int Z, bf16* bx8, bf16* b1, bf16* b2,
+                      bf16* b3) {
         const size_t V = (size_t)P * X * Y * Z;
         launch("pack_input", k_pack_input, dim3(cdiv(V, 256)), dim3(256), x, V, 1.0f / scale[0], 1.0f / scale[1],
                1.0f / scale[2], 1.0f / scale[3], 1.0f / scale[4], 1.0f / scale[5], 1.0f / scale[6],
-               reinterpret_cast<bf16x8*>(x8.p));
-        conv<8, 2, EP_RELU>("conv1", x8.p, fw[0].p, P, X, Y, Z, a1.p, nullptr, nullptr, nullptr);
-        conv<32, 2, EP_RELU>("conv2", a1.p, fw[1].p, P, X, Y, Z, a2.p, nullptr, nullptr, nullptr);
-        conv<32, 2, EP_RELU>("conv3", a2.p, fw[2].p, P, X, Y, Z, a3.p, nullptr, nullptr, nullptr);
-        conv<32, 1, EP_RESID>("conv4", a3.p, fw[3].p, P, X, Y, Z, nullptr, nullptr, yo, x);
+               reinterpret_cast<bf16x8*>(bx8));
+        conv<8, 2, EP_RELU>("conv1", bx8, fw[0].p, P, X, Y, Z, b1, nullptr, nullptr, nullptr);
+        conv<32, 2, EP_RELU>("conv2", b1, fw[1].p, P, X, Y, Z, b2, nullptr, nullptr, nullptr);
+        conv<32, 2, EP_RELU>("conv3", b2, fw[2].p, P, X, Y, Z, b3, nullptr, nullptr, nullptr);
+        conv<32, 1, EP_RESID>("conv4", b3, fw[3].p, P, X, Y, Z, nullptr, nullptr, yo, x);
+    }
+
+    // the backward pass reads all three activations: separate buffers
+    void forward(const float* x, float* yo, int P, int X, int Y, int Z) {
+        forward_with(x, yo, P, X, Y, Z, x8.p, a1.p, a2.p, a3.p);
     }
 
     template <int CIN, int COUT>
@@ -688,8 +736,10 @@ int rh_dn_create(int64_t max_voxels, rh_denoiser** out) {
     *out = nullptr;
     rh_denoiser* d = new rh_denoiser();
     const int rc = guarded(d, [&] {
+        DN_HIP(hipGetDevice(&d->device));
         DN_HIP(hipStreamCreate(&d->own_stream));
         d->stream = d->own_stream;
+        d->static_scratch = static_cast<unsigned char*>(scratch_on(d->device));
         d->alloc(max_voxels);
         DN_HIP(hipStreamSynchronize(d->stream));
     });
@@ -754,6 +804,34 @@ int rh_dn_forward(rh_denoiser* d, const float* x, float* y, int P, int X, int Y,
         d->check_dims(P, X, Y, Z);
         d->forward(x, y, P, X, Y, Z);
         if (d->timing) d->collect();
+    });
+}
+
+int rh_dn_forward_static(rh_denoiser* d, const float* x, float* y, int P, int X, int Y, int Z) {
+    if (!d) return RH_DN_ERR_ARG;
+    return guarded(d, [&] {
+        if (!x || !y) throw ArgErr{"null volume"};
+        rh_denoiser::check_shape(P, X, Y, Z);
+        if ((int64_t)P * X * Y * Z > RH_DN_STATIC_MAX_VOXELS)
+            throw CapErr{"P*X*Y*Z = " + std::to_string((int64_t)P * X * Y * Z) +
+                         " exceeds RH_DN_STATIC_MAX_VOXELS = " + std::to_string((int)RH_DN_STATIC_MAX_VOXELS) +
+                         ", the size of the static scratch"};
+        // the kernels run on the denoiser's stream, which must be on the
+        // device whose copy of the scratch was looked up at creation
+        int sdev = -1;
+        DN_HIP(hipStreamGetDevice(d->stream, &sdev));
+        if (sdev != d->device) throw ArgErr{"the denoiser's stream is on another device than the one it was created on"};
+        bf16* x8s = reinterpret_cast<bf16*>(d->static_scratch);
+        bf16* a1s = reinterpret_cast<bf16*>(d->static_scratch + STATIC_X8_BYTES);
+        bf16* a2s = reinterpret_cast<bf16*>(d->static_scratch + STATIC_X8_BYTES + STATIC_ACT_BYTES);
+        d->no_events = true;
+        try {
+            d->forward_with(x, y, P, X, Y, Z, x8s, a1s, a2s, a1s);
+        } catch (...) {
+            d->no_events = false;
+            throw;
+        }
+        d->no_events = false;
     });
 }
 

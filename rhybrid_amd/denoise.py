@@ -18,6 +18,7 @@ from .gpu import load_library
 N_WEIGHTS = 64800
 LAYERS = ((7, 32), (32, 32), (32, 32), (32, 4))  # (cin, cout), 3x3x3 kernels, no biases
 RH_DN_OK, RH_DN_ERR_ARG, RH_DN_ERR_HIP, RH_DN_ERR_CAPACITY = 0, -1, -2, -4
+STATIC_MAX_VOXELS = 262144  # RH_DN_STATIC_MAX_VOXELS: the static scratch of rh_dn_forward_static
 
 
 class DenoiserError(RuntimeError):
@@ -43,6 +44,7 @@ def _lib():
         L.rh_dn_get_weights.argtypes = [vp, vp, i64]
         L.rh_dn_set_scales.argtypes = [vp, vp]
         L.rh_dn_forward.argtypes = [vp, vp, vp, i32, i32, i32, i32]
+        L.rh_dn_forward_static.argtypes = [vp, vp, vp, i32, i32, i32, i32]
         L.rh_dn_train_step.argtypes = [vp, vp, vp, i32, i32, i32, i32, f32, C.POINTER(f32)]
         L.rh_dn_gradient.argtypes = [vp, vp, vp, i32, i32, i32, i32, vp, C.POINTER(f32)]
         L.rh_dn_synchronize.argtypes = [vp]
@@ -131,6 +133,17 @@ class Denoiser:
         if y is None:
             y = torch.empty((P, Z, Y, X, 4), dtype=torch.float32, device=x.device)
         self._check(_lib().rh_dn_forward(self._h, x.data_ptr(), y.data_ptr(), P, X, Y, Z))
+        return y
+
+    def forward_static(self, x, y):
+        """forward() through rh_dn_forward_static: scratch in the library's
+        static device array (at most STATIC_MAX_VOXELS voxels), nothing
+        allocated; the output tensor y must be given."""
+        P, X, Y, Z = self._dims(x, 7)
+        if tuple(y.shape) != (P, Z, Y, X, 4):
+            raise ValueError("y must have shape [P][Z][Y][X][4]")
+        self._dims(y, 4)
+        self._check(_lib().rh_dn_forward_static(self._h, x.data_ptr(), y.data_ptr(), P, X, Y, Z))
         return y
 
     def gradient(self, x, target) -> Tuple[np.ndarray, float]:
