@@ -969,8 +969,9 @@ public:
             a.aveB = ave_on ? aveB.p : nullptr;
             if (ave_on) ++ave_count;
             const dim3 grid(cdiv(g.nx, FieldTile<Real>::X), cdiv(g.ny, FieldTile<Real>::Y), cdiv(g.nz, FieldTile<Real>::Z));
-            if (full) launch("field_update", k_field_update<Real, true>, grid, 256, 0, g, a);
-            else launch("field_update", k_field_update<Real, false>, grid, 256, 0, g, a);
+            constexpr int nt = FieldUpdateThreads<Real>::N;
+            if (full) launch("field_update", k_field_update<Real, true>, grid, nt, 0, g, a);
+            else launch("field_update", k_field_update<Real, false>, grid, nt, 0, g, a);
             std::swap(fld[RH_FACE_B].p, faceB_alt.p);
             pf_dirty = false;
             return;

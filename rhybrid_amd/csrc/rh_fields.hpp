@@ -639,46 +639,82 @@ __device__ __forceinline__ void imf_values(const Geom& g, int i, int j, int k, i
     }
 }
 
+// Threads of the fused update: a tile's loops (1331 halo faces, 1000 cells,
+// 729 nodes, 512 own cells for FP32 8x8x8 tiles) take a few rounds each.
+template <typename Real> struct FieldUpdateThreads;
+template <> struct FieldUpdateThreads<float> { static constexpr int N = 512; };
+template <> struct FieldUpdateThreads<double> { static constexpr int N = 256; };
+
 template <typename Real, bool FULL>
-__global__ __launch_bounds__(256) void k_field_update(Geom g, FieldUpdateArgs<Real> a) {
+__global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Geom g, FieldUpdateArgs<Real> a) {
     constexpr int TX = FieldTile<Real>::X, TY = FieldTile<Real>::Y, TZ = FieldTile<Real>::Z;
     constexpr int X2 = TX + 3, Y2 = TY + 3, Z2 = TZ + 3, N2 = X2 * Y2 * Z2;  // faceB: cells -2..T
     constexpr int X1 = TX + 2, Y1 = TY + 2, Z1 = TZ + 2, N1 = X1 * Y1 * Z1;  // cells -1..T
     constexpr int XN = TX + 1, YN = TY + 1, ZN = TZ + 1, NN = XN * YN * ZN;  // nodes -1..T-1
+    constexpr int NT = FieldUpdateThreads<Real>::N;
+    constexpr int R2 = (N2 + NT - 1) / NT;  // rounds over the face halo
     __shared__ Real sF[3][N2];
     __shared__ Real sB[3][N1], sJi[3][N1], sR[N1];
     __shared__ Real nJ[3][NN], nE[3][NN], nR[NN];
+    // per cell of the cell region: nodeEta of its node, and the inner flags
+    // (bit 0 innerFlagField, bit 1 innerFlagNode, bit 2 innerFlagCellEp)
+    __shared__ Real sEta[N1];
+    __shared__ uint8_t sFl[N1];
     const int tid = threadIdx.x;
     const int ox = blockIdx.x * TX, oy = blockIdx.y * TY, oz = blockIdx.z * TZ;
     const V3<Real>& F = a.faceB_in;
 
-    // 1. faceB (exterior cells: neumannFace, then setIMFFace), cellRhoQi,
-    //    cellJi and the exterior cells' cellB (neumannCell of the source's
-    //    face2Cell, then setIMFCell)
-    for (int e = tid; e < N2; e += blockDim.x) {
+    // 1. faceB, cellRhoQi, cellJi, nodeEta and the inner flags of the halo.
+    //    All of a thread's loads are issued before its LDS stores; the cells
+    //    without all neighbours then get their copies (neumannFace and
+    //    setIMFFace on faceB; the Neumann source's face2Cell with setIMFCell
+    //    for cellB, and its cellRhoQi, cellJi).
+    int en[R2];                       // cell index of the entry (-1: none or outside the grid)
+    Real lf[R2][3], lj[R2][3], lr[R2], le[R2];
+    uint8_t lfl[R2];
+#pragma unroll
+    for (int r = 0; r < R2; ++r) {
+        const int e = tid + r * NT;
+        en[r] = -1;
+        lf[r][0] = lf[r][1] = lf[r][2] = Real(0);
+        lj[r][0] = lj[r][1] = lj[r][2] = Real(0);
+        lr[r] = le[r] = Real(0);
+        lfl[r] = 0;
+        if (e >= N2) continue;
         const int lx = e % X2, ly = (e / X2) % Y2, lz = e / (X2 * Y2);
         const int i = wrap_axis(ox - 2 + lx, g.nx, g.per[0]), j = wrap_axis(oy - 2 + ly, g.ny, g.per[1]),
                   k = wrap_axis(oz - 2 + lz, g.nz, g.per[2]);
+        if (i < 0 || j < 0 || k < 0) continue;
+        const int n = cell_index(g, i, j, k);
+        en[r] = n;
+        lf[r][0] = F.x[n]; lf[r][1] = F.y[n]; lf[r][2] = F.z[n];
+        if (lx >= 1 && ly >= 1 && lz >= 1) {
+            lj[r][0] = a.Ji.x[n]; lj[r][1] = a.Ji.y[n]; lj[r][2] = a.Ji.z[n];
+            lr[r] = a.rho[n];
+            le[r] = a.eta[n];
+            lfl[r] = (a.inner_field[n] ? 1 : 0) | (a.inner_node[n] ? 2 : 0) | (a.inner_cellEp[n] ? 4 : 0);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R2; ++r) {
+        const int e = tid + r * NT;
+        if (e >= N2) continue;
+        const int lx = e % X2, ly = (e / X2) % Y2, lz = e / (X2 * Y2);
         const bool in1 = lx >= 1 && ly >= 1 && lz >= 1;
         const int e1 = in1 ? (lx - 1) + X1 * ((ly - 1) + Y1 * (lz - 1)) : 0;
-        if (i < 0 || j < 0 || k < 0) {
-            sF[0][e] = sF[1][e] = sF[2][e] = Real(0);
-            if (in1) {
-                sB[0][e1] = sB[1][e1] = sB[2][e1] = Real(0);
-                sJi[0][e1] = sJi[1][e1] = sJi[2][e1] = Real(0);
-                sR[e1] = Real(0);
-            }
-            continue;
+        sF[0][e] = lf[r][0]; sF[1][e] = lf[r][1]; sF[2][e] = lf[r][2];
+        if (in1) {
+            sB[0][e1] = sB[1][e1] = sB[2][e1] = Real(0);
+            sJi[0][e1] = lj[r][0]; sJi[1][e1] = lj[r][1]; sJi[2][e1] = lj[r][2];
+            sR[e1] = lr[r];
+            sEta[e1] = le[r];
+            sFl[e1] = lfl[r];
         }
-        const int n = cell_index(g, i, j, k);
-        if (all_nbrs(g, i, j, k)) {
-            sF[0][e] = F.x[n]; sF[1][e] = F.y[n]; sF[2][e] = F.z[n];
-            if (in1) {
-                sJi[0][e1] = a.Ji.x[n]; sJi[1][e1] = a.Ji.y[n]; sJi[2][e1] = a.Ji.z[n];
-                sR[e1] = a.rho[n];
-            }
-            continue;
-        }
+        const int n = en[r];
+        if (n < 0) continue;
+        int i, j, k;
+        cell_ijk(g, n, i, j, k);
+        if (all_nbrs(g, i, j, k)) continue;
         const int miss = missing_mask(g, i, j, k);
         Real f[3];
         for (int l = 0; l < 3; ++l) {
@@ -758,7 +794,7 @@ __global__ __launch_bounds__(256) void k_field_update(Geom g, FieldUpdateArgs<Re
             const Real J[3] = {(ex1 + ex2) * af, (ey1 + ey2) * af, (ez1 + ez2) * af};
             // calcNodeUe (clamps the node's rho)
             Real u[3];
-            ue_core<Real, true>(J, ji, r, a.inner_node[n] != 0, a.use_hall, a.oue, a.max_ue2, a.min_rho, u, c_ue, c_rho);
+            ue_core<Real, true>(J, ji, r, (sFl[c] & 2) != 0, a.use_hall, a.oue, a.max_ue2, a.min_rho, u, c_ue, c_rho);
             // upwindNodeB: cells in the reference's order 111,112,121,211,122,221,212,222
             Real xu = Real(-0.5), yu = Real(0), zu = Real(0);
             const Real ue = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
@@ -789,7 +825,7 @@ __global__ __launch_bounds__(256) void k_field_update(Geom g, FieldUpdateArgs<Re
                 b[0] = b[1] = b[2] = Real(0);
             }
             Real E[3];
-            c_e = node_E_at(u, b, a.eta[n], J[0], J[1], J[2], a.max_E2, E);
+            c_e = node_E_at(u, b, sEta[c], J[0], J[1], J[2], a.max_E2, E);
             for (int l = 0; l < 3; ++l) { nJ[l][q] = J[l]; nE[l][q] = E[l]; }
             nR[q] = r;
             if (FULL && own) {
@@ -833,7 +869,7 @@ __global__ __launch_bounds__(256) void k_field_update(Geom g, FieldUpdateArgs<Re
                 cellJ_at(qn, cj);
                 Real r = sR[c1];
                 const Real ji[3] = {sJi[0][c1], sJi[1][c1], sJi[2][c1]};
-                ue_core<Real, false>(cj, ji, r, a.inner_field[n] != 0, a.use_hall, a.oue, a.max_ue2, a.min_rho, u, c_ue,
+                ue_core<Real, false>(cj, ji, r, (sFl[c1] & 1) != 0, a.use_hall, a.oue, a.max_ue2, a.min_rho, u, c_ue,
                                      c_rho);
             } else {
                 const int src = neumann_source(g, i, j, k);
@@ -846,7 +882,7 @@ __global__ __launch_bounds__(256) void k_field_update(Geom g, FieldUpdateArgs<Re
                 Real r = sR[cs1];
                 const Real ji[3] = {sJi[0][cs1], sJi[1][cs1], sJi[2][cs1]};
                 bool c0 = false;
-                ue_core<Real, false>(cs, ji, r, a.inner_field[src] != 0, a.use_hall, a.oue, a.max_ue2, a.min_rho, u, c0,
+                ue_core<Real, false>(cs, ji, r, (sFl[cs1] & 1) != 0, a.use_hall, a.oue, a.max_ue2, a.min_rho, u, c0,
                                      c_rho);
             }
             a.cellUe.x[n] = u[0]; a.cellUe.y[n] = u[1]; a.cellUe.z[n] = u[2];
@@ -854,7 +890,7 @@ __global__ __launch_bounds__(256) void k_field_update(Geom g, FieldUpdateArgs<Re
             Real ep[3] = {Real(0), Real(0), Real(0)};
             if (a.use_e_pressure) {
                 if (interior) {
-                    if (!a.inner_cellEp[n]) {
+                    if (!(sFl[c1] & 4)) {
                         auto N = [&](int ax, int ay, int az) { return nR[qn + ax + XN * (ay + YN * az)]; };
                         const Real xP = Real(0.25) * (N(0, 0, 0) + N(0, 0, -1) + N(0, -1, -1) + N(0, -1, 0));
                         const Real xN = Real(0.25) * (N(-1, 0, 0) + N(-1, 0, -1) + N(-1, -1, -1) + N(-1, -1, 0));

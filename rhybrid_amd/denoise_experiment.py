@@ -50,9 +50,9 @@ def run(train_box=(128, 96, 96), R_train=10.0, hi_ppc=256, lo_ppc=8, patches=200
                                every=every, crops_per_snapshot=crops, crop_seed=8, log=log)
     log("pairs done: %d training, %d held-out patches" % (X.shape[0], Xh.shape[0]))
     dn = D.Denoiser(batch * DT.PATCH ** 3, scales=scales, weights=D.init_weights(seed=2, gain=0.1))
-    raw = DT.patch_errors(None, Xh, Th, scales)
+    raw = DT.patch_errors(None, Xh, Th, scales, batch=batch)
     tr = DT.train(dn, X, T, epochs=epochs, batch=batch, lr=lr, lr_final=lr_final, log=log)
-    den = DT.patch_errors(dn, Xh, Th, scales)
+    den = DT.patch_errors(dn, Xh, Th, scales, batch=batch)
     w = dn.weights()
     dn.close()
     if weights_out:
