@@ -159,7 +159,7 @@ def compare_uneven(dn, nx=256, ny=192, nz=192, R_cells=12.0, H_cells=5.0, steps=
     n and J errors over the plasma cells every `every` steps, time per step,
     fallback fraction."""
     make = lambda ppc: P.config_exosphere(nx=nx, ny=ny, nz=nz, ppc=ppc, R_cells=R_cells, H_cells=H_cells,
-                                          seed=seed, o_per_ppc=True)
+                                          seed=seed, o_per_ppc=True, steps=steps)
     runs = {}
     judge_ppc, low = max(ppcs), min(ppcs)
     for ppc in sorted(ppcs, reverse=True):

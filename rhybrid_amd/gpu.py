@@ -18,6 +18,7 @@ from typing import Dict, List, Optional
 
 import numpy as np
 
+from . import problem as P
 from .problem import FIELD_DIM, FIELD_NAMES, FLAG_NAMES, Problem
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -289,15 +290,44 @@ class Context:
 
     def inject(self, s: int) -> None: self._check(load_library().rh_inject_particles(self._h, s))
 
+    # places of one injection launch (32-bit place indices)
+    MAX_PLACES = (1 << 31) - 1
+
+    def _slabs(self, inj):
+        """A uniform injector whose fixed places (ceil(ppc) per cell) would
+        exceed MAX_PLACES, split into z slabs of its box that hold every cell
+        centre exactly once (slab bounds on cell faces).  The particles drawn
+        do not depend on the split: each one's generator counter is its cell,
+        the timestep and its index in the cell."""
+        import dataclasses
+        import math
+        g = self.grid
+        if inj is None or inj.kind != P.INJ_UNIFORM:
+            return [inj]
+        per_plane = math.ceil(inj.ppc) * g.nx * g.ny
+        planes = max(1, self.MAX_PLACES // max(per_plane, 1))
+        if per_plane * g.nz <= self.MAX_PLACES:
+            return [inj]
+        out = []
+        for k0 in range(0, g.nz, planes):
+            k1 = min(g.nz, k0 + planes)
+            b = list(inj.box)
+            b[4] = max(b[4], g.z0 + k0 * g.dx)
+            b[5] = min(b[5], g.z0 + k1 * g.dx)
+            if b[4] < b[5]:
+                out.append(dataclasses.replace(inj, box=tuple(b)))
+        return out
+
     def fill(self, s: int, injector, timesteps=(1,)) -> None:
         """Run `injector` once per timestep in `timesteps` for species s (e.g.
         the reference's InjectorUniform at timestep 1), then restore the
         species' own injector and the timestep."""
         t0 = self.timestep()
-        self.set_injector(s, injector)
-        for t in timesteps:
-            self.set_timestep(t)
-            self.inject(s)
+        for part in self._slabs(injector):
+            self.set_injector(s, part)
+            for t in timesteps:
+                self.set_timestep(t)
+                self.inject(s)
         own = (self.problem.injectors or [None] * (s + 1))
         self.set_injector(s, own[s] if s < len(own) else None)
         self.set_timestep(t0)

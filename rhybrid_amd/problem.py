@@ -420,8 +420,10 @@ def config_exosphere(nx: int = 256, ny: int = 192, nz: int = 192, ppc: int = 24,
         macro_per_dt *= ppc / 24.0
     neut = exosphere_neutrals(grid, n0, R, H, R_exobase=R, sub=sub)
     total_rate = ionization_rate * neut.sum()
+    # O+ accumulates until pick-up carries it out of the box (~1400 steps
+    # at the default sizes): room for `steps` steps of production
     spO = Species("O+", QE, 16 * MP, total_rate * dt / macro_per_dt, R2_obstacle=R * R,
-                  capacity=int(4 * macro_per_dt * 400 + 1e6))
+                  capacity=int(macro_per_dt * max(1600, steps + 200) + 1e6))
     p = Params(dt=dt, use_hall=1, use_e_pressure=1, isothermal=1, e_pressure_coeff=KB * Te / QE, imf=B0,
                imf_cell=(1, 0, 0, 0, 0, 0), imf_face=(1, 0, 0, 0, 0, 0))
     _constraints(p, dx, dt, 1500e3, 1500e3, n_sw * QE, 0.05, 1500e3)

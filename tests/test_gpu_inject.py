@@ -109,6 +109,30 @@ def test_uniform_fill_matches_oracle(gpu_mod, oracle_mod, prec):
     assert ctx.injected(0) == ora.injected(0) > 0
 
 
+def test_uniform_fill_split_in_slabs(gpu_mod):
+    # a fill with more fixed places than one launch takes runs slab by slab
+    # along z; the particles do not depend on the split
+    g = P.Grid(16, 8, 12, (0, 0, 0), 1e5, -8e5, -4e5, -6e5)
+    p = P.Params(dt=0.05)
+    n, ppc = 2e6, 5.5
+    sp = P.Species("H+", P.QE, P.MP, n * g.dV / ppc, capacity=200000)
+    inj = P.uniform_injector((-3e5, 1e4, 0.0), n, 1e5, ppc, seed=34, box=(-6e5, 6e5, -1e50, 1e50, -5e5, 1e50))
+    empty = P.Particles(np.zeros(0, dtype=np.int64), np.zeros((0, 7)))
+    pr = P.Problem("uniform_slabs", g, p, [sp], P.make_fields(g), P.make_flags(g), [empty], injectors=[inj])
+    whole = gpu_mod.Context(pr, precision=gpu_mod.RH_FP64)
+    assert len(whole._slabs(inj)) == 1
+    whole.fill(0, inj)
+    split = gpu_mod.Context(pr, precision=gpu_mod.RH_FP64)
+    split.MAX_PLACES = 6 * g.nx * g.ny * 5  # slabs of five z planes
+    assert len(split._slabs(inj)) == 3
+    split.fill(0, inj)
+    (ca, sa), (cb, sb) = whole.particles(0), split.particles(0)
+    assert len(ca) > 0
+    assert np.array_equal(ca, cb) and np.array_equal(sa, sb)
+    whole.close()
+    split.close()
+
+
 def test_pending_particles_in_diagnostics(gpu_mod, oracle_mod):
     pr = wall_box((-3e5, 0, 0))
     ctx, ora = injected_pair(gpu_mod, oracle_mod, pr, 1, steps=[4])
