@@ -174,13 +174,27 @@ def gaussian_coeffs(sigma: float) -> Tuple[float, float, float, float]:
     return tuple(float(x / s) for x in c)
 
 
-def make_fields(grid: Grid, B0=(0.0, 0.0, 0.0), eta: float = 0.0) -> Dict[str, np.ndarray]:
+def make_fields(grid: Grid, B0=(0.0, 0.0, 0.0), eta: float = 0.0, intrinsic_B=None, resistivity=None,
+                dt: float = 0.0, u_bulk: float = 0.0) -> Dict[str, np.ndarray]:
+    """The initial grid arrays: faceB and cellB of the constantB profile B0
+    (USE_B_INITIAL, src/user.cpp:2107-2141) or of `intrinsic_B`
+    (profiles.IntrinsicB: the reference's magnetic_field.h profiles), and
+    nodeEta = eta or the `resistivity` profile (profiles.Resistivity, with its
+    unit conversion by dt and u_bulk and the outer-boundary multipliers)."""
+    from . import profiles
     f = {n: np.zeros(grid.nc * d, dtype=np.float64) for n, d in zip(FIELD_NAMES, FIELD_DIM)}
-    # constantB profile on faces and cells (USE_B_INITIAL, src/user.cpp:2107-2141)
-    for l in range(3):
-        f["faceB"][l::3] = B0[l]
-        f["cellB"][l::3] = B0[l]
-    f["nodeEta"][:] = eta
+    if intrinsic_B is None:
+        for l in range(3):
+            f["faceB"][l::3] = B0[l]
+            f["cellB"][l::3] = B0[l]
+    else:
+        f["faceB"], f["cellB"] = profiles.initial_B(grid, intrinsic_B)
+    if resistivity is None:
+        f["nodeEta"][:] = eta
+    else:
+        if not dt > 0:
+            raise ValueError("a resistivity profile needs dt (its grid and td units)")
+        f["nodeEta"] = profiles.node_eta(grid, resistivity, dt, u_bulk)
     return f
 
 
