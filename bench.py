@@ -272,6 +272,78 @@ def denoiser_forward_volume(shape=(256, 192, 192), reps: int = 5):
             "forward_tflops": D.forward_flops(V) / t / 1e12, "ns_per_voxel": t / V * 1e9, "kernels": kern}
 
 
+DENOISER_DATA = os.path.join(ROOT, "rhybrid_amd", "data")
+
+
+def denoiser_config5(steps: int = 10, warmup: int = 2, ppcs=(8, 64)):
+    """The denoiser in the loop at the uneven-load geometry (256x192x192
+    cells, exospheric O+ in proportion to the H+ ppc): time per step of the
+    plain FP32 method at each of `ppcs` and of the lowest-ppc run with the
+    denoiser attached (gather, forward over the 9.4 M-voxel volume, fallback
+    bound, scatter, every step), each device-filled and timed over `steps`
+    steps after `warmup`.  The trained weights and the 2000-step error curves
+    against a 256-ppc truth run come from rhybrid_amd/denoise_experiment.py
+    (its committed result, rhybrid_amd/data/denoiser_experiment.json)."""
+    from rhybrid_amd import denoise as D
+    from rhybrid_amd import denoise_data as DD
+    from rhybrid_amd import problem as P
+    from rhybrid_amd.denoise_train import filled_context
+    make = lambda ppc: P.config_exosphere(ppc=ppc, o_per_ppc=True, seed=23)
+    wpath = os.path.join(DENOISER_DATA, "denoiser_weights.npy")
+    w = np.load(wpath) if os.path.exists(wpath) else D.init_weights(seed=0)
+    out = {"geometry": "256x192x192 uneven-load box (H+ solar wind, O+ in proportion to the H+ ppc), FP32",
+           "weights": os.path.relpath(wpath, ROOT) if os.path.exists(wpath) else "untrained (timing only)",
+           "runs": {}}
+
+    def timed(ppc, dn=None):
+        pr = make(ppc)
+        ctx = filled_context(pr)
+        if dn is not None:
+            ctx.set_denoiser(dn, 0.5)
+        ctx.step(warmup)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.step(steps)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        n = [ctx.num_particles(s) for s in range(2)]
+        fb = ctx.counters()["denoise_fallback"] if dn is not None else None
+        if dn is not None:
+            ctx.set_denoiser(None)
+        ctx.close()
+        return ms, n, fb, pr
+
+    for ppc in sorted(ppcs):
+        ms, n, _, pr = timed(ppc)
+        out["runs"]["plain_%d" % ppc] = {"ms_per_step": round(ms, 3), "particles": n}
+        print("# config-5 timing: plain %d ppc %.2f ms/step" % (ppc, ms), file=sys.stderr, flush=True)
+    low = min(ppcs)
+    pr = make(low)
+    g = pr.grid
+    dn = D.Denoiser(g.nc, scales=DD.channel_scales(pr), weights=w)
+    ms, n, fb, _ = timed(low, dn)
+    dn.close()
+    out["runs"]["denoised_%d" % low] = {"ms_per_step": round(ms, 3), "particles": n,
+                                        "fallback_fraction": fb / float((warmup + steps) * g.nc)}
+    print("# config-5 timing: denoised %d ppc %.2f ms/step" % (low, ms), file=sys.stderr, flush=True)
+    if 64 in ppcs:
+        out["time_ratio_plain64_over_denoised8"] = out["runs"]["plain_64"]["ms_per_step"] / ms
+    rpath = os.path.join(DENOISER_DATA, "denoiser_experiment.json")
+    if os.path.exists(rpath):
+        exp = json.load(open(rpath))
+        loop = exp.get("in_loop", {})
+        out["experiment"] = {"file": os.path.relpath(rpath, ROOT),
+                             "command": "python -m rhybrid_amd.denoise_experiment",
+                             "training": {k: exp["training"].get(k) for k in
+                                          ("pairs", "patches", "epochs", "heldout_error_raw", "heldout_error_denoised")},
+                             "truth": loop.get("truth"), "steps": loop.get("geometry"),
+                             "denoised_vs_plain_64": loop.get("denoised_vs_plain_64"),
+                             "final_errors": {k: v["errors"][-1] for k, v in loop.get("runs", {}).items()},
+                             "ms_per_step": {k: v["ms_per_step"] for k, v in loop.get("runs", {}).items()},
+                             "truth_fp32_vs_fp64": exp.get("truth_fp32_vs_fp64")}
+    return out
+
+
 def _rel_errors(x_gpu: np.ndarray, x_orc: np.ndarray, dim: int) -> dict:
     a = x_gpu.reshape(-1, dim)
     b = x_orc.reshape(-1, dim)
@@ -364,29 +436,44 @@ def cpu_baseline(threads: int, pr, steps: int = 4):
     return cpu, err
 
 
-def error_curve(pr, steps: int = 500, every: int = 50):
+def error_curve(pr, steps: int = 500, every: int = 50, order_tile=(2, 4, 8)):
     """FP32 against FP64 on the headline box over `steps` full steps, both
     on the GPU from the same upload (the FP64 instantiation runs the same
     kernels in double and matches the FP64 oracle to 1e-9 in the tests): the
-    relative errors of n, J, E and B every `every` steps."""
+    relative errors of n, J, E and B every `every` steps.  Beside it, FP64
+    against FP64 with another tile shape (`order_tile`): the same arithmetic
+    in double, only the order of the deposit's and the ghost fold's sums
+    differs, so the two start apart by rounding in the last bits.  Where that
+    difference grows at the same rate as the FP32 one, the growth is the
+    flow's amplification of any perturbation, not an inaccuracy of FP32."""
     from rhybrid_amd.gpu import RH_FP32, RH_FP64, Context
     c32 = Context(pr, precision=RH_FP32, tile=BENCH_TILE)
     c64 = Context(pr, precision=RH_FP64, tile=BENCH_TILE)
+    c64o = Context(pr, precision=RH_FP64, tile=order_tile)
+    names = (("n", "cellRhoQi", 1), ("J", "cellJi", 3), ("E", "nodeE", 3), ("B", "faceB", 3))
     rows = []
     for done in range(every, steps + 1, every):
         c32.step(every)
         c64.step(every)
+        c64o.step(every)
         row = {"step": done}
-        for key, nm, dim in (("n", "cellRhoQi", 1), ("J", "cellJi", 3), ("E", "nodeE", 3), ("B", "faceB", 3)):
-            row[key] = _rel_errors(c32.field(nm), c64.field(nm), dim)
+        ref = {nm: c64.field(nm) for _, nm, _ in names}
+        for key, nm, dim in names:
+            row[key] = _rel_errors(c32.field(nm), ref[nm], dim)
+        row["fp64_other_order"] = {key: _rel_errors(c64o.field(nm), ref[nm], dim) for key, nm, dim in names}
         row["particles_fp32"] = [c32.num_particles(s) for s in range(len(pr.species))]
         row["particles_fp64"] = [c64.num_particles(s) for s in range(len(pr.species))]
         rows.append(row)
-        print("# error curve step %d: n %.2e J %.2e E %.2e B %.2e" % (done, row["n"]["rel_l2"], row["J"]["rel_l2"],
-              row["E"]["rel_l2"], row["B"]["rel_l2"]), file=sys.stderr, flush=True)
+        o = row["fp64_other_order"]
+        print("# error curve step %d: n %.2e J %.2e E %.2e B %.2e | fp64 other order: n %.2e B %.2e"
+              % (done, row["n"]["rel_l2"], row["J"]["rel_l2"], row["E"]["rel_l2"], row["B"]["rel_l2"],
+                 o["n"]["rel_l2"], o["B"]["rel_l2"]), file=sys.stderr, flush=True)
     c32.close()
     c64.close()
-    return {"workload": "the headline box, FP32 vs FP64 GPU paths from the same upload, %d full steps" % steps,
+    c64o.close()
+    return {"workload": "the headline box, FP32 vs FP64 GPU paths from the same upload, %d full steps; "
+                        "fp64_other_order: FP64 at tile %s vs FP64 at tile %s (summation order only)"
+                        % (steps, "x".join(map(str, order_tile)), "x".join(map(str, BENCH_TILE))),
             "tile": list(BENCH_TILE), "rows": rows}
 
 
@@ -520,6 +607,7 @@ def main():
     if dn is not None and not args.no_in_loop:
         from rhybrid_amd import denoise_loop
         dn["in_loop"] = denoise_loop.compare()
+        dn["in_loop_config5"] = denoiser_config5()
     exo = None
     if not args.no_exosphere:
         try:

@@ -14,7 +14,8 @@
  * Activations and weights are BF16 on the device, products accumulate in
  * FP32 on the matrix cores; the residual and the loss are FP32.
  *
- * Layouts (all device pointers are hipMalloc'd memory of the caller):
+ * Layouts (all device pointers are hipMalloc'd memory of the caller; output
+ * volumes 16-byte aligned):
  *   volumes   float [P][Z][Y][X][C], C = 7 for inputs, 4 for outputs/targets
  *   weights   float, the four layers concatenated, each [cout][cin][kz][ky][kx]
  *             (64,800 values: 6,048 + 27,648 + 27,648 + 3,456)

@@ -5,17 +5,19 @@
 // backward-data and backward-weights -- is an implicit GEMM on
 // v_mfma_f32_16x16x32_bf16 (BF16 operands, FP32 accumulation):
 //
-//  * forward / backward-data (k_conv): M = 16 output voxels along x, N = 16
-//    output channels, K = 32 = one tap's 32 input channels (or four taps of
-//    8 channels for the 8-channel layers).  A workgroup computes a 16x4x4
-//    voxel tile (one wave per z plane, four M tiles of one x row each); the
-//    18x6x6 input halo sits in LDS in channels-last order, so a lane's A
+//  * forward / backward-data (k_conv): M = 16 output channels, N = 16 output
+//    voxels along x, K = 32 = one tap's 32 input channels (or four taps of 8
+//    channels for the 8-channel layers).  A workgroup computes a 16x4x4
+//    voxel tile (one wave per z plane, four N tiles of one x row each); the
+//    18x6x6 input halo sits in LDS in channels-last order, so a lane's voxel
 //    fragment (8 channels of one voxel) is one 16-byte LDS read and a wave
 //    reads 1 KB contiguously; each input row's fragment serves the (up to)
-//    three output rows it reaches through the dy taps.  B fragments (weights) are pre-packed in the
-//    lane order of the instruction and read from L1/L2.  The epilogue applies
-//    ReLU (forward), the ReLU mask of the forward activation (backward), or
-//    the FP32 residual (last layer).
+//    three output rows it reaches through the dy taps.  Weight fragments are
+//    pre-packed in the lane order of the instruction and read from L1/L2.
+//    With the weights as the A operand a lane ends with four consecutive
+//    channels of one voxel, stored as one vector.  The epilogue applies ReLU
+//    (forward), the ReLU mask of the forward activation (backward), or the
+//    FP32 residual (last layer).
 //  * backward-weights (k_wgrad): dW[tap][ci][co] = sum over voxels of
 //    in[v + tap][ci] * G[v][co]: M = 16 input channels, N = 16 output
 //    channels, K = 32 voxels.  The workgroups are persistent; each stages a
@@ -43,6 +45,7 @@ namespace rhdn {
 
 typedef __bf16 bf16;
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 
 constexpr int TX = 16, TY = 4, TZ = 4;             // forward output tile
@@ -147,7 +150,7 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
                     const int dy = r - my;
                     if (dy < 0 || dy > 2) continue;
 #pragma unroll
-                    for (int nt = 0; nt < NT; ++nt) acc[my][nt] = mfma(av, b[dy][nt], acc[my][nt]);
+                    for (int nt = 0; nt < NT; ++nt) acc[my][nt] = mfma(b[dy][nt], av, acc[my][nt]);
                 }
             }
         }
@@ -167,30 +170,40 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
                 const int v = ((w + dz) * HY + (my + dy)) * HX + (m + dx);
                 const bf16x8 av = tile[v];
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt) acc[my][nt] = mfma(av, b[nt], acc[my][nt]);
+                for (int nt = 0; nt < NT; ++nt) acc[my][nt] = mfma(b[nt], av, acc[my][nt]);
             }
         }
     }
-    // lane holds D[voxel x = 4*kg + i][channel nt*16 + m] of row (my, w)
+    // The weights are the A operand and the voxels the B operand, so lane
+    // (m, kg) holds D[channel nt*16 + 4*kg + i][voxel x = m] of row (my, w):
+    // four consecutive channels of one voxel, stored as one 8-byte vector
+    // (the FP32 residual of the last layer: channels 0-3 as one 16-byte one).
     const int z = bz + w;
+    const int x = bx + m;
 #pragma unroll
     for (int my = 0; my < TY; ++my) {
         const int y = by + my;
+        const size_t v = (size_t)p * vol + ((size_t)z * a.Y + y) * a.X + x;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int x = bx + 4 * kg + i;
-            const size_t v = (size_t)p * vol + ((size_t)z * a.Y + y) * a.X + x;
+        for (int nt = 0; nt < NT; ++nt) {
+            const int c0 = nt * 16 + 4 * kg;
+            if constexpr (EP == EP_RELU) {
+                bf16x4 o;
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                const float r = acc[my][nt][i];
-                const int c = nt * 16 + m;
-                if constexpr (EP == EP_RELU) {
-                    a.out[v * (NT * 16) + c] = (bf16)fmaxf(r, 0.0f);
-                } else if constexpr (EP == EP_MASK) {
-                    const float keep = (float)a.mask[v * (NT * 16) + c] > 0.0f ? 1.0f : 0.0f;
-                    a.out[v * (NT * 16) + c] = (bf16)(r * keep);
-                } else {
-                    if (m < 4) a.yout[v * 4 + m] = a.xres[v * 7 + m] + a.s[m] * r;
+                for (int i = 0; i < 4; ++i) o[i] = (bf16)fmaxf(acc[my][nt][i], 0.0f);
+                *reinterpret_cast<bf16x4*>(a.out + v * (NT * 16) + c0) = o;
+            } else if constexpr (EP == EP_MASK) {
+                const bf16x4 mk = *reinterpret_cast<const bf16x4*>(a.mask + v * (NT * 16) + c0);
+                bf16x4 o;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) o[i] = (bf16)(acc[my][nt][i] * ((float)mk[i] > 0.0f ? 1.0f : 0.0f));
+                *reinterpret_cast<bf16x4*>(a.out + v * (NT * 16) + c0) = o;
+            } else {
+                if (nt == 0 && kg == 0) {
+                    floatx4 o;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) o[i] = a.xres[v * 7 + i] + a.s[i] * acc[my][nt][i];
+                    *reinterpret_cast<floatx4*>(a.yout + v * 4) = o;
                 }
             }
         }
@@ -801,6 +814,7 @@ int rh_dn_forward(rh_denoiser* d, const float* x, float* y, int P, int X, int Y,
     if (!d) return RH_DN_ERR_ARG;
     return guarded(d, [&] {
         if (!x || !y) throw ArgErr{"null volume"};
+        if ((uintptr_t)y % 16) throw ArgErr{"the output volume must be 16-byte aligned"};
         d->check_dims(P, X, Y, Z);
         d->forward(x, y, P, X, Y, Z);
         if (d->timing) d->collect();
@@ -811,6 +825,7 @@ int rh_dn_forward_static(rh_denoiser* d, const float* x, float* y, int P, int X,
     if (!d) return RH_DN_ERR_ARG;
     return guarded(d, [&] {
         if (!x || !y) throw ArgErr{"null volume"};
+        if ((uintptr_t)y % 16) throw ArgErr{"the output volume must be 16-byte aligned"};
         rh_denoiser::check_shape(P, X, Y, Z);
         if ((int64_t)P * X * Y * Z > RH_DN_STATIC_MAX_VOXELS)
             throw CapErr{"P*X*Y*Z = " + std::to_string((int64_t)P * X * Y * Z) +

@@ -602,6 +602,11 @@ struct FieldUpdateArgs {
     int full, write_cellB;
 };
 
+// 1/sqrt(x): the hardware reciprocal square root in FP32 (1 ulp), the
+// correctly rounded quotient in FP64
+__device__ __forceinline__ float inv_sqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double inv_sqrt(double x) { return 1.0 / sqrt(x); }
+
 // calcCellUe / calcNodeUe on values (ue_at above, NODE clamps r)
 template <typename Real, bool NODE>
 __device__ __forceinline__ void ue_core(const Real (&jj)[3], const Real (&ji)[3], Real& r, bool inner, int use_hall,
@@ -612,10 +617,8 @@ __device__ __forceinline__ void ue_core(const Real (&jj)[3], const Real (&ji)[3]
         return;
     }
     if (NODE && r < min_rho) { r = min_rho; c_rho = true; }
-    for (int l = 0; l < 3; ++l) {
-        if (fabs(r) > Real(0)) u[l] = use_hall ? -(jj[l] - ji[l]) / r : ji[l] / r;
-        else u[l] = Real(0);
-    }
+    const Real ir = fabs(r) > Real(0) ? Real(1) / r : Real(0);
+    for (int l = 0; l < 3; ++l) u[l] = use_hall ? -(jj[l] - ji[l]) * ir : ji[l] * ir;
     const Real u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
     if (u2 > max_ue2) {
         const Real norm = sqrt(max_ue2 / u2);
@@ -640,9 +643,12 @@ __device__ __forceinline__ void imf_values(const Geom& g, int i, int j, int k, i
 }
 
 // Threads of the fused update: a tile's loops (1331 halo faces, 1000 cells,
-// 729 nodes, 512 own cells for FP32 8x8x8 tiles) take a few rounds each.
+// 729 nodes, 512 own cells for FP32 8x8x8 tiles) take one or two rounds each
+// (768 threads: the node stage in one round; 133 us per update on the
+// 128x96x96 box against 151 us with 512 threads and 180 us with 8x8x4
+// tiles of 256 threads).
 template <typename Real> struct FieldUpdateThreads;
-template <> struct FieldUpdateThreads<float> { static constexpr int N = 512; };
+template <> struct FieldUpdateThreads<float> { static constexpr int N = 768; };
 template <> struct FieldUpdateThreads<double> { static constexpr int N = 256; };
 
 template <typename Real, bool FULL>
@@ -670,6 +676,7 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
     //    setIMFFace on faceB; the Neumann source's face2Cell with setIMFCell
     //    for cellB, and its cellRhoQi, cellJi).
     int en[R2];                       // cell index of the entry (-1: none or outside the grid)
+    int ei[R2], ej[R2], ek[R2];       // its grid coordinates
     Real lf[R2][3], lj[R2][3], lr[R2], le[R2];
     uint8_t lfl[R2];
 #pragma unroll
@@ -687,6 +694,7 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
         if (i < 0 || j < 0 || k < 0) continue;
         const int n = cell_index(g, i, j, k);
         en[r] = n;
+        ei[r] = i; ej[r] = j; ek[r] = k;
         lf[r][0] = F.x[n]; lf[r][1] = F.y[n]; lf[r][2] = F.z[n];
         if (lx >= 1 && ly >= 1 && lz >= 1) {
             lj[r][0] = a.Ji.x[n]; lj[r][1] = a.Ji.y[n]; lj[r][2] = a.Ji.z[n];
@@ -712,8 +720,7 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
         }
         const int n = en[r];
         if (n < 0) continue;
-        int i, j, k;
-        cell_ijk(g, n, i, j, k);
+        const int i = ei[r], j = ej[r], k = ek[r];
         if (all_nbrs(g, i, j, k)) continue;
         const int miss = missing_mask(g, i, j, k);
         Real f[3];
@@ -809,7 +816,7 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
                 const Real xc = off[t][0] ? Real(0.5) : Real(-0.5);
                 const Real yc = off[t][1] ? Real(0.5) : Real(-0.5);
                 const Real zc = off[t][2] ? Real(0.5) : Real(-0.5);
-                w[t] = Real(1) / sqrt((xc - xu) * (xc - xu) + (yc - yu) * (yc - yu) + (zc - zu) * (zc - zu));
+                w[t] = inv_sqrt((xc - xu) * (xc - xu) + (yc - yu) * (yc - yu) + (zc - zu) * (zc - zu));
                 wsum += w[t];
             }
             Real b[3];
@@ -820,7 +827,8 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
                     b1 += w[t] * sB[1][cc[t]];
                     b2 += w[t] * sB[2][cc[t]];
                 }
-                b[0] = b0 / wsum; b[1] = b1 / wsum; b[2] = b2 / wsum;
+                const Real iw = Real(1) / wsum;
+                b[0] = b0 * iw; b[1] = b1 * iw; b[2] = b2 * iw;
             } else {
                 b[0] = b[1] = b[2] = Real(0);
             }
@@ -898,13 +906,14 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
                         const Real yN = Real(0.25) * (N(0, -1, 0) + N(0, -1, -1) + N(-1, -1, -1) + N(-1, -1, 0));
                         const Real zP = Real(0.25) * (N(0, 0, 0) + N(0, -1, 0) + N(-1, -1, 0) + N(-1, 0, 0));
                         const Real zN = Real(0.25) * (N(0, 0, -1) + N(0, -1, -1) + N(-1, -1, -1) + N(-1, 0, -1));
-                        const Real dx = (Real)g.dx;
-                        const Real gx = (xP - xN) / dx, gy = (yP - yN) / dx, gz = (zP - zN) / dx;
+                        const Real idx = Real(1) / (Real)g.dx;
+                        const Real gx = (xP - xN) * idx, gy = (yP - yN) * idx, gz = (zP - zN) * idx;
                         const Real crho = sR[c1];
                         if (!a.isothermal) {
                             ep[0] = -a.e_coeff * gx; ep[1] = -a.e_coeff * gy; ep[2] = -a.e_coeff * gz;
                         } else if (fabs(crho) > Real(0)) {
-                            ep[0] = -a.e_coeff * gx / crho; ep[1] = -a.e_coeff * gy / crho; ep[2] = -a.e_coeff * gz / crho;
+                            const Real ic = Real(1) / crho;
+                            ep[0] = -a.e_coeff * gx * ic; ep[1] = -a.e_coeff * gy * ic; ep[2] = -a.e_coeff * gz * ic;
                         }
                     }
                     a.cellEp.x[n] = ep[0]; a.cellEp.y[n] = ep[1]; a.cellEp.z[n] = ep[2];
@@ -918,21 +927,21 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
             Real fb[3] = {sF[0][f], sF[1][f], sF[2][f]};
             if (!upper_missing(g, i, j, k)) {
                 const bool xn = has_neg(i, g.per[0]), yn = has_neg(j, g.per[1]), zn = has_neg(k, g.per[2]);
-                const Real dx = (Real)g.dx;
+                const Real idx = Real(1) / (Real)g.dx;
                 auto E = [&](int l, int ax, int ay, int az) { return nE[l][qn + ax + XN * (ay + YN * az)]; };
                 if (yn && zn) {
                     const Real curl = Real(0.5) * (+E(2, 0, -1, 0) + E(2, 0, -1, -1) - E(1, 0, -1, -1) - E(1, 0, 0, -1) -
-                                                   E(2, 0, 0, -1) - E(2, 0, 0, 0) + E(1, 0, 0, 0) + E(1, 0, -1, 0)) / dx;
+                                                   E(2, 0, 0, -1) - E(2, 0, 0, 0) + E(1, 0, 0, 0) + E(1, 0, -1, 0)) * idx;
                     fb[0] += a.dt * curl;
                 }
                 if (xn && zn) {
                     const Real curl = Real(0.5) * (-E(0, -1, 0, 0) - E(0, 0, 0, 0) + E(2, 0, 0, 0) + E(2, 0, 0, -1) +
-                                                   E(0, 0, 0, -1) + E(0, -1, 0, -1) - E(2, -1, 0, -1) - E(2, -1, 0, 0)) / dx;
+                                                   E(0, 0, 0, -1) + E(0, -1, 0, -1) - E(2, -1, 0, -1) - E(2, -1, 0, 0)) * idx;
                     fb[1] += a.dt * curl;
                 }
                 if (xn && yn) {
                     const Real curl = Real(0.5) * (-E(0, -1, -1, 0) - E(0, 0, -1, 0) - E(1, 0, -1, 0) - E(1, 0, 0, 0) +
-                                                   E(0, 0, 0, 0) + E(0, -1, 0, 0) + E(1, -1, 0, 0) + E(1, -1, -1, 0)) / dx;
+                                                   E(0, 0, 0, 0) + E(0, -1, 0, 0) + E(1, -1, 0, 0) + E(1, -1, -1, 0)) * idx;
                     fb[2] += a.dt * curl;
                 }
             }

@@ -9,7 +9,7 @@ running it in the loop on the uneven-load geometry.
   2. training on the GPU by the library's kernels (Adam, BF16 MFMA);
   3. the denoiser in the loop on the uneven-load geometry against the plain
      method at 8, 64 and 256 ppc (denoise_loop.compare_uneven), and the
-     judge's FP32-vs-FP64 precision on a sub-box.
+     truth run's FP32-vs-FP64 precision on a sub-box.
 
     python -m rhybrid_amd.denoise_experiment [--patches 20000] [--steps 2000] [--out file.json]
 """
@@ -35,11 +35,19 @@ def log(msg: str) -> None:
 
 def run(train_box=(128, 96, 96), R_train=10.0, hi_ppc=256, lo_ppc=8, patches=20000, crops=36, warm=300, every=2,
         heldout_patches=1152, epochs=8, batch=8, lr=2e-3, lr_final=1e-4, eval_box=(256, 192, 192),
-        R_eval=12.0, H_eval=5.0, steps=2000, every_eval=250, bound=0.5, judge_box=(64, 48, 48),
-        judge_steps=100, weights_out=None):
+        R_eval=12.0, H_eval=5.0, steps=2000, every_eval=250, bound=0.5, truth_box=(64, 48, 48),
+        truth_steps=100, weights_out=None, train_geometry="obstacle"):
     t_all = time.perf_counter()
     nx, ny, nz = train_box
-    make = lambda ppc, seed: P.config_obstacle_filled(nx=nx, ny=ny, nz=nz, ppc=ppc, seed=seed, R_cells=R_train)
+    if train_geometry == "obstacle":
+        # config 4: pairs from the inflow/outflow obstacle box (H+ and He++)
+        make = lambda ppc, seed: P.config_obstacle_filled(nx=nx, ny=ny, nz=nz, ppc=ppc, seed=seed, R_cells=R_train)
+    else:
+        # pairs from the uneven-load geometry itself (H+ and exospheric O+),
+        # scaled to the training box
+        make = lambda ppc, seed: P.config_exosphere(nx=nx, ny=ny, nz=nz, ppc=ppc, R_cells=R_train,
+                                                    H_cells=H_eval * R_train / R_eval, seed=seed, o_per_ppc=True,
+                                                    steps=warm + 4 * patches // crops)
     scales = DD.channel_scales(make(1, 1))
     snapshots = -(-patches // crops)
     X, T, tp = DT.paired_runs(make, hi_ppc, lo_ppc, seed_hi=101, seed_lo=202, warm=warm, snapshots=snapshots,
@@ -61,9 +69,9 @@ def run(train_box=(128, 96, 96), R_train=10.0, hi_ppc=256, lo_ppc=8, patches=200
     import torch
     torch.cuda.empty_cache()
     flops = 3 * D.forward_flops(DT.PATCH ** 3) * patches * epochs
-    training = {"pairs": "%dx%dx%d obstacle box (R=%g cells), %d-ppc target and independent %d-ppc input runs, "
+    training = {"pairs": "%dx%dx%d %s box (R=%g cells), %d-ppc target and independent %d-ppc input runs, "
                          "%d warm-up steps, snapshots every %d steps, %d crops of 32^3 each"
-                         % (nx, ny, nz, R_train, hi_ppc, lo_ppc, warm, every, crops),
+                         % (nx, ny, nz, train_geometry, R_train, hi_ppc, lo_ppc, warm, every, crops),
                 "patches": patches, "heldout_patches": heldout_patches, "epochs": epochs, "batch": batch,
                 "epoch_loss": tr["epoch_loss"], "seconds_pairs": tp["seconds_total"], "seconds_train": tr["seconds"],
                 "train_TFLOPs": flops / tr["seconds"] / 1e12,
@@ -74,10 +82,10 @@ def run(train_box=(128, 96, 96), R_train=10.0, hi_ppc=256, lo_ppc=8, patches=200
     loop = DL.compare_uneven(dn, nx=ex, ny=ey, nz=ez, R_cells=R_eval, H_cells=H_eval, steps=steps,
                              every=every_eval, bound=bound, log=log)
     dn.close()
-    jx, jy, jz = judge_box
-    judge = DL.judge_precision(nx=jx, ny=jy, nz=jz, R_cells=R_eval * jx / ex, H_cells=H_eval * jx / ex,
-                               steps=judge_steps)
-    return {"training": training, "in_loop": loop, "judge_fp32_vs_fp64": judge,
+    jx, jy, jz = truth_box
+    truth = DL.truth_precision(nx=jx, ny=jy, nz=jz, R_cells=R_eval * jx / ex, H_cells=H_eval * jx / ex,
+                               steps=truth_steps)
+    return {"training": training, "in_loop": loop, "truth_fp32_vs_fp64": truth,
             "seconds_total": time.perf_counter() - t_all}
 
 
@@ -95,14 +103,15 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--every", type=int, default=250)
     ap.add_argument("--bound", type=float, default=0.5)
-    ap.add_argument("--judge-box", type=int, nargs=3, default=[64, 48, 48])
+    ap.add_argument("--truth-box", type=int, nargs=3, default=[64, 48, 48])
     ap.add_argument("--weights-out", default=None)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--train-geometry", choices=("obstacle", "exosphere"), default="obstacle")
     a = ap.parse_args()
     res = run(train_box=tuple(a.train_box), R_train=a.R_train, patches=a.patches, heldout_patches=a.heldout,
               warm=a.warm, epochs=a.epochs, eval_box=tuple(a.eval_box), R_eval=a.R_eval, H_eval=a.H_eval,
-              steps=a.steps, every_eval=a.every, bound=a.bound, judge_box=tuple(a.judge_box),
-              weights_out=a.weights_out)
+              steps=a.steps, every_eval=a.every, bound=a.bound, truth_box=tuple(a.truth_box),
+              weights_out=a.weights_out, train_geometry=a.train_geometry)
     line = json.dumps(res)
     if a.out:
         with open(a.out, "w") as f:

@@ -155,26 +155,26 @@ def compare_uneven(dn, nx=256, ny=192, nz=192, R_cells=12.0, H_cells=5.0, steps=
     """The denoiser in the loop on the uneven-load geometry (config_exosphere:
     H+ solar wind at ppc, O+ photoions in proportion): plain FP32 runs at each
     of `ppcs` and a denoised FP32 run at the lowest, from the same seeded
-    distribution (device fill), against the highest-ppc run as the judge;
+    distribution (device fill), against the highest-ppc run as the truth;
     n and J errors over the plasma cells every `every` steps, time per step,
     fallback fraction."""
     make = lambda ppc: P.config_exosphere(nx=nx, ny=ny, nz=nz, ppc=ppc, R_cells=R_cells, H_cells=H_cells,
                                           seed=seed, o_per_ppc=True, steps=steps)
     runs = {}
-    judge_ppc, low = max(ppcs), min(ppcs)
+    truth_ppc, low = max(ppcs), min(ppcs)
     for ppc in sorted(ppcs, reverse=True):
         runs["fp32_%d" % ppc] = _checkpointed_run(make(ppc), RH_FP32, steps, every, log=log, name="fp32_%d" % ppc)
     runs["fp32_%d_denoised" % low] = _checkpointed_run(make(low), RH_FP32, steps, every, dn=dn, bound=bound,
                                                        log=log, name="denoised")
     g = make(1)
     plasma = g.grid.all_neighbours() & (g.flags["innerFlagField"] == 0)
-    judge = runs["fp32_%d" % judge_ppc]["checkpoints"]
+    truth = runs["fp32_%d" % truth_ppc]["checkpoints"]
     out = {"geometry": "%dx%dx%d cells, exospheric O+ around an absorbing sphere of %g cells, H+ solar wind; "
                        "%d steps" % (nx, ny, nz, R_cells, steps),
-           "judge": "plain FP32 at %d ppc" % judge_ppc, "fallback_bound": bound, "runs": {}}
+           "truth": "plain FP32 at %d ppc" % truth_ppc, "fallback_bound": bound, "runs": {}}
     for name, r in runs.items():
         rows = []
-        for (st, rho, J), (_, rho_j, J_j) in zip(r["checkpoints"], judge):
+        for (st, rho, J), (_, rho_j, J_j) in zip(r["checkpoints"], truth):
             en = np.linalg.norm((rho - rho_j)[plasma]) / np.linalg.norm(rho_j[plasma])
             eJ = np.linalg.norm((J - J_j)[plasma]) / np.linalg.norm(J_j[plasma])
             rows.append({"step": st, "n_error": float(en), "J_error": float(eJ)})
@@ -195,8 +195,8 @@ def compare_uneven(dn, nx=256, ny=192, nz=192, R_cells=12.0, H_cells=5.0, steps=
     return out
 
 
-def judge_precision(nx=64, ny=48, nz=48, R_cells=6.0, H_cells=2.5, ppc=256, steps=100, seed=23):
-    """The judge's precision: FP32 against FP64 at the judge's ppc on a
+def truth_precision(nx=64, ny=48, nz=48, R_cells=6.0, H_cells=2.5, ppc=256, steps=100, seed=23):
+    """The truth run's precision: FP32 against FP64 at the truth run's ppc on a
     sub-box of the uneven-load geometry, from the same device fill."""
     from .denoise_train import filled_context
     pr = P.config_exosphere(nx=nx, ny=ny, nz=nz, ppc=ppc, R_cells=R_cells, H_cells=H_cells, seed=seed,
