@@ -968,7 +968,8 @@ public:
             a.write_cellB = dn_attached != nullptr;
             a.aveB = ave_on ? aveB.p : nullptr;
             if (ave_on) ++ave_count;
-            const dim3 grid(cdiv(g.nx, FieldTile<Real>::X), cdiv(g.ny, FieldTile<Real>::Y), cdiv(g.nz, FieldTile<Real>::Z));
+            const int nft = cdiv(g.nx, FieldTile<Real>::X) * cdiv(g.ny, FieldTile<Real>::Y) * cdiv(g.nz, FieldTile<Real>::Z);
+            const dim3 grid(N_XCD * cdiv(nft, N_XCD));
             constexpr int nt = FieldUpdateThreads<Real>::N;
             if (full) launch("field_update", k_field_update<Real, true>, grid, nt, 0, g, a);
             else launch("field_update", k_field_update<Real, false>, grid, nt, 0, g, a);

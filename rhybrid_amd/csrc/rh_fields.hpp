@@ -667,7 +667,12 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
     __shared__ Real sEta[N1];
     __shared__ uint8_t sFl[N1];
     const int tid = threadIdx.x;
-    const int ox = blockIdx.x * TX, oy = blockIdx.y * TY, oz = blockIdx.z * TZ;
+    // tiles x-fastest, dealt to the XCDs in contiguous ranges: neighbouring
+    // tiles, which read the same halo lines, share an L2 and run close in time
+    const int ntx = (g.nx + TX - 1) / TX, nty = (g.ny + TY - 1) / TY, ntz = (g.nz + TZ - 1) / TZ;
+    const int T = xcd_index(ntx * nty * ntz, (int)blockIdx.x);
+    if (T >= ntx * nty * ntz) return;
+    const int ox = (T % ntx) * TX, oy = ((T / ntx) % nty) * TY, oz = (T / (ntx * nty)) * TZ;
     const V3<Real>& F = a.faceB_in;
 
     // 1. faceB, cellRhoQi, cellJi, nodeEta and the inner flags of the halo.

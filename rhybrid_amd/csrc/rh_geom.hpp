@@ -113,6 +113,12 @@ RH_HD int xcd_tile(const Geom& g, int b) {
     return (b % N_XCD) * chunk + b / N_XCD;
 }
 
+// The same dealing for any count of tiles (n), e.g. the field update's.
+RH_HD int xcd_index(int n, int b) {
+    const int chunk = (n + N_XCD - 1) / N_XCD;
+    return (b % N_XCD) * chunk + b / N_XCD;
+}
+
 RH_HD void tile_coords(const Geom& g, int T, int& bx, int& by, int& bz) {
     bx = T % g.ntx;
     by = (T / g.ntx) % g.nty;
