@@ -77,14 +77,18 @@ PROFILE_KERNELS = {"push": "rh::k_push_move<float>_[pass_0:_push]", "move": "rh:
                    "sort_slots": "rh::k_slot_offsets"}
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
+def measured_traffic(kernel: str, profile: str = "hbm_traffic_obstacle128x96x96_ppc32_inflow.txt",
+                     species: str = ""):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, each scaled by the factor a 1 GiB copy
-    gives); None if the summary does not list the kernel."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_obstacle128x96x96_ppc32_inflow.txt")
+    gives; `species` selects a per-species line "<kernel>:<species>"); None if
+    the summary does not list the kernel."""
+    path = os.path.join(ROOT, "profiles", profile)
     name = PROFILE_KERNELS.get(kernel)
     if name is None or not os.path.exists(path):
         return None
+    if species:
+        name += ":" + species
     for line in open(path):
         if line.startswith("#"):
             continue
@@ -477,7 +481,7 @@ def error_curve(pr, steps: int = 500, every: int = 50, order_tile=(2, 4, 8)):
             "tile": list(BENCH_TILE), "rows": rows}
 
 
-def exosphere_bench(steps: int = 10, warmup: int = 2, prefill: int = 600, tune_shapes: int = 4):
+def exosphere_bench(steps: int = 10, warmup: int = 2, prefill: int = 600, tune_shapes: int = 4, tile=None):
     """The uneven-load box (256x192x192 cells, dx=200 km, H+ at 24 ppc with
     +x-wall inflow, O+ photoions of an exponential corona around an absorbing
     sphere of 12 cells): tile shape from the autotuner, H+ filled on the device
@@ -492,10 +496,17 @@ def exosphere_bench(steps: int = 10, warmup: int = 2, prefill: int = 600, tune_s
 
     def prepare(ctx):
         ctx.fill(0, pr.meta["fill"][0])
-        ctx.fill(1, pr.injectors[1], timesteps=range(1, prefill + 1))
+        for t in range(1, prefill + 1, 100):  # in parts, each reported (long under a profiler)
+            ctx.fill(1, pr.injectors[1], timesteps=range(t, min(t + 100, prefill + 1)))
+            ctx.synchronize()
+            print("# exosphere prefill: O+ injection steps %d-%d done" % (t, min(t + 99, prefill)),
+                  file=sys.stderr, flush=True)
 
-    shapes = autotune.candidates(pr.grid)[:tune_shapes]
-    tuned = autotune.tune(pr, steps=3, shapes=shapes, prepare=prepare)
+    if tile is None:
+        shapes = autotune.candidates(pr.grid)[:tune_shapes]
+        tuned = autotune.tune(pr, steps=3, shapes=shapes, prepare=prepare)
+    else:  # a given tile shape (profiled runs): no autotuning
+        tuned = {"best": tuple(tile), "ms_per_step": None}
     ctx = Context(pr, precision=RH_FP32, tile=tuned["best"])
     t0 = time.time()
     prepare(ctx)
@@ -542,7 +553,8 @@ def exosphere_bench(steps: int = 10, warmup: int = 2, prefill: int = 600, tune_s
     dk = per_species[dom_sp]["kernels"][dom_nm]
     dom_bytes = kernel_bytes(dom_nm, ctx, 0.0, 1, npp=per_species[dom_sp]["particles"])
     roof = {"bound": "hbm", "kernel": "%s (%s)" % (dom_nm, dom_sp), "achieved": dk["GBps"],
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dk["GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dk["GBps"] / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(dom_nm, "hbm_traffic_exosphere256x192x192.txt", dom_sp),
             "algorithmic_bytes": dom_bytes}
     ctx.close()
     np_mean = 0.5 * (sum(n0) + sum(n1))
@@ -585,6 +597,8 @@ def main():
     ap.add_argument("--only-denoiser", action="store_true", help="the denoiser section alone (its kernels' profile)")
     ap.add_argument("--no-exosphere", action="store_true", help="skip the uneven-load exosphere box")
     ap.add_argument("--no-error-curve", action="store_true", help="skip the 500-step FP32-vs-FP64 error curve")
+    ap.add_argument("--only-exosphere", action="store_true", help="the uneven-load box alone (its kernels' profile)")
+    ap.add_argument("--exosphere-tile", type=int, nargs=3, default=None, help="skip the autotuner, use this tile")
     ap.add_argument("--calibration-copy-gib", type=float, default=0.0,
                     help="first copy this many GiB device to device (a traffic counter calibration dispatch)")
     args = ap.parse_args()
@@ -592,6 +606,12 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (one domain per GPU; "
               "the halo and migration exchange between GPUs is not built).", file=sys.stderr)
         sys.exit(2)
+    if args.only_exosphere:
+        if args.calibration_copy_gib > 0:
+            calibration_copy(args.calibration_copy_gib)
+        print(json.dumps({"exosphere_box": exosphere_bench(steps=args.steps, warmup=args.warmup,
+                                                           tile=args.exosphere_tile)}), flush=True)
+        return
     if args.only_denoiser:
         print(json.dumps({"denoiser": denoiser_bench(), "forward_volume": denoiser_forward_volume()}), flush=True)
         return
@@ -611,7 +631,7 @@ def main():
     exo = None
     if not args.no_exosphere:
         try:
-            exo = exosphere_bench()
+            exo = exosphere_bench(tile=args.exosphere_tile)
         except Exception as e:  # reported in the line, the headline number stands
             exo = {"error": repr(e)}
     cpu, err = None, None
