@@ -207,6 +207,9 @@ public:
         DevBuf<uint32_t> sub, sub_new;
         DevBuf<uint16_t> cnt_tile;
         DevBuf<Real> partial;  // deposit: per-tile ghost partials (ntiles x next x 4)
+        // per tile, 1 if it held particles at the push (its slot table was
+        // written)
+        DevBuf<uint8_t> live_src;
         DevBuf<double> ave_n, ave_v;  // temporal averages: running charge and current sums (FP64)
         bool deposited = false;
         bool pushed = false;  // pass 0 done, pass 1 pending
@@ -526,6 +529,8 @@ public:
         RH_HIP(hipMemsetAsync(s->sub.p, 0, sizeof(uint32_t) * ((size_t)g.nkeys + 1), stream));
         s->cnt_tile.alloc((size_t)g.ntiles * g.extk);
         s->partial.alloc((size_t)g.ntiles * g.next * 4);
+        s->live_src.alloc((size_t)g.ntiles);
+        fill(s->live_src.p, (uint8_t)0, s->live_src.n);
         s->pcount.alloc((size_t)C_N * COUNTER_SLOTS);
         RH_HIP(hipMemsetAsync(s->pcount.p, 0, sizeof(unsigned long long) * C_N * COUNTER_SLOTS, stream));
         s->bc_acc.alloc((size_t)std::max(n_bc, 1) * 4);
@@ -697,6 +702,7 @@ public:
         pp.use_gravity = P.use_gravity;
         pp.initial_flow_through = P.initial_flow_through;
         pp.still = 0;
+        pp.live = S_.live_src.p;
         prepare_push_fields();
         launch("push", k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, pp,
                (const uint32_t*)pf_tile.p,
@@ -713,6 +719,7 @@ public:
         if (S_.pushed) throw StateError("injected particles pending between propagate and wait_particle_sends");
         PushParams pp{};
         pp.still = 1;
+        pp.live = S_.live_src.p;
         prepare_push_fields();
         launch("sort_still", k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, pp,
                (const uint32_t*)pf_tile.p, S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.appendix(), S_.cnt_tile.p,
@@ -745,7 +752,7 @@ public:
     void finish_sort(Species& S_, const char* move_name) {
         // totals and in-subcell offsets in one pass, then the scan of the totals
         launch("sort_slots", k_slot_offsets, cdiv((uint32_t)g.ntiles * (uint32_t)g.tc, (uint32_t)SLOT_THREADS),
-               SLOT_THREADS, 0, g, S_.cnt_tile.p, S_.sub_new.p, dcount.p);
+               SLOT_THREADS, 0, g, S_.cnt_tile.p, (const uint8_t*)S_.live_src.p, S_.sub_new.p, dcount.p);
         scan(S_.sub_new.p, g.nkeys, S_.sub_new.p, 0);
         launch(move_name, k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, S_.pp,
                (const uint32_t*)pf_tile.p,

@@ -39,6 +39,7 @@ struct PushParams {
     double GMdt;
     int accelerate, use_e_pressure, use_gravity, initial_flow_through;
     int still;           // sort only: no acceleration, no move
+    uint8_t* live;       // pass 0 writes, per tile, whether it holds particles
 };
 
 enum DevCounter { C_MAX_VI = 0, C_MIN_CELL_RHO, C_MAX_CELL_UE, C_MIN_NODE_RHO, C_MAX_NODE_UE,
@@ -221,14 +222,11 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
         if (c1 > c0) { a0 = sub[g.nkeys] + app.off[c0]; na = app.off[c1] - app.off[c0]; }
     }
     const uint32_t n0 = p1 - p0, ntot = n0 + na;
-    if (ntot == 0) {
-        // an empty tile (common for a species confined to part of the grid):
-        // nothing to push or move; pass 0 leaves zero slot counts
-        if (pass == 0)
-            for (int ec = tid; ec < g.next; ec += PM_THREADS)
-                *reinterpret_cast<uint4*>(cnt_tile + ((size_t)T * g.next + ec) * 8) = make_uint4(0u, 0u, 0u, 0u);
-        return;
-    }
+    if (pass == 0 && tid == 0) pp.live[T] = ntot > 0 ? 1 : 0;
+    // an empty tile (common for a species confined to part of the grid):
+    // nothing to push or move, and its slot table is not written (the slot
+    // kernel reads the tables of live tiles only)
+    if (ntot == 0) return;
     const uint32_t deadkey = (uint32_t)MAX_EXTK;
     // prefetch of the first chunk; an injected particle beyond the store's
     // capacity was never written (the injection counted the overflow)
@@ -462,6 +460,7 @@ __device__ __forceinline__ uint4 slot_offsets8(uint32_t (&run)[8], uint4 v) {
 constexpr int SLOT_THREADS = 256;
 
 __global__ __launch_bounds__(SLOT_THREADS) void k_slot_offsets(Geom g, uint16_t* __restrict__ cnt_tile,
+                                                              const uint8_t* __restrict__ live,
                                                               uint32_t* __restrict__ out,
                                                               unsigned long long* __restrict__ counters) {
     const uint32_t id = blockIdx.x * (uint32_t)SLOT_THREADS + threadIdx.x;  // = tile * tc + local cell
@@ -483,8 +482,10 @@ __global__ __launch_bounds__(SLOT_THREADS) void k_slot_offsets(Geom g, uint16_t*
         const int mx = axis_candidates_fast(i, g.ltx, g.ntx, g.per[0], tx, ex);
         const int my = axis_candidates_fast(j, g.lty, g.nty, g.per[1], ty, ey);
         const int mz = axis_candidates_fast(k, g.ltz, g.ntz, g.per[2], tz, ez);
+        // (an empty candidate tile has no slot table: it adds nothing)
         uint32_t slot[8];
         uint4 v[8];
+        bool use[8];
 #pragma unroll
         for (int c = 0; c < 2; ++c)
 #pragma unroll
@@ -492,21 +493,15 @@ __global__ __launch_bounds__(SLOT_THREADS) void k_slot_offsets(Geom g, uint16_t*
 #pragma unroll
                 for (int a = 0; a < 2; ++a) {
                     const int q = a + 2 * b + 4 * c;
-                    const bool ok = a < mx && b < my && c < mz;
-                    const int TT = ok ? tx[a] + g.ntx * (ty[b] + g.nty * tz[c]) : T;
-                    slot[q] = (uint32_t)TT * (uint32_t)g.next + (uint32_t)(ok ? ex[a] + g.ex * (ey[b] + g.ey * ez[c]) : 0);
-                    v[q] = ok ? tab[slot[q]] : make_uint4(0u, 0u, 0u, 0u);
+                    const bool inb = a < mx && b < my && c < mz;
+                    const int TT = inb ? tx[a] + g.ntx * (ty[b] + g.nty * tz[c]) : T;
+                    use[q] = inb && live[TT];
+                    slot[q] = (uint32_t)TT * (uint32_t)g.next + (uint32_t)(inb ? ex[a] + g.ex * (ey[b] + g.ey * ez[c]) : 0);
+                    v[q] = use[q] ? tab[slot[q]] : make_uint4(0u, 0u, 0u, 0u);
                 }
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int a = 0; a < 2; ++a) {
-                    const int q = a + 2 * b + 4 * c;
-                    if (!(a < mx && b < my && c < mz)) continue;
-                    tab[slot[q]] = slot_offsets8(run, v[q]);  // offsets inside the subcells' ranges
-                }
+        for (int q = 0; q < 8; ++q)
+            if (use[q]) tab[slot[q]] = slot_offsets8(run, v[q]);  // offsets inside the subcells' ranges
     } else {
         const int mx = fx ? axis_candidates_fast(i, g.ltx, g.ntx, g.per[0], tx, ex)
                           : axis_candidates(i, g.nx, g.ltx, g.ntx, g.per[0], tx, ex);
@@ -518,6 +513,7 @@ __global__ __launch_bounds__(SLOT_THREADS) void k_slot_offsets(Geom g, uint16_t*
             for (int b = 0; b < my; ++b)
                 for (int a = 0; a < mx; ++a) {
                     const int TT = tx[a] + g.ntx * (ty[b] + g.nty * tz[c]);
+                    if (!live[TT]) continue;
                     const size_t slot = (size_t)TT * g.next + (size_t)(ex[a] + g.ex * (ey[b] + g.ey * ez[c]));
                     tab[slot] = slot_offsets8(run, tab[slot]);
                 }
@@ -796,7 +792,9 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
     if (T >= g.ntiles) return;
     const uint32_t t0 = sub[(size_t)T * NS], t1 = sub[(size_t)(T + 1) * NS];
     if (t1 == t0) {
-        // an empty tile: zero ghost partials (the fold reads every tile's)
+        // an empty tile: zero ghost partials (the fold reads every tile's;
+        // a per-tile flag that let the fold skip them cost it more, 35 -> 45
+        // us on the dense box, than the zero stores cost here)
         Real4<Real>* out = reinterpret_cast<Real4<Real>*>(partial) + (size_t)T * g.next;
         Real4<Real> z;
         z.v[0] = z.v[1] = z.v[2] = z.v[3] = Real(0);
