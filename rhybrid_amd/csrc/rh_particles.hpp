@@ -59,6 +59,22 @@ __device__ __forceinline__ void count_event(unsigned long long* counters, int id
     }
 }
 
+// The same for per-lane event counts, once per wave (every lane of the wave
+// calls it)
+__device__ __forceinline__ void count_events(unsigned long long* counters, int id, uint32_t n) {
+    if (!__ballot(n != 0u)) return;
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+    if (((threadIdx.x + blockDim.x * threadIdx.y) & 63) == 0) {
+        const unsigned slot = (blockIdx.x + 7u * blockIdx.y + 13u * blockIdx.z) & (COUNTER_SLOTS - 1);
+        atomicAdd(counters + slot * C_N + id, (unsigned long long)n);
+    }
+}
+
+// 2 / (1 + t^2) of the Boris rotation: the hardware reciprocal (1 ulp) in
+// FP32, the correctly rounded quotient in FP64
+__device__ __forceinline__ float boris_s_factor(float t2) { return 2.0f * __builtin_amdgcn_rcpf(1.0f + t2); }
+__device__ __forceinline__ double boris_s_factor(double t2) { return 2.0 / (1.0 + t2); }
+
 template <typename Real> __device__ __forceinline__ Real below_one();
 template <> __device__ __forceinline__ float below_one<float>() { return 0x1.fffffep-1f; }
 template <> __device__ __forceinline__ double below_one<double>() { return 0x1.fffffffffffffp-1; }
@@ -277,6 +293,7 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
         for (int q = tid; q < nw4; q += PM_THREADS) dst[q] = src[q];
     }
     __syncthreads();
+    uint32_t n_clamped = 0, n_dropped = 0;  // this lane's events (pass 0)
     for (uint32_t ub = 0; ub < ntot; ub += PM_THREADS) {
         const uint32_t u = ub + tid;
         const bool act = u < ntot;
@@ -296,7 +313,7 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
         }
         uint32_t key = deadkey;
         bool clamped = false, dropped = false;
-        uint8_t nlcell = 0;  // destination cell inside the destination tile
+        int di = 0, dj = 0, dk = 0;  // destination cell, local coordinates of the source tile
         if (act && X >= Real(0)) {
             int li, lj, lk;
             local_ijk(g, lc, li, lj, lk);
@@ -318,7 +335,7 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
                 const Real dvx = qm * Ex, dvy = qm * Ey, dvz = qm * Ez;
                 const Real tx = qm * Bx, ty = qm * By, tz = qm * Bz;
                 const Real t2 = tx * tx + ty * ty + tz * tz;
-                const Real b2 = Real(2) / (Real(1) + t2);
+                const Real b2 = boris_s_factor(t2);
                 const Real sx = b2 * tx, sy = b2 * ty, sz = b2 * tz;
                 const Real vmx = vx + dvx, vmy = vy + dvy, vmz = vz + dvz;
                 const Real v0x = vmx + vmy * tz - vmz * ty;
@@ -363,10 +380,8 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
             }
             if (ok && ((nm >> ((sx + 1) + 3 * (sy + 1) + 9 * (sz + 1))) & 1u)) {
                 const int o = (X >= Real(0.5)) + 2 * (Y >= Real(0.5)) + 4 * (Z >= Real(0.5));
-                key = (uint32_t)(((li + sx + 1) + g.ex * ((lj + sy + 1) + g.ey * (lk + sz + 1))) * 8 + o);
-                // tiles divide the grid, so a periodic wrap keeps the local index
-                nlcell = (uint8_t)(((li + sx) & (g.tx - 1)) + (((lj + sy) & (g.ty - 1)) << g.ltx) +
-                                   (((lk + sz) & (g.tz - 1)) << (g.ltx + g.lty)));
+                di = li + sx; dj = lj + sy; dk = lk + sz;
+                key = (uint32_t)(((di + 1) + g.ex * ((dj + 1) + g.ey * (dk + 1))) * 8 + o);
             } else {
                 dropped = true;
             }
@@ -380,12 +395,16 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
         // loads outstanding.)
         asm volatile("" ::"v"(nX), "v"(nY), "v"(nZ), "v"(nvx), "v"(nvy), "v"(nvz), "v"(nlc));
         if (pass == 0) {
-            // counts only: integer LDS atomics, so the order does not matter
+            // counts only: integer LDS atomics, so the order does not matter;
+            // the events are counted once per tile
             if (key != deadkey) atomicAdd(&cnt[key], 1u);
-            count_event(counters, C_MAX_VI, clamped);
-            count_event(counters, C_DROPPED, dropped);
+            n_clamped += clamped;
+            n_dropped += dropped;
             continue;
         }
+        // tiles divide the grid, so a periodic wrap keeps the local index
+        const uint8_t nlcell = (uint8_t)((di & (g.tx - 1)) + ((dj & (g.ty - 1)) << g.ltx) +
+                                         ((dk & (g.tz - 1)) << (g.ltx + g.lty)));
         // rank among this wave's lanes holding the same slot
         unsigned long long match = __ballot(key != deadkey);
         for (int b = 0; b < kbits; ++b) {
@@ -418,6 +437,10 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
         __syncthreads();
     }
     __syncthreads();
+    if (pass == 0) {
+        count_events(counters, C_MAX_VI, n_clamped);
+        count_events(counters, C_DROPPED, n_dropped);
+    }
     if (pass == 0)
         for (int ec = tid; ec < g.next; ec += PM_THREADS) {
             const uint4 a = cnt4[2 * ec], b = cnt4[2 * ec + 1];
