@@ -261,8 +261,10 @@ public:
     std::vector<std::unique_ptr<Species>> sp;
     std::vector<DevBuf<uint32_t>> scan_lv;  // per level: block sums
     DevBuf<uint32_t> zero_u32;
-    DevBuf<uint16_t> dep_list;   // deposit gather lists (next x 64)
+    DevBuf<uint16_t> dep_list;   // deposit gather lists of subcell terms (next x 64; crowded tiles)
     DevBuf<uint8_t> dep_count;
+    DevBuf<uint16_t> dep_tlist;  // deposit gather lists of cell-target terms (next x 32)
+    DevBuf<uint8_t> dep_tcount;
     int kbits = 12;
     bool timing = false;
     std::string timing_only;  // when not empty, only launches of this name are timed
@@ -326,6 +328,13 @@ public:
             dep_count.alloc(hc.size());
             RH_HIP(hipMemcpy(dep_list.p, hl.data(), hl.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
             RH_HIP(hipMemcpy(dep_count.p, hc.data(), hc.size(), hipMemcpyHostToDevice));
+            std::vector<uint16_t> tl((size_t)g.next * 32, 0);
+            std::vector<uint8_t> tc((size_t)g.next, 0);
+            deposit_target_lists(g, tl.data(), tc.data());
+            dep_tlist.alloc(tl.size());
+            dep_tcount.alloc(tc.size());
+            RH_HIP(hipMemcpy(dep_tlist.p, tl.data(), tl.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+            RH_HIP(hipMemcpy(dep_tcount.p, tc.data(), tc.size(), hipMemcpyHostToDevice));
         }
         zero_u32.alloc(1);
         RH_HIP(hipMemsetAsync(zero_u32.p, 0, sizeof(uint32_t), stream));
@@ -779,8 +788,8 @@ public:
         // subcell the one-lane path is faster, at 16 three times slower)
         const uint32_t dense_min = 10u * (uint32_t)NS;
         launch("deposit", k_deposit<Real>, tile_grid(), NS, deposit_lds_bytes<Real>(g), g, S_.A(),
-               (const uint32_t*)S_.sub.p, (const uint16_t*)dep_list.p, (const uint8_t*)dep_count.p, S_.partial.p,
-               dense_min);
+               (const uint32_t*)S_.sub.p, (const uint16_t*)dep_list.p, (const uint8_t*)dep_count.p,
+               (const uint16_t*)dep_tlist.p, (const uint8_t*)dep_tcount.p, S_.partial.p, dense_min);
         S_.deposited = true;
     }
 

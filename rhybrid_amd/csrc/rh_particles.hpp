@@ -720,6 +720,73 @@ inline int deposit_gather_lists(const Geom& g, uint16_t* list, uint8_t* count) {
     return 0;
 }
 
+// The one-lane-per-subcell path first combines, in registers, the corner
+// sums of a cell's eight octant lanes into the cell's 27 targets (cell + d,
+// d in {-1, 0, 1}^3), then gathers for every cell of the extended region the
+// (up to 27) cell terms that reach it.  LDS: [quantity][target][cell], cell
+// blocks of deposit_octant_block words.  The lists hold, for every cell of
+// the extended region, the offsets (target, cell) of its terms in ascending
+// order, at most 27 per cell (stride 32); built once on the host.
+inline int deposit_target_lists(const Geom& g, uint16_t* list, uint8_t* count) {
+    const int CB = deposit_octant_block(g);
+    for (int e = 0; e < g.next; ++e) {
+        const int lx = e % g.ex - 1, ly = (e / g.ex) % g.ey - 1, lz = e / (g.ex * g.ey) - 1;
+        int m = 0;
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int cz = lz - dz;
+            if (cz < 0 || cz >= g.tz) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int cy = ly - dy;
+                if (cy < 0 || cy >= g.ty) continue;
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int cx = lx - dx;
+                    if (cx < 0 || cx >= g.tx) continue;
+                    const int t27 = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+                    list[e * 32 + m++] = (uint16_t)(t27 * CB + cx + g.tx * (cy + g.ty * cz));
+                }
+            }
+        }
+        for (int a = 1; a < m; ++a)
+            for (int b = a; b > 0 && list[e * 32 + b - 1] > list[e * 32 + b]; --b) {
+                const uint16_t t = list[e * 32 + b];
+                list[e * 32 + b] = list[e * 32 + b - 1];
+                list[e * 32 + b - 1] = t;
+            }
+        count[e] = (uint8_t)m;
+    }
+    return 0;
+}
+
+// The value of lane ^ M (M = 1, 2, 4: inside the 8 lanes of one cell's
+// octants): DPP quad permutations for 1 and 2, a swizzle in xor mode for 4.
+template <int M>
+__device__ __forceinline__ int octant_xor_bits(int v) {
+    if constexpr (M == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    else if constexpr (M == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    else return __builtin_amdgcn_ds_swizzle(v, (M << 10) | 0x1F);  // and 31, or 0, xor M
+}
+template <int M>
+__device__ __forceinline__ float octant_xor(float v) { return __int_as_float(octant_xor_bits<M>(__float_as_int(v))); }
+template <int M>
+__device__ __forceinline__ double octant_xor(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = octant_xor_bits<M>((int)(b & 0xFFFFFFFFll)), hi = octant_xor_bits<M>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// One exchange stage along an axis with octant bit `bit` (lane mask M):
+// pairs (c0, c1) of corner 0 and 1 along the axis.  The lane with bit 0 keeps
+// target -1 (its corner 0) in c0 and target 0 (its corner 1 plus the
+// partner's corner 0) in c1; the lane with bit 1 keeps target +1 (its corner
+// 1) in c0.  A fixed order: the same bits in every run.
+template <int M, typename Real>
+__device__ __forceinline__ void octant_stage(Real& c0, Real& c1, bool bit) {
+    const Real p = octant_xor<M>(c0);
+    const Real a0 = c0, a1 = c1;
+    c0 = bit ? a1 : a0;
+    c1 = bit ? Real(0) : a1 + p;
+}
+
 // Particles per chunk when the one-lane-per-subcell path stages the tile's
 // particles (6 arrays) through LDS: as many as fit in the subcell-sum
 // staging, which the chunks alias (it is written only after the last one).
@@ -803,7 +870,9 @@ __device__ void deposit_dense_tile(const Geom& g, const PArrays<Real>& A, const 
 template <typename Real>
 __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const uint32_t* __restrict__ sub,
                                                  const uint16_t* __restrict__ glist,
-                                                 const uint8_t* __restrict__ gcount, Real* __restrict__ partial,
+                                                 const uint8_t* __restrict__ gcount,
+                                                 const uint16_t* __restrict__ tlist,
+                                                 const uint8_t* __restrict__ tcount, Real* __restrict__ partial,
                                                  uint32_t dense_min) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int NS = g.tc * 8;
@@ -866,35 +935,43 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
             cic_accumulate(acc, hx, hy, hz, r.v[0], r.v[1], r.v[2], r.v[3], t.v[0], t.v[1]);
         }
     }
-    __syncthreads();  // the staging is read; S may be written
+    // the eight octant lanes of each cell: 64 corner sums -> 27 targets
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) octant_stage<1>(acc[q][2 * r], acc[q][2 * r + 1], hx);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) octant_stage<2>(acc[q][(r & 1) + 4 * (r >> 1)], acc[q][(r & 1) + 4 * (r >> 1) + 2], hy);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) octant_stage<4>(acc[q][r], acc[q][r + 4], hz);
+    }
+    __syncthreads();  // the staging is read; the target planes may be written
+    const int CB = OB, QP = 27 * CB;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int sx = c & 1, sy = (c >> 1) & 1, sz = c >> 2;
+        if ((hx && sx) || (hy && sy) || (hz && sz)) continue;  // slot unused by this octant
+        const int dx = hx ? 1 : sx - 1, dy = hy ? 1 : sy - 1, dz = hz ? 1 : sz - 1;
+        const int t27 = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) S[q * QP + t27 * CB + cell] = acc[q][c];
+    }
+    __syncthreads();
     Real4<Real>* out = reinterpret_cast<Real4<Real>*>(partial) + (size_t)T * g.next;
-#pragma unroll
-    for (int half = 0; half < 2; ++half) {
-        if (half) __syncthreads();  // the first half's reads are done
-#pragma unroll
-        for (int qq = 0; qq < 2; ++qq)
-#pragma unroll
-            for (int c = 0; c < 8; ++c) S[(qq * 8 + c) * QS + o * OB + cell] = acc[2 * half + qq][c];
-        __syncthreads();
-        for (int e = ls; e < g.next; e += blockDim.x) {
-            // the subcell terms reaching this cell: a list shared by all tiles
-            const int cnt_e = gcount[e];
-            const uint16_t* lst = glist + (size_t)e * 64;
-            Real res[2] = {Real(0), Real(0)};
-            for (int t = 0; t < cnt_e; ++t) {
-                const int off = lst[t];
-                res[0] += S[off];
-                res[1] += S[8 * QS + off];
-            }
-            if (half == 0) {
-                R[2 * e] = res[0];
-                R[2 * e + 1] = res[1];
-            } else {
-                Real4<Real> r;
-                r.v[0] = R[2 * e]; r.v[1] = R[2 * e + 1]; r.v[2] = res[0]; r.v[3] = res[1];
-                out[e] = r;
-            }
+    for (int e = ls; e < g.next; e += blockDim.x) {
+        // the cell terms reaching this cell: a list shared by all tiles
+        const int cnt_e = tcount[e];
+        const uint16_t* lst = tlist + (size_t)e * 32;
+        Real4<Real> r;
+        r.v[0] = r.v[1] = r.v[2] = r.v[3] = Real(0);
+        for (int t = 0; t < cnt_e; ++t) {
+            const int off = lst[t];
+            r.v[0] += S[off];
+            r.v[1] += S[QP + off];
+            r.v[2] += S[2 * QP + off];
+            r.v[3] += S[3 * QP + off];
         }
+        out[e] = r;
     }
 }
 
