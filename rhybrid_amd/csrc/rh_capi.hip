@@ -413,6 +413,7 @@ public:
         g.tc = g.tx * g.ty * g.tz;
         g.ntx = g.nx / g.tx; g.nty = g.ny / g.ty; g.ntz = g.nz / g.tz;
         g.ntiles = g.ntx * g.nty * g.ntz;
+        xcd_bricks(g);
         g.ex = g.tx + 2; g.ey = g.ty + 2; g.ez = g.tz + 2;
         g.next = g.ex * g.ey * g.ez;
         g.extk = g.next * 8;
@@ -514,7 +515,7 @@ public:
 
     static unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
     // per-tile launches: one workgroup per tile in XCD-aware order (xcd_tile)
-    dim3 tile_grid() const { return dim3(N_XCD * cdiv((size_t)g.ntiles, N_XCD)); }
+    dim3 tile_grid() const { return dim3(N_XCD * g.xchunk); }
     dim3 cell_grid() const { return dim3(cdiv(g.nx, 32), cdiv(g.ny, 8), g.nz); }
     static dim3 cell_block() { return dim3(32, 8, 1); }
 

@@ -46,6 +46,7 @@ struct Geom {
     int ex, ey, ez, next;      // extended region shape and cell count
     int extk;                  // next * 8 subcell slots in the extended region
     uint32_t nkeys;            // ntiles * tc * 8
+    int xpx, xpy, xpz, xchunk; // XCD bricks: parts per axis (product 8), most tiles in one brick
 };
 
 RH_HD void cell_ijk(const Geom& g, int n, int& i, int& j, int& k) {
@@ -100,17 +101,55 @@ RH_HD void local_ijk(const Geom& g, int lc, int& li, int& lj, int& lk) {
     lk = lc >> (g.ltx + g.lty);
 }
 
-// XCD-aware tile order.  The hardware deals the workgroups of a launch to the
-// eight XCDs (each with its own L2) in turn, workgroup b to XCD b % 8.  Per-
-// tile launches use a 1D grid of 8 * ceil(ntiles / 8) workgroups and give XCD
-// x the contiguous tile range [x * chunk, (x + 1) * chunk) in order, so that
-// tiles that are neighbours in memory -- whose particle ranges share cache
-// lines at their ends, written by both in the move -- run on one XCD, close
-// in time, instead of on two.  A result >= ntiles is no tile.
+// XCD-aware tile orders.  The hardware deals the workgroups of a launch to
+// the eight XCDs (each with its own L2) in turn, workgroup b to XCD b % 8.
+// Per-tile launches use a 1D grid of 8 * xchunk workgroups (xchunk covers
+// both orders); a result >= ntiles is no tile.
+//   xcd_tile: XCD x takes the contiguous range [x * chunk, (x + 1) * chunk)
+//   of the x-fastest tile order (slabs of whole tile layers), so that tiles
+//   neighbouring in memory run on one XCD close in time.  The push and the
+//   deposit, which read each tile's particle range, run in this order.
+//   xcd_brick_tile: the tile grid cut into eight bricks (xpx x xpy x xpz
+//   parts per axis), XCD x taking brick x, x-fastest.  The move writes the
+//   particles that cross into a neighbour tile into that tile's subcell
+//   ranges, in cache lines the neighbour's own workgroup also writes; within
+//   a brick those two workgroups share an L2 and run closer in time than in
+//   slabs, and bricks share fewer faces between XCDs (the move 519 -> 500 us
+//   per launch on the 128x96x96 box; the deposit and the push were slower
+//   in this order, 289 -> 307 us).
 constexpr int N_XCD = 8;
 RH_HD int xcd_tile(const Geom& g, int b) {
     const int chunk = (g.ntiles + N_XCD - 1) / N_XCD;
-    return (b % N_XCD) * chunk + b / N_XCD;
+    const int i = b / N_XCD;
+    return i < chunk ? (b % N_XCD) * chunk + i : g.ntiles;
+}
+
+RH_HD int xcd_brick_tile(const Geom& g, int b) {
+    const int x = b % N_XCD, i = b / N_XCD;
+    const int ix = x % g.xpx, iy = (x / g.xpx) % g.xpy, iz = x / (g.xpx * g.xpy);
+    const int x0 = ix * g.ntx / g.xpx, x1 = (ix + 1) * g.ntx / g.xpx;
+    const int y0 = iy * g.nty / g.xpy, y1 = (iy + 1) * g.nty / g.xpy;
+    const int z0 = iz * g.ntz / g.xpz, z1 = (iz + 1) * g.ntz / g.xpz;
+    const int bx = x1 - x0, by = y1 - y0, bz = z1 - z0;
+    if (i >= bx * by * bz) return g.ntiles;
+    return (x0 + i % bx) + g.ntx * ((y0 + (i / bx) % by) + g.nty * (z0 + i / (bx * by)));
+}
+
+// The brick split: eight parts, each doubling of an axis's parts going to
+// the axis with the most tiles per part (z first on ties).
+inline void xcd_bricks(Geom& g) {
+    int p[3] = {1, 1, 1};
+    const int n[3] = {g.ntx, g.nty, g.ntz};
+    for (int s = 0; s < 3; ++s) {
+        int best = 2;
+        for (int d = 1; d >= 0; --d)
+            if (n[d] * p[best] > n[best] * p[d]) best = d;  // n[d]/p[d] > n[best]/p[best]
+        p[best] *= 2;
+    }
+    g.xpx = p[0]; g.xpy = p[1]; g.xpz = p[2];
+    const int brick = ((g.ntx + p[0] - 1) / p[0]) * ((g.nty + p[1] - 1) / p[1]) * ((g.ntz + p[2] - 1) / p[2]);
+    const int slab = (g.ntiles + N_XCD - 1) / N_XCD;
+    g.xchunk = brick > slab ? brick : slab;
 }
 
 // The same dealing for any count of tiles (n), e.g. the field update's.

@@ -223,7 +223,7 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
     Real* sue = fl + 3 * nh;   // cellUe, cellEp of the tile's cells
     Real* sep = sue + 3 * g.tc;
     uint32_t* nmask = reinterpret_cast<uint32_t*>(sep + 3 * g.tc);  // neighbourhood of each tile cell
-    const int T = xcd_tile(g, (int)blockIdx.x);
+    const int T = pass == 1 ? xcd_brick_tile(g, (int)blockIdx.x) : xcd_tile(g, (int)blockIdx.x);
     if (T >= g.ntiles) return;
     int bx, by, bz;
     tile_coords(g, T, bx, by, bz);
