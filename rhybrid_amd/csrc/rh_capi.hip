@@ -761,7 +761,8 @@ public:
 
     void finish_sort(Species& S_, const char* move_name) {
         // totals and in-subcell offsets in one pass, then the scan of the totals
-        launch("sort_slots", k_slot_offsets, cdiv((uint32_t)g.ntiles * (uint32_t)g.tc, (uint32_t)SLOT_THREADS),
+        launch("sort_slots", k_slot_offsets,
+               N_XCD * cdiv(cdiv((uint32_t)g.ntiles * (uint32_t)g.tc, (uint32_t)SLOT_THREADS), (uint32_t)N_XCD),
                SLOT_THREADS, 0, g, S_.cnt_tile.p, (const uint8_t*)S_.live_src.p, S_.sub_new.p, dcount.p);
         scan(S_.sub_new.p, g.nkeys, S_.sub_new.p, 0);
         launch(move_name, k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, S_.pp,

@@ -486,7 +486,12 @@ __global__ __launch_bounds__(SLOT_THREADS) void k_slot_offsets(Geom g, uint16_t*
                                                               const uint8_t* __restrict__ live,
                                                               uint32_t* __restrict__ out,
                                                               unsigned long long* __restrict__ counters) {
-    const uint32_t id = blockIdx.x * (uint32_t)SLOT_THREADS + threadIdx.x;  // = tile * tc + local cell
+    // blocks dealt to the XCDs in contiguous ranges (slab order): a tile's
+    // slot table is read by its own cells and its neighbours' face cells
+    const uint32_t nb = ((uint32_t)g.ntiles * (uint32_t)g.tc + SLOT_THREADS - 1) / SLOT_THREADS;
+    const uint32_t blk = (uint32_t)xcd_index((int)nb, (int)blockIdx.x);
+    if (blk >= nb) return;
+    const uint32_t id = blk * (uint32_t)SLOT_THREADS + threadIdx.x;  // = tile * tc + local cell
     if (id >= (uint32_t)g.ntiles * (uint32_t)g.tc) return;
     const int T = (int)(id >> (g.ltx + g.lty + g.ltz)), lc = (int)(id & (uint32_t)(g.tc - 1));
     int li, lj, lk;
