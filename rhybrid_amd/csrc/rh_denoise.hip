@@ -76,10 +76,13 @@ __device__ __forceinline__ floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {
 }
 
 // Convolution over zero-padded volumes; CIN in {8, 32}, NT output tiles of 16.
-template <int CIN, int NT, int EP>
-__global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
+// A workgroup computes a 16x4xTZ_ tile, one wave per z plane (TZ_ = 4 or 8:
+// the taller tile stages 10 halo planes for 8 output planes instead of 6 for 4).
+template <int CIN, int NT, int EP, int TZ_>
+__global__ __launch_bounds__(64 * TZ_) void k_conv(ConvArgs a) {
     constexpr int CH = CIN / 8;  // 16-byte chunks per voxel
-    __shared__ bf16x8 tile[HV * CH];
+    constexpr int HZ_ = TZ_ + 2, HV_ = HX * HY * HZ_, NTH = 64 * TZ_;
+    __shared__ bf16x8 tile[HV_ * CH];
     // the first weight group of the 32-channel path, loaded while the halo is staged
     bf16x8 bnx[3][NT];
     if constexpr (CIN == 32) {
@@ -90,27 +93,27 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
             for (int nt = 0; nt < NT; ++nt) bnx[dy][nt] = a.wf[((dy * 3) * NT + nt) * 64 + l];
     }
     const int bx = blockIdx.x * TX, by = blockIdx.y * TY;
-    const int p = blockIdx.z / a.nzb, bz = (blockIdx.z % a.nzb) * TZ;
+    const int p = blockIdx.z / a.nzb, bz = (blockIdx.z % a.nzb) * TZ_;
     const size_t vol = (size_t)a.X * a.Y * a.Z;
     const bf16* inp = a.in + (size_t)p * vol * CIN;
     // halo staging: all of a thread's loads are issued before its LDS stores
-    constexpr int NST = (HV * CH + 255) / 256;
+    constexpr int NST = (HV_ * CH + NTH - 1) / NTH;
     bf16x8 stage[NST];
 #pragma unroll
     for (int q = 0; q < NST; ++q) {
-        const int idx = (int)threadIdx.x + 256 * q;
+        const int idx = (int)threadIdx.x + NTH * q;
         const int v = idx / CH, c8 = idx - v * CH;
         const int hx = v % HX, hy = (v / HX) % HY, hz = v / (HX * HY);
         const int gx = bx + hx - 1, gy = by + hy - 1, gz = bz + hz - 1;
 #pragma unroll
         for (int j = 0; j < 8; ++j) stage[q][j] = (bf16)0.0f;
-        if (idx < HV * CH && gx >= 0 && gx < a.X && gy >= 0 && gy < a.Y && gz >= 0 && gz < a.Z)
+        if (idx < HV_ * CH && gx >= 0 && gx < a.X && gy >= 0 && gy < a.Y && gz >= 0 && gz < a.Z)
             stage[q] = *reinterpret_cast<const bf16x8*>(inp + (((size_t)gz * a.Y + gy) * a.X + gx) * CIN + c8 * 8);
     }
 #pragma unroll
     for (int q = 0; q < NST; ++q) {
-        const int idx = (int)threadIdx.x + 256 * q;
-        if (idx < HV * CH) tile[idx] = stage[q];
+        const int idx = (int)threadIdx.x + NTH * q;
+        if (idx < HV_ * CH) tile[idx] = stage[q];
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -649,13 +652,18 @@ struct rh_denoiser {
         a.X = X;
         a.Y = Y;
         a.Z = Z;
-        a.nzb = Z / TZ;
         a.out = out;
         a.mask = mask;
         a.yout = yo;
         a.xres = xres;
         for (int c = 0; c < 4; ++c) a.s[c] = scale[c];
-        launch(name, k_conv<CIN, NT, EP>, dim3(X / TX, Y / TY, P * (Z / TZ)), dim3(256), a);
+        if (Z % 8 == 0) {
+            a.nzb = Z / 8;
+            launch(name, k_conv<CIN, NT, EP, 8>, dim3(X / TX, Y / TY, P * a.nzb), dim3(512), a);
+        } else {
+            a.nzb = Z / 4;
+            launch(name, k_conv<CIN, NT, EP, 4>, dim3(X / TX, Y / TY, P * a.nzb), dim3(256), a);
+        }
     }
 
     // The forward pass with the packed input in bx8 and the activations of
