@@ -631,11 +631,14 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const uint32_t* __r
 // src/particle_accumulator.cpp:43-156).  One workgroup per tile, one thread
 // per subcell: a subcell's particles share one 2x2x2 footprint, so the thread
 // sums, in particle order, the 8 corner weights times (1, vx, vy, vz).  The
-// subcell sums are staged in LDS (two quantities at a time) and every cell of
-// the tile's extended region adds the (up to 64) subcell terms that reach it
-// in a fixed order.  The result is the tile's ghost partial of this species
-// (unscaled); k_fold sums partials over tiles and species in FP64.  No
-// atomics, so every run gives the same bits.
+// eight octant threads of a cell then combine their corner sums into the
+// cell's 27 targets by lane exchanges, the targets are staged in LDS, and
+// every cell of the tile's extended region adds the (up to 27) cell terms
+// that reach it in a fixed order (crowded tiles: eight threads per subcell
+// and the (up to 64) subcell terms, two quantities at a time).  The result
+// is the tile's ghost partial of this species (unscaled); k_fold sums
+// partials over tiles and species in FP64.  No atomics, so every run gives
+// the same bits.
 template <typename Real>
 __device__ __forceinline__ void cic_accumulate(Real (&acc)[4][8], bool hx, bool hy, bool hz, Real X, Real Y, Real Z,
                                                Real vx, Real vy, Real vz) {
