@@ -481,12 +481,17 @@ def error_curve(pr, steps: int = 500, every: int = 50, order_tile=(2, 4, 8)):
             "tile": list(BENCH_TILE), "rows": rows}
 
 
-def exosphere_bench(steps: int = 10, warmup: int = 2, prefill: int = 600, tune_shapes: int = 4, tile=None):
+def exosphere_bench(steps: int = 10, warmup: int = 2, prefill: int = 600, tune_shapes: int = 4, tile=None,
+                    species_streams: bool = True):
     """The uneven-load box (256x192x192 cells, dx=200 km, H+ at 24 ppc with
     +x-wall inflow, O+ photoions of an exponential corona around an absorbing
     sphere of 12 cells): tile shape from the autotuner, H+ filled on the device
     (InjectorUniform at timestep 1), O+ accumulated over `prefill` injection
-    steps at their creation points, then `steps` timed full steps."""
+    steps at their creation points, then `steps` timed full steps with the
+    species' particle phases on streams of their own (the O+ launches, bound
+    by a few crowded tiles, run beside the H+ ones), then `steps` more in the
+    one-stream order for comparison, and the per-kernel table from further
+    one-stream steps (launches that overlap have no time of their own)."""
     from rhybrid_amd import autotune
     from rhybrid_amd import problem as P
     from rhybrid_amd.gpu import RH_FP32, Context
@@ -516,12 +521,20 @@ def exosphere_bench(steps: int = 10, warmup: int = 2, prefill: int = 600, tune_s
     ctx.synchronize()
     nO_cells = np.bincount(ctx.particles(1)[0], minlength=pr.grid.nc)
     n0 = [ctx.num_particles(s) for s in range(2)]
+    ctx.set_species_streams(species_streams)
     ctx.synchronize()
     t0 = time.perf_counter()
     ctx.step(steps)
     ctx.synchronize()
     wall = time.perf_counter() - t0
     n1 = [ctx.num_particles(s) for s in range(2)]
+    ctx.set_species_streams(False)
+    wall_one = None
+    if species_streams:
+        t0 = time.perf_counter()
+        ctx.step(steps)
+        ctx.synchronize()
+        wall_one = time.perf_counter() - t0
     # kernel breakdown per species from further steps with every launch timed
     nb = min(steps, 5)
     ctx.reset_timing()
@@ -564,6 +577,8 @@ def exosphere_bench(steps: int = 10, warmup: int = 2, prefill: int = 600, tune_s
                         "absorbing sphere R=12 cells, FP32",
             "tile": list(tuned["best"]), "autotune_ms_per_step": tuned["ms_per_step"],
             "ms_per_step": wall / steps * 1e3, "particle_steps_per_s": np_mean * steps / wall,
+            "species_streams": species_streams,
+            "ms_per_step_one_stream": round(wall_one / steps * 1e3, 4) if wall_one else None,
             "particles": {"H+": n1[0], "O+": n1[1]},
             "O+_ppc": {"max": int(occ.max()) if occ.size else 0, "mean_occupied": float(occ.mean()) if occ.size else 0.0,
                        "cells_occupied": int(occ.size), "prefill_steps": prefill},
@@ -599,6 +614,9 @@ def main():
     ap.add_argument("--no-error-curve", action="store_true", help="skip the 500-step FP32-vs-FP64 error curve")
     ap.add_argument("--only-exosphere", action="store_true", help="the uneven-load box alone (its kernels' profile)")
     ap.add_argument("--exosphere-tile", type=int, nargs=3, default=None, help="skip the autotuner, use this tile")
+    ap.add_argument("--exosphere-one-stream", action="store_true",
+                    help="time the uneven-load box in the one-stream order only (profiled runs: every launch "
+                         "then has a time of its own and the species' launches a fixed order)")
     ap.add_argument("--calibration-copy-gib", type=float, default=0.0,
                     help="first copy this many GiB device to device (a traffic counter calibration dispatch)")
     args = ap.parse_args()
@@ -610,7 +628,9 @@ def main():
         if args.calibration_copy_gib > 0:
             calibration_copy(args.calibration_copy_gib)
         print(json.dumps({"exosphere_box": exosphere_bench(steps=args.steps, warmup=args.warmup,
-                                                           tile=args.exosphere_tile)}), flush=True)
+                                                           tile=args.exosphere_tile,
+                                                           species_streams=not args.exosphere_one_stream)}),
+              flush=True)
         return
     if args.only_denoiser:
         print(json.dumps({"denoiser": denoiser_bench(), "forward_volume": denoiser_forward_volume()}), flush=True)
@@ -631,7 +651,7 @@ def main():
     exo = None
     if not args.no_exosphere:
         try:
-            exo = exosphere_bench(tile=args.exosphere_tile)
+            exo = exosphere_bench(tile=args.exosphere_tile, species_streams=not args.exosphere_one_stream)
         except Exception as e:  # reported in the line, the headline number stands
             exo = {"error": repr(e)}
     cpu, err = None, None

@@ -160,6 +160,14 @@ int rh_propagate_B(rh_ctx* ctx);
  * Synchronises at the end; returns RH_ERR_CAPACITY when a particle store
  * overflowed in these steps. */
 int rh_step(rh_ctx* ctx, int32_t nsteps);
+/* No counterpart in the reference (its ranks push one species after the
+ * other): with enable != 0, rh_step runs each species' push, migration and
+ * deposit on a stream of its own, side by side, and joins them before the
+ * ghost sums.  The results are bit for bit those of the default one-stream
+ * order.  For species whose load is concentrated in a few crowded tiles (a
+ * photoion cloud), whose launches leave most of the device idle.  Off by
+ * default; the per-kernel times of rh_kernel_times then include the overlap. */
+int rh_set_species_streams(rh_ctx* ctx, int32_t enable);
 
 /* ---- injection (src/particle_injector.cpp) ----------------------------- */
 

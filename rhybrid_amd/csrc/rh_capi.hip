@@ -112,6 +112,7 @@ public:
     virtual void synchronize() = 0;
     virtual void set_timing(bool on) = 0;
     virtual void set_timing_only(const char* name) = 0;
+    virtual void set_species_streams(bool on) = 0;
     virtual void kernel_time(const char* name, double* ms, int64_t* launches) = 0;
     virtual void reset_timing() = 0;
     virtual void layout(int64_t* out) = 0;
@@ -245,11 +246,27 @@ public:
             std::swap(a_lc.p, dst_lc.p);
             std::swap(sub.p, sub_new.p);
         }
+        // the species' own stream for the particle phase of a step, the event
+        // that marks the end of its work there, and its scan's block sums
+        hipStream_t ss = nullptr;
+        hipEvent_t done = nullptr;
+        std::vector<DevBuf<uint32_t>> scan_lv;
+        Species() = default;
+        Species(const Species&) = delete;
+        Species& operator=(const Species&) = delete;
+        ~Species() {
+            if (ss) (void)hipStreamSynchronize(ss);
+            if (done) (void)hipEventDestroy(done);
+            if (ss) (void)hipStreamDestroy(ss);
+        }
     };
 
     Geom g{};
     rh_params P{};
     hipStream_t stream = nullptr;
+    hipStream_t ls = nullptr;      // the stream launches go to (stream, or a species' own)
+    hipEvent_t fork_ev = nullptr;  // the particle phase's species streams wait for it
+    bool species_streams = false;  // step(): the species' particle phases side by side
     DevBuf<Real> fld[RH_N_FIELDS];
     DevBuf<Real> tmpv;            // old nodeE for the gaussian smoothing
     DevBuf<Real> faceB_alt;       // the fused field update's second faceB buffer
@@ -295,6 +312,8 @@ public:
         RH_HIP(hipFuncSetAttribute((const void*)k_push_move<Real>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)push_move_lds_bytes<Real>(g)));
         RH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        ls = stream;
+        RH_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
         for (int f = 0; f < RH_N_FIELDS; ++f) {
             fld[f].alloc((size_t)g.nc * FIELD_DIM[f]);
             fill(fld[f].p, Real(0), fld[f].n);
@@ -338,27 +357,35 @@ public:
         }
         zero_u32.alloc(1);
         RH_HIP(hipMemsetAsync(zero_u32.p, 0, sizeof(uint32_t), stream));
-        // scan levels
-        size_t n = (size_t)g.nkeys;
-        while (n > (size_t)SCAN_TILE) {
-            const size_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-            scan_lv.emplace_back();
-            scan_lv.back().alloc(nb + 1);
-            scan_lv.emplace_back();
-            scan_lv.back().alloc(nb + 1);
-            n = nb;
-        }
+        alloc_scan_levels(scan_lv);
         kbits = 1;
         while ((1 << kbits) < g.extk) ++kbits;
         RH_HIP(hipStreamSynchronize(stream));
     }
 
+    // block sums of a scan over nkeys elements, per level
+    void alloc_scan_levels(std::vector<DevBuf<uint32_t>>& lv) {
+        size_t n = (size_t)g.nkeys;
+        while (n > (size_t)SCAN_TILE) {
+            const size_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+            lv.emplace_back();
+            lv.back().alloc(nb + 1);
+            lv.emplace_back();
+            lv.back().alloc(nb + 1);
+            n = nb;
+        }
+    }
+
     ~Sim() override {
+        for (auto& s_ : sp)
+            if (s_->ss) (void)hipStreamSynchronize(s_->ss);
         if (stream) (void)hipStreamSynchronize(stream);
         if (dn_attached) (void)rh_dn_set_stream(dn_attached, nullptr);  // before the stream goes
         for (auto& t : timers)
             for (auto& pe : t.second.pending) { (void)hipEventDestroy(pe.e0); (void)hipEventDestroy(pe.e1); }
         for (auto e : event_pool) (void)hipEventDestroy(e);
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        sp.clear();
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -493,12 +520,12 @@ public:
         if (timed) {
             e0 = get_event();
             e1 = get_event();
-            RH_HIP(hipEventRecord(e0, stream));
+            RH_HIP(hipEventRecord(e0, ls));
         }
-        hipLaunchKernelGGL(kernel, grid, block, shmem, stream, args...);
+        hipLaunchKernelGGL(kernel, grid, block, shmem, ls, args...);
         RH_HIP(hipGetLastError());
         if (timed) {
-            RH_HIP(hipEventRecord(e1, stream));
+            RH_HIP(hipEventRecord(e1, ls));
             Timer& t = timers[name];
             t.pending.push_back({e0, e1, timing_species});
             t.launches++;
@@ -547,6 +574,7 @@ public:
         fill(s->bc_acc.p, 0.0, s->bc_acc.n);
         ensure_capacity(*s, (size_t)std::max<int64_t>(d->capacity, 0));
         ensure_averages(*s);
+        alloc_scan_levels(s->scan_lv);
         sp.push_back(std::move(s));
         RH_HIP(hipStreamSynchronize(stream));
         return (int)sp.size() - 1;
@@ -739,16 +767,16 @@ public:
         finish_sort(S_, "sort_still");
     }
 
-    void scan(const uint32_t* in, uint32_t n, uint32_t* out, size_t level) {
+    void scan(const uint32_t* in, uint32_t n, uint32_t* out, std::vector<DevBuf<uint32_t>>& lv, size_t level) {
         if (n <= (uint32_t)SCAN_TILE) {
             launch("scan", k_scan_apply, 1, SCAN_THREADS, 0, in, n, (const uint32_t*)zero_u32.p, out);
             return;
         }
         const uint32_t nb = cdiv(n, SCAN_TILE);
-        uint32_t* bsum = scan_lv[level].p;
-        uint32_t* boff = scan_lv[level + 1].p;
+        uint32_t* bsum = lv[level].p;
+        uint32_t* boff = lv[level + 1].p;
         launch("scan", k_scan_reduce, nb, SCAN_THREADS, 0, in, n, bsum);
-        scan(bsum, nb, boff, level + 2);
+        scan(bsum, nb, boff, lv, level + 2);
         launch("scan", k_scan_apply, nb, SCAN_THREADS, 0, in, n, (const uint32_t*)boff, out);
     }
 
@@ -764,7 +792,7 @@ public:
         launch("sort_slots", k_slot_offsets,
                N_XCD * cdiv(cdiv((uint32_t)g.ntiles * (uint32_t)g.tc, (uint32_t)SLOT_THREADS), (uint32_t)N_XCD),
                SLOT_THREADS, 0, g, S_.cnt_tile.p, (const uint8_t*)S_.live_src.p, S_.sub_new.p, dcount.p);
-        scan(S_.sub_new.p, g.nkeys, S_.sub_new.p, 0);
+        scan(S_.sub_new.p, g.nkeys, S_.sub_new.p, S_.scan_lv, 0);
         launch(move_name, k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, S_.pp,
                (const uint32_t*)pf_tile.p,
                S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.appendix(), S_.cnt_tile.p,
@@ -1061,10 +1089,14 @@ public:
     void step(int n) override {
         for (int it = 0; it < n; ++it) {
             ++tstep;
-            for (size_t s = 0; s < sp.size(); ++s) propagate((int)s);
-            clear_accumulation();
-            for (size_t s = 0; s < sp.size(); ++s) wait_particle_sends((int)s);
-            for (size_t s = 0; s < sp.size(); ++s) accumulate((int)s);
+            if (species_streams && sp.size() > 1) {
+                particle_phase_side_by_side();
+            } else {
+                for (size_t s = 0; s < sp.size(); ++s) propagate((int)s);
+                clear_accumulation();
+                for (size_t s = 0; s < sp.size(); ++s) wait_particle_sends((int)s);
+                for (size_t s = 0; s < sp.size(); ++s) accumulate((int)s);
+            }
             add_remote_updates();
             for (size_t s = 0; s < sp.size(); ++s) apply_bcs((int)s);
             for (size_t s = 0; s < sp.size(); ++s) inject((int)s);
@@ -1074,6 +1106,45 @@ public:
             propagate_B_impl(it == n - 1);
         }
         check_overflow();
+    }
+
+    // propagate, wait_particle_sends and accumulate of every species, each
+    // species on a stream of its own after the work before them on the
+    // context's stream, which then waits for all of them.  The species share
+    // only read-only inputs (the fields and the push blocks, written before
+    // the fork) and integer counters; every buffer they write is their own,
+    // so the results are bit for bit those of the one-stream order.  A
+    // species whose few crowded tiles keep a handful of CUs busy while the
+    // rest of the device idles (photoions piled up near an obstacle) then
+    // runs beside the other species' full-device launches.
+    struct LaunchStream {
+        hipStream_t& slot;
+        hipStream_t prev;
+        LaunchStream(hipStream_t& s, hipStream_t to) : slot(s), prev(s) { slot = to; }
+        ~LaunchStream() { slot = prev; }
+    };
+    void particle_phase_side_by_side() {
+        for (auto& s_ : sp)
+            if (s_->pushed) throw StateError("step called between propagate and wait_particle_sends");
+        prepare_push_fields();
+        clear_accumulation();
+        RH_HIP(hipEventRecord(fork_ev, stream));
+        for (size_t s = 0; s < sp.size(); ++s) {
+            Species& S_ = *sp[s];
+            if (!S_.ss) {
+                RH_HIP(hipStreamCreateWithFlags(&S_.ss, hipStreamNonBlocking));
+                RH_HIP(hipEventCreateWithFlags(&S_.done, hipEventDisableTiming));
+            }
+            RH_HIP(hipStreamWaitEvent(S_.ss, fork_ev, 0));
+            {
+                LaunchStream on(ls, S_.ss);
+                propagate((int)s);
+                wait_particle_sends((int)s);
+                accumulate((int)s);
+            }
+            RH_HIP(hipEventRecord(S_.done, S_.ss));
+        }
+        for (auto& s_ : sp) RH_HIP(hipStreamWaitEvent(stream, s_->done, 0));
     }
 
     // The sort and the injection count overflows of a particle store into
@@ -1253,7 +1324,7 @@ public:
         const double* rate = S_.inj_rate.p;
         launch("inject_count", k_inject_count<Real>, nb, 256, 0, g, ip, S_.inj_ncell, (const int*)S_.inj_cells.p,
                rate, S_.inj_off.p);
-        scan(S_.inj_off.p, (uint32_t)S_.inj_ncell, S_.inj_off.p, 0);
+        scan(S_.inj_off.p, (uint32_t)S_.inj_ncell, S_.inj_off.p, scan_lv, 0);
         launch("inject_fill", k_inject_fill<Real>, nb, 256, 0, g, ip, S_.inj_ncell, (const int*)S_.inj_cells.p, rate,
                (const uint32_t*)S_.inj_off.p, (const uint32_t*)(S_.sub.p + g.nkeys), S_.A(), (uint32_t)S_.cap,
                S_.inj_total.p, dcount.p, S_.inj_acc.p);
@@ -1749,6 +1820,7 @@ public:
     }
 
     void set_timing_only(const char* name) override { timing_only = name ? name : ""; }
+    void set_species_streams(bool on) override { species_streams = on; }
 
     void kernel_time(const char* name, double* ms, int64_t* launches) override {
         resolve_timers();
@@ -1898,6 +1970,10 @@ int rh_set_timing(rh_ctx* ctx, int32_t on) {
 }
 int rh_set_timing_only(rh_ctx* ctx, const char* kernel) {
     return guarded(ctx, [&](rh::Ctx& c) { c.set_timing_only(kernel); });
+}
+
+int rh_set_species_streams(rh_ctx* ctx, int32_t enable) {
+    return guarded(ctx, [&](rh::Ctx& c) { c.set_species_streams(enable != 0); });
 }
 int rh_kernel_times(rh_ctx* ctx, const char* kernel, double* ms, int64_t* launches) {
     return guarded(ctx, [&](rh::Ctx& c) { c.kernel_time(kernel, ms, launches); });

@@ -39,7 +39,8 @@ EXPORTS = ["rh_create", "rh_destroy", "rh_last_error", "rh_add_species", "rh_upl
            "rh_reset_timing",
            "rh_layout_info", "rh_denoise_moments", "rh_set_denoiser", "rh_set_injector",
            "rh_inject_particles", "rh_set_timestep", "rh_get_timestep", "rh_injected", "rh_log_values",
-           "rh_log_begin", "rh_log_open", "rh_log_write", "rh_write_vlsv", "rh_set_temporal_averages"]
+           "rh_log_begin", "rh_log_open", "rh_log_write", "rh_write_vlsv", "rh_set_temporal_averages",
+           "rh_set_species_streams"]
 
 
 class GridDesc(C.Structure):
@@ -115,7 +116,7 @@ def load_library(path: str = LIB_PATH):
                  "rh_propagate_B", "rh_synchronize", "rh_reset_timing"]:
         getattr(L, name).argtypes = [vp]
     for name in ["rh_propagate", "rh_wait_particle_sends", "rh_accumulate",
-                 "rh_apply_boundary_conditions", "rh_step", "rh_set_timing"]:
+                 "rh_apply_boundary_conditions", "rh_step", "rh_set_timing", "rh_set_species_streams"]:
         getattr(L, name).argtypes = [vp, i32]
     L.rh_diagnostics.argtypes = [vp, i32, vp]
     L.rh_counters.argtypes = [vp, vp]
@@ -393,6 +394,12 @@ class Context:
         self._check(load_library().rh_set_timing_only(self._h, name.encode() if name else None))
 
     def reset_timing(self) -> None: self._check(load_library().rh_reset_timing(self._h))
+
+    def set_species_streams(self, on: bool) -> None:
+        """step(): each species' push, migration and deposit on a stream of its
+        own, side by side (the same results; for species crowded into few
+        tiles)."""
+        self._check(load_library().rh_set_species_streams(self._h, int(on)))
 
     def kernel_time(self, name: str):
         ms, n = C.c_double(), C.c_int64()

@@ -171,3 +171,31 @@ def test_uneven_load_box_vs_oracle(gpu_mod, oracle_mod, tile):
         assert r32[name] < allow[name], ("fp32", name, r32[name], allow[name])
     for s in range(2):
         assert abs(ng[s] - no[s]) <= 2, (s, ng[s], no[s])
+
+
+def test_species_streams_bitwise(gpu_mod):
+    """rh_set_species_streams: each species' push, migration and deposit on
+    a stream of its own changes nothing — the reduced uneven-load box (crowded
+    O+ tiles beside a filled H+ box) gives bit-identical grid arrays and
+    particle stores after 6 steps with and without it, including the steps'
+    injection and boundary conditions on the context's stream."""
+    pr = exosphere_small(seed=45)
+    out = []
+    for on in (False, True):
+        print("case species_streams=%s" % on, flush=True)
+        ctx = gpu_mod.Context(pr, precision=gpu_mod.RH_FP32, tile=(4, 4, 4))
+        exo_prepare(pr)(ctx)
+        ctx.set_species_streams(on)
+        ctx.step(3)
+        ctx.step(3)
+        fields = [ctx.field(n) for n in ("cellRhoQi", "cellJi", "faceB", "nodeE", "cellUe")]
+        parts = [ctx.particles(s) for s in range(2)]
+        out.append((fields, parts))
+        ctx.close()
+    (fa, pa), (fb, pb) = out
+    for a, b in zip(fa, fb):
+        assert np.array_equal(a, b)
+    for s in range(2):
+        assert len(pa[s][0]) > 1000
+        for a, b in zip(pa[s], pb[s]):
+            assert np.array_equal(a, b), s
