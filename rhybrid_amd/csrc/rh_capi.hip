@@ -881,8 +881,14 @@ public:
     void alloc_push_fields() {
         if (pf_tile.p) return;
         pf_tile.alloc((size_t)g.ntiles * push_field_words<Real>(g));
-        // halo entries of cells outside a non-periodic grid stay zero
+        // halo entries of cells outside a non-periodic grid stay zero; every
+        // cell's entries, its neighbourhood mask included, are written here
+        // once (the fused field update then rewrites only the fields)
         launch("fill", k_fill<uint32_t>, cdiv(pf_tile.n, 256), 256, 0, pf_tile.p, 0u, pf_tile.n);
+        const V3<Real> fb = v3(RH_FACE_B), ue = v3(RH_CELL_UE), ep = v3(RH_CELL_EP);
+        PushFields<Real> pf{fb.x, fb.y, fb.z, ue.x, ue.y, ue.z, ep.x, ep.y, ep.z};
+        launch("push_fields", k_push_fields<Real>, cell_grid(), cell_block(), 0, g, pf, (int)P.use_e_pressure,
+               pf_tile.p);
     }
 
     void prepare_push_fields() {

@@ -606,6 +606,12 @@ struct FieldUpdateArgs {
 // correctly rounded quotient in FP64
 __device__ __forceinline__ float inv_sqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double inv_sqrt(double x) { return 1.0 / sqrt(x); }
+// 1/x and sqrt(x): the hardware instructions in FP32 (1 ulp), correctly
+// rounded in FP64
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double fast_sqrt(double x) { return sqrt(x); }
 
 // calcCellUe / calcNodeUe on values (ue_at above, NODE clamps r)
 template <typename Real, bool NODE>
@@ -617,7 +623,7 @@ __device__ __forceinline__ void ue_core(const Real (&jj)[3], const Real (&ji)[3]
         return;
     }
     if (NODE && r < min_rho) { r = min_rho; c_rho = true; }
-    const Real ir = fabs(r) > Real(0) ? Real(1) / r : Real(0);
+    const Real ir = fabs(r) > Real(0) ? fast_rcp(r) : Real(0);
     for (int l = 0; l < 3; ++l) u[l] = use_hall ? -(jj[l] - ji[l]) * ir : ji[l] * ir;
     const Real u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
     if (u2 > max_ue2) {
@@ -674,6 +680,11 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
     if (T >= ntx * nty * ntz) return;
     const int ox = (T % ntx) * TX, oy = ((T / ntx) % nty) * TY, oz = (T / (ntx * nty)) * TZ;
     const V3<Real>& F = a.faceB_in;
+    // a tile whose halo lies inside the grid without a wrap and holds no cell
+    // of the grid's outer layer: no wrap, no wall rule (the same for the
+    // whole workgroup)
+    const bool deep = ox - 2 >= 1 && oy - 2 >= 1 && oz - 2 >= 1 && ox + TX <= g.nx - 2 && oy + TY <= g.ny - 2 &&
+                      oz + TZ <= g.nz - 2;
 
     // 1. faceB, cellRhoQi, cellJi, nodeEta and the inner flags of the halo.
     //    All of a thread's loads are issued before its LDS stores; the cells
@@ -694,9 +705,11 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
         lfl[r] = 0;
         if (e >= N2) continue;
         const int lx = e % X2, ly = (e / X2) % Y2, lz = e / (X2 * Y2);
-        const int i = wrap_axis(ox - 2 + lx, g.nx, g.per[0]), j = wrap_axis(oy - 2 + ly, g.ny, g.per[1]),
-                  k = wrap_axis(oz - 2 + lz, g.nz, g.per[2]);
-        if (i < 0 || j < 0 || k < 0) continue;
+        int i = ox - 2 + lx, j = oy - 2 + ly, k = oz - 2 + lz;
+        if (!deep) {
+            i = wrap_axis(i, g.nx, g.per[0]); j = wrap_axis(j, g.ny, g.per[1]); k = wrap_axis(k, g.nz, g.per[2]);
+            if (i < 0 || j < 0 || k < 0) continue;
+        }
         const int n = cell_index(g, i, j, k);
         en[r] = n;
         ei[r] = i; ej[r] = j; ek[r] = k;
@@ -726,7 +739,7 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
         const int n = en[r];
         if (n < 0) continue;
         const int i = ei[r], j = ej[r], k = ek[r];
-        if (all_nbrs(g, i, j, k)) continue;
+        if (deep || all_nbrs(g, i, j, k)) continue;
         const int miss = missing_mask(g, i, j, k);
         Real f[3];
         for (int l = 0; l < 3; ++l) {
@@ -753,9 +766,11 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
     //    changed by the exterior copies: see k_cell_B_bc)
     for (int e = tid; e < N1; e += blockDim.x) {
         const int lx = e % X1, ly = (e / X1) % Y1, lz = e / (X1 * Y1);
-        const int i = wrap_axis(ox - 1 + lx, g.nx, g.per[0]), j = wrap_axis(oy - 1 + ly, g.ny, g.per[1]),
-                  k = wrap_axis(oz - 1 + lz, g.nz, g.per[2]);
-        if (i < 0 || j < 0 || k < 0 || !all_nbrs(g, i, j, k)) continue;
+        if (!deep) {
+            const int i = wrap_axis(ox - 1 + lx, g.nx, g.per[0]), j = wrap_axis(oy - 1 + ly, g.ny, g.per[1]),
+                      k = wrap_axis(oz - 1 + lz, g.nz, g.per[2]);
+            if (i < 0 || j < 0 || k < 0 || !all_nbrs(g, i, j, k)) continue;
+        }
         const int f = (lx + 1) + X2 * ((ly + 1) + Y2 * (lz + 1));
         sB[0][e] = Real(0.5) * (sF[0][f] + sF[0][f - 1]);
         sB[1][e] = Real(0.5) * (sF[1][f] + sF[1][f - X2]);
@@ -765,9 +780,12 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
     // 3. the nodes: cell2Node, calcNodeJ, calcNodeUe, upwindNodeB, calcNodeE
     for (int q = tid; q < NN; q += blockDim.x) {
         const int qx = q % XN, qy = (q / XN) % YN, qz = q / (XN * YN);
-        const int i = wrap_axis(ox - 1 + qx, g.nx, g.per[0]), j = wrap_axis(oy - 1 + qy, g.ny, g.per[1]),
-                  k = wrap_axis(oz - 1 + qz, g.nz, g.per[2]);
-        const bool ok = i >= 0 && j >= 0 && k >= 0 && !upper_missing(g, i, j, k);
+        int i = ox - 1 + qx, j = oy - 1 + qy, k = oz - 1 + qz;
+        bool ok = true;
+        if (!deep) {
+            i = wrap_axis(i, g.nx, g.per[0]); j = wrap_axis(j, g.ny, g.per[1]); k = wrap_axis(k, g.nz, g.per[2]);
+            ok = i >= 0 && j >= 0 && k >= 0 && !upper_missing(g, i, j, k);
+        }
         // the node of one of the tile's own cells (a partial tile's halo may
         // wrap onto cells of another tile along a periodic axis)
         const bool own = ok && qx >= 1 && qy >= 1 && qz >= 1 && ox - 1 + qx < g.nx && oy - 1 + qy < g.ny &&
@@ -797,21 +815,22 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
             const Real qe = Real(1.60217653e-19);
             const Real mu0 = Real(4.0e-7 * 3.14159265358979323846);
             Real d = Real(1), vw = Real(0);
-            const Real ne = r / qe;
-            const Real Btot = sqrt(bx * bx + by * by + bz * bz);
+            const Real ne = r * Real(1.0 / 1.60217653e-19);
+            const Real Btot = fast_sqrt(bx * bx + by * by + bz * bz);
             const Real dx = (Real)g.dx;
-            if (ne > Real(0) && dx > Real(0)) vw = Real(2) * Btot * Real(3.14159265358979323846) / (mu0 * ne * qe * dx);
+            if (ne > Real(0) && dx > Real(0))
+                vw = Real(2) * Btot * Real(3.14159265358979323846) * fast_rcp(mu0 * ne * qe * dx);
             if (vw > a.max_vw && a.max_vw > Real(0)) { d = vw / a.max_vw; c_vw = true; }
-            const Real af = Real(0.5) / (dx * mu0 * d);
+            const Real af = Real(0.5) * fast_rcp(dx * mu0 * d);
             const Real J[3] = {(ex1 + ex2) * af, (ey1 + ey2) * af, (ez1 + ez2) * af};
             // calcNodeUe (clamps the node's rho)
             Real u[3];
             ue_core<Real, true>(J, ji, r, (sFl[c] & 2) != 0, a.use_hall, a.oue, a.max_ue2, a.min_rho, u, c_ue, c_rho);
             // upwindNodeB: cells in the reference's order 111,112,121,211,122,221,212,222
             Real xu = Real(-0.5), yu = Real(0), zu = Real(0);
-            const Real ue = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            const Real ue = fast_sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
             if (ue > Real(0)) {
-                const Real am = Real(-0.5) / ue;
+                const Real am = Real(-0.5) * fast_rcp(ue);
                 xu = u[0] * am; yu = u[1] * am; zu = u[2] * am;
             }
             const int cc[8] = {c, c001, c010, c100, c011, c110, c101, c111};
@@ -832,7 +851,7 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
                     b1 += w[t] * sB[1][cc[t]];
                     b2 += w[t] * sB[2][cc[t]];
                 }
-                const Real iw = Real(1) / wsum;
+                const Real iw = fast_rcp(wsum);
                 b[0] = b0 * iw; b[1] = b1 * iw; b[2] = b2 * iw;
             } else {
                 b[0] = b[1] = b[2] = Real(0);
@@ -865,7 +884,7 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
         if (i < g.nx && j < g.ny && k < g.nz) {
             const int n = cell_index(g, i, j, k);
             const int c1 = (rx + 1) + X1 * ((ry + 1) + Y1 * (rz + 1));
-            const bool interior = all_nbrs(g, i, j, k);
+            const bool interior = deep || all_nbrs(g, i, j, k);
             // node2Cell of nodeJ at the cell whose node sits at node index qn
             auto cellJ_at = [&](int qn, Real (&cj)[3]) {
                 const int m100 = qn - 1, m010 = qn - XN, m001 = qn - XN * YN;
@@ -917,7 +936,7 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
                         if (!a.isothermal) {
                             ep[0] = -a.e_coeff * gx; ep[1] = -a.e_coeff * gy; ep[2] = -a.e_coeff * gz;
                         } else if (fabs(crho) > Real(0)) {
-                            const Real ic = Real(1) / crho;
+                            const Real ic = fast_rcp(crho);
                             ep[0] = -a.e_coeff * gx * ic; ep[1] = -a.e_coeff * gy * ic; ep[2] = -a.e_coeff * gz * ic;
                         }
                     }
@@ -930,8 +949,9 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
             // faceCurl (Faraday) on the faces after their exterior copies
             const int f = (rx + 2) + X2 * ((ry + 2) + Y2 * (rz + 2));
             Real fb[3] = {sF[0][f], sF[1][f], sF[2][f]};
-            if (!upper_missing(g, i, j, k)) {
-                const bool xn = has_neg(i, g.per[0]), yn = has_neg(j, g.per[1]), zn = has_neg(k, g.per[2]);
+            if (deep || !upper_missing(g, i, j, k)) {
+                const bool xn = deep || has_neg(i, g.per[0]), yn = deep || has_neg(j, g.per[1]),
+                           zn = deep || has_neg(k, g.per[2]);
                 const Real idx = Real(1) / (Real)g.dx;
                 auto E = [&](int l, int ax, int ay, int az) { return nE[l][qn + ax + XN * (ay + YN * az)]; };
                 if (yn && zn) {
@@ -969,7 +989,7 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
             }
             if (a.push_blocks)
                 push_fields_store(g, i, j, k, fb[0], fb[1], fb[2], u[0], u[1], u[2], ep[0], ep[1], ep[2], a.use_e_pressure,
-                                  a.push_blocks);
+                                  a.push_blocks, false);
         }
         count_event(a.counters, C_MAX_CELL_UE, c_ue);
     }

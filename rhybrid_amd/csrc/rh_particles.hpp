@@ -147,7 +147,8 @@ RH_HD uint32_t neighbour_mask(const Geom& g, int i, int j, int k) {
 template <typename Real>
 __device__ __forceinline__ void push_fields_store(const Geom& g, int i, int j, int k, Real bx, Real by, Real bz,
                                                   Real uex, Real uey, Real uez, Real epx, Real epy, Real epz,
-                                                  int use_e_pressure, uint32_t* __restrict__ out) {
+                                                  int use_e_pressure, uint32_t* __restrict__ out,
+                                                  bool write_mask = true) {
     const int hx = g.tx + 1, hy = g.ty + 1, nh = (g.tx + 1) * (g.ty + 1) * (g.tz + 1);
     const size_t nw = push_field_words<Real>(g);
     const int li = i & (g.tx - 1), lj = j & (g.ty - 1), lk = k & (g.tz - 1);
@@ -161,7 +162,9 @@ __device__ __forceinline__ void push_fields_store(const Geom& g, int i, int j, i
         sue[lc] = uex; sue[g.tc + lc] = uey; sue[2 * g.tc + lc] = uez;
         if (use_e_pressure) { sep[lc] = epx; sep[g.tc + lc] = epy; sep[2 * g.tc + lc] = epz; }
         else { sep[lc] = sep[g.tc + lc] = sep[2 * g.tc + lc] = Real(0); }
-        reinterpret_cast<uint32_t*>(sep + 3 * g.tc)[lc] = neighbour_mask(g, i, j, k);
+        // (the mask depends on the grid only: a caller that rewrites the
+        // fields of blocks written once in full may leave it)
+        if (write_mask) reinterpret_cast<uint32_t*>(sep + 3 * g.tc)[lc] = neighbour_mask(g, i, j, k);
     }
     // halo copies: the cell is local coordinate -1 of the tile above along
     // each axis where it is the tile's last cell
