@@ -271,6 +271,7 @@ public:
     DevBuf<Real> tmpv;            // old nodeE for the gaussian smoothing
     DevBuf<Real> faceB_alt;       // the fused field update's second faceB buffer
     DevBuf<uint8_t> flg[RH_N_FLAGS];
+    DevBuf<uint8_t> flg_fu;       // the fused field update's three inner flags in one byte per cell
     DevBuf<int> ext_cells, bc_cells;
     int n_ext = 0, n_bc = 0;
     DevBuf<unsigned long long> dcount;
@@ -324,6 +325,8 @@ public:
             flg[f].alloc(g.nc);
             RH_HIP(hipMemsetAsync(flg[f].p, 0, g.nc, stream));
         }
+        flg_fu.alloc(g.nc);
+        RH_HIP(hipMemsetAsync(flg_fu.p, 0, g.nc, stream));
         h_inner_particle.assign(g.nc, 0);
         // exterior cells: missing any neighbour
         std::vector<int> ext;
@@ -614,6 +617,9 @@ public:
     void upload_flags(int id, const uint8_t* data) override {
         if (id < 0 || id >= RH_N_FLAGS || !data) throw ArgError("bad flag id or null data");
         RH_HIP(hipMemcpyAsync(flg[id].p, data, g.nc, hipMemcpyHostToDevice, stream));
+        if (id == RH_INNER_FIELD || id == RH_INNER_NODE || id == RH_INNER_CELLEP)
+            launch("fill", k_pack_inner_flags, cdiv(g.nc, 256), 256, 0, g.nc, (const uint8_t*)flg[RH_INNER_FIELD].p,
+                   (const uint8_t*)flg[RH_INNER_NODE].p, (const uint8_t*)flg[RH_INNER_CELLEP].p, flg_fu.p);
         RH_HIP(hipStreamSynchronize(stream));
         if (id == RH_INNER_PARTICLE) {
             for (int n = 0; n < g.nc; ++n) h_inner_particle[n] = data[n] ? 1 : 0;
@@ -987,9 +993,7 @@ public:
             a.rho = fld[RH_CELL_RHOQ].p;
             a.Ji = v3(RH_CELL_JI);
             a.eta = fld[RH_NODE_ETA].p;
-            a.inner_field = flg[RH_INNER_FIELD].p;
-            a.inner_node = flg[RH_INNER_NODE].p;
-            a.inner_cellEp = flg[RH_INNER_CELLEP].p;
+            a.inner3 = flg_fu.p;
             a.cellUe = v3(RH_CELL_UE);
             a.cellEp = v3(RH_CELL_EP);
             a.cellB = v3(RH_CELL_B);

@@ -555,6 +555,16 @@ __global__ __launch_bounds__(256) void k_face_curl_Ep(Geom g, V3<Real> E, V3<Rea
     }
 }
 
+// The three inner flags the fused field update reads, one byte per cell
+// (bit 0 innerFlagField, bit 1 innerFlagNode, bit 2 innerFlagCellEp),
+// rebuilt whenever one of them is uploaded.
+__global__ __launch_bounds__(256) void k_pack_inner_flags(int nc, const uint8_t* __restrict__ field,
+                                                         const uint8_t* __restrict__ node,
+                                                         const uint8_t* __restrict__ cellEp, uint8_t* __restrict__ out) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n < nc) out[n] = (field[n] ? 1 : 0) | (node[n] ? 2 : 0) | (cellEp[n] ? 4 : 0);
+}
+
 // ---- the fused field update ------------------------------------------------
 //
 // propagateB without E smoothing (src/hybrid_propagator.cpp:42-440, steps
@@ -584,9 +594,7 @@ struct FieldUpdateArgs {
     Real* rho;                // cellRhoQi (exterior cells: their Neumann copies, full update)
     V3<Real> Ji;              // cellJi (idem)
     const Real* eta;          // nodeEta
-    const uint8_t* inner_field;
-    const uint8_t* inner_node;
-    const uint8_t* inner_cellEp;
+    const uint8_t* inner3;    // per cell: bit 0 innerFlagField, bit 1 innerFlagNode, bit 2 innerFlagCellEp
     V3<Real> cellUe, cellEp;
     // the reference's other arrays (full update; cellB also when write_cellB)
     V3<Real> cellB, cellJ, nodeB, nodeJi, nodeJ, nodeUe, nodeE;
@@ -718,7 +726,7 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
             lj[r][0] = a.Ji.x[n]; lj[r][1] = a.Ji.y[n]; lj[r][2] = a.Ji.z[n];
             lr[r] = a.rho[n];
             le[r] = a.eta[n];
-            lfl[r] = (a.inner_field[n] ? 1 : 0) | (a.inner_node[n] ? 2 : 0) | (a.inner_cellEp[n] ? 4 : 0);
+            lfl[r] = a.inner3[n];
         }
     }
 #pragma unroll
@@ -989,7 +997,7 @@ __global__ __launch_bounds__(FieldUpdateThreads<Real>::N) void k_field_update(Ge
             }
             if (a.push_blocks)
                 push_fields_store(g, i, j, k, fb[0], fb[1], fb[2], u[0], u[1], u[2], ep[0], ep[1], ep[2], a.use_e_pressure,
-                                  a.push_blocks, false);
+                                  a.push_blocks, false, deep);
         }
         count_event(a.counters, C_MAX_CELL_UE, c_ue);
     }

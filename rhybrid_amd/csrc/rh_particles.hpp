@@ -148,7 +148,7 @@ template <typename Real>
 __device__ __forceinline__ void push_fields_store(const Geom& g, int i, int j, int k, Real bx, Real by, Real bz,
                                                   Real uex, Real uey, Real uez, Real epx, Real epy, Real epz,
                                                   int use_e_pressure, uint32_t* __restrict__ out,
-                                                  bool write_mask = true) {
+                                                  bool write_mask = true, bool nowrap = false) {
     const int hx = g.tx + 1, hy = g.ty + 1, nh = (g.tx + 1) * (g.ty + 1) * (g.tz + 1);
     const size_t nw = push_field_words<Real>(g);
     const int li = i & (g.tx - 1), lj = j & (g.ty - 1), lk = k & (g.tz - 1);
@@ -167,15 +167,18 @@ __device__ __forceinline__ void push_fields_store(const Geom& g, int i, int j, i
         if (write_mask) reinterpret_cast<uint32_t*>(sep + 3 * g.tc)[lc] = neighbour_mask(g, i, j, k);
     }
     // halo copies: the cell is local coordinate -1 of the tile above along
-    // each axis where it is the tile's last cell
+    // each axis where it is the tile's last cell (nowrap: the caller knows
+    // that those tiles lie inside the grid without a wrap)
     const int ux = li == g.tx - 1, uy = lj == g.ty - 1, uz = lk == g.tz - 1;
     for (int c = 0; c <= uz; ++c)
         for (int b = 0; b <= uy; ++b)
             for (int a = 0; a <= ux; ++a) {
                 if (a + b + c == 0) continue;
-                const int ti = wrap_axis(i + a, g.nx, g.per[0]), tj = wrap_axis(j + b, g.ny, g.per[1]),
-                          tk = wrap_axis(k + c, g.nz, g.per[2]);
-                if (ti < 0 || tj < 0 || tk < 0) continue;
+                int ti = i + a, tj = j + b, tk = k + c;
+                if (!nowrap) {
+                    ti = wrap_axis(ti, g.nx, g.per[0]); tj = wrap_axis(tj, g.ny, g.per[1]); tk = wrap_axis(tk, g.nz, g.per[2]);
+                    if (ti < 0 || tj < 0 || tk < 0) continue;
+                }
                 Real* blk = reinterpret_cast<Real*>(out + (size_t)tile_of(g, ti, tj, tk) * nw);
                 const int h = (a ? 0 : li + 1) + hx * ((b ? 0 : lj + 1) + hy * (c ? 0 : lk + 1));
                 blk[h] = bx; blk[nh + h] = by; blk[2 * nh + h] = bz;
