@@ -500,6 +500,7 @@ def exosphere_bench(steps: int = 10, warmup: int = 2, prefill: int = 600, tune_s
     t_gen = time.time() - t0
 
     def prepare(ctx):
+        ctx.set_species_streams(species_streams)  # the autotuner times the shapes as the box then runs
         ctx.fill(0, pr.meta["fill"][0])
         for t in range(1, prefill + 1, 100):  # in parts, each reported (long under a profiler)
             ctx.fill(1, pr.injectors[1], timesteps=range(t, min(t + 100, prefill + 1)))
@@ -509,7 +510,7 @@ def exosphere_bench(steps: int = 10, warmup: int = 2, prefill: int = 600, tune_s
 
     if tile is None:
         shapes = autotune.candidates(pr.grid)[:tune_shapes]
-        tuned = autotune.tune(pr, steps=3, shapes=shapes, prepare=prepare)
+        tuned = autotune.tune(pr, steps=6, shapes=shapes, prepare=prepare)
     else:  # a given tile shape (profiled runs): no autotuning
         tuned = {"best": tuple(tile), "ms_per_step": None}
     ctx = Context(pr, precision=RH_FP32, tile=tuned["best"])
