@@ -730,6 +730,9 @@ inline int deposit_gather_lists(const Geom& g, uint16_t* list, uint8_t* count) {
                 list[e * 64 + b - 1] = t;
             }
         count[e] = (uint8_t)m;
+        // padded to a multiple of 8 entries with a word the deposit keeps at
+        // zero (corner 0, octant 0, the cell past the tile's: cell padding)
+        for (int t = m; t < ((m + 7) & ~7); ++t) list[e * 64 + t] = (uint16_t)g.tc;
     }
     return 0;
 }
@@ -740,7 +743,10 @@ inline int deposit_gather_lists(const Geom& g, uint16_t* list, uint8_t* count) {
 // (up to 27) cell terms that reach it.  LDS: [quantity][target][cell], cell
 // blocks of deposit_octant_block words.  The lists hold, for every cell of
 // the extended region, the offsets (target, cell) of its terms in ascending
-// order, at most 27 per cell (stride 32); built once on the host.
+// order, at most 27 per cell (stride 32), padded to a multiple of 8 entries
+// with the offset of a word the deposit keeps at zero (target 0, the cell
+// past the tile's: inside the padding of the cell blocks); built once on the
+// host.
 inline int deposit_target_lists(const Geom& g, uint16_t* list, uint8_t* count) {
     const int CB = deposit_octant_block(g);
     for (int e = 0; e < g.next; ++e) {
@@ -767,6 +773,7 @@ inline int deposit_target_lists(const Geom& g, uint16_t* list, uint8_t* count) {
                 list[e * 32 + b - 1] = t;
             }
         count[e] = (uint8_t)m;
+        for (int t = m; t < ((m + 7) & ~7); ++t) list[e * 32 + t] = (uint16_t)g.tc;
     }
     return 0;
 }
@@ -859,15 +866,22 @@ __device__ void deposit_dense_tile(const Geom& g, const PArrays<Real>& A, const 
 #pragma unroll
                     for (int c = 0; c < 8; ++c) S[(q * 8 + c) * QS + o * OB + cell] = acc[q][c];
         }
+        if (tid < 2) S[tid * 8 * QS + g.tc] = Real(0);  // the lists' padding entries read these words
         __syncthreads();
         for (int e = tid; e < g.next; e += blockDim.x) {
-            const int cnt_e = gcount[e];
-            const uint16_t* lst = glist + (size_t)e * 64;
+            // eight entries at a time (the padding adds zeros after the terms)
+            const int ng = (gcount[e] + 7) >> 3;
+            const uint4* lst = reinterpret_cast<const uint4*>(glist + (size_t)e * 64);
             Real res[2] = {Real(0), Real(0)};
-            for (int t = 0; t < cnt_e; ++t) {
-                const int off = lst[t];
-                res[0] += S[off];
-                res[1] += S[8 * QS + off];
+            for (int gi = 0; gi < ng; ++gi) {
+                const uint4 l8 = lst[gi];
+                const uint32_t w[4] = {l8.x, l8.y, l8.z, l8.w};
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const int off = (int)((w[t >> 1] >> (16 * (t & 1))) & 0xFFFFu);
+                    res[0] += S[off];
+                    res[1] += S[8 * QS + off];
+                }
             }
             if (half == 0) {
                 R[2 * e] = res[0];
@@ -970,20 +984,27 @@ __global__ __launch_bounds__(512) void k_deposit(Geom g, PArrays<Real> A, const 
 #pragma unroll
         for (int q = 0; q < 4; ++q) S[q * QP + t27 * CB + cell] = acc[q][c];
     }
+    if (ls < 4) S[ls * QP + g.tc] = Real(0);  // the lists' padding entries read this word
     __syncthreads();
     Real4<Real>* out = reinterpret_cast<Real4<Real>*>(partial) + (size_t)T * g.next;
     for (int e = ls; e < g.next; e += blockDim.x) {
-        // the cell terms reaching this cell: a list shared by all tiles
-        const int cnt_e = tcount[e];
-        const uint16_t* lst = tlist + (size_t)e * 32;
+        // the cell terms reaching this cell: a list shared by all tiles, read
+        // eight entries at a time (the padding adds zeros after the terms)
+        const int ng = (tcount[e] + 7) >> 3;
+        const uint4* lst = reinterpret_cast<const uint4*>(tlist + (size_t)e * 32);
         Real4<Real> r;
         r.v[0] = r.v[1] = r.v[2] = r.v[3] = Real(0);
-        for (int t = 0; t < cnt_e; ++t) {
-            const int off = lst[t];
-            r.v[0] += S[off];
-            r.v[1] += S[QP + off];
-            r.v[2] += S[2 * QP + off];
-            r.v[3] += S[3 * QP + off];
+        for (int gi = 0; gi < ng; ++gi) {
+            const uint4 l8 = lst[gi];
+            const uint32_t w[4] = {l8.x, l8.y, l8.z, l8.w};
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int off = (int)((w[t >> 1] >> (16 * (t & 1))) & 0xFFFFu);
+                r.v[0] += S[off];
+                r.v[1] += S[QP + off];
+                r.v[2] += S[2 * QP + off];
+                r.v[3] += S[3 * QP + off];
+            }
         }
         out[e] = r;
     }
