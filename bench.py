@@ -98,6 +98,18 @@ def measured_traffic(kernel: str, profile: str = "hbm_traffic_obstacle128x96x96_
     return None
 
 
+def step_bound(np_mean: float, ncells: int, nspec: int, ms_per_step: float) -> dict:
+    """The whole step against the path's per-unit bytes: 56 B per
+    particle-step (push + deposit, FP32), 52 B per cell-step per species
+    (slot and ghost traffic) and 68 B per cell-step (the fused field update);
+    the time those bytes take at the HBM peak over the measured step."""
+    b = 56.0 * np_mean + 52.0 * ncells * nspec + 68.0 * ncells
+    bound_ms = b / (HBM_PEAK_GBS * 1e9) * 1e3
+    return {"per_unit": "56 B/particle-step + 52 B/cell-step per species + 68 B/cell-step",
+            "bytes_per_step": b, "bound_ms_at_peak": round(bound_ms, 4),
+            "frac": round(bound_ms / ms_per_step, 4) if ms_per_step > 0 else None}
+
+
 def gpu_bench(args, pr):
     from rhybrid_amd.gpu import RH_FP32, Context
     n0 = pr.n_particles
@@ -173,6 +185,7 @@ def gpu_bench(args, pr):
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
                      "algorithmic_bytes": dom_bytes},
+        "step_bound": step_bound(np_mean, lay["ncells"], nspec, wall / args.steps * 1e3),
         "push_deposit": {"particle_steps_per_s": np_mean * nb / t_pd if t_pd > 0 else None,
                          "ms_per_step": t_pd / nb * 1e3,
                          "effective_GBps_vs_56B_per_particle": round(pd_bytes * nb / t_pd / 1e9, 1)},
@@ -687,6 +700,7 @@ def main():
         "error_curve_fp32_vs_fp64": curve,
         "denoiser": dn,
         "exosphere_box": exo,
+        "step_bound": res["step_bound"],
         "push_deposit": res["push_deposit"],
         "kernels": res["kernels"],
     }
