@@ -305,13 +305,16 @@ public:
             RH_HIP(hipGetDevice(&dev));
             RH_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
             lds_limit = (size_t)lds;
+            int lds_cu = 0;
+            RH_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+            lds_per_cu = (size_t)lds_cu;
         }
         choose_tiles(gd->tile);
         // dynamic LDS above the default 64 KiB (FP64 tiles of 64 cells)
         RH_HIP(hipFuncSetAttribute((const void*)k_deposit<Real>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)deposit_lds_bytes<Real>(g)));
         RH_HIP(hipFuncSetAttribute((const void*)k_push_move<Real>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)push_move_lds_bytes<Real>(g)));
+                                   (int)std::max(push_move_lds_bytes<Real>(g), move_lds_bytes())));
         RH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         ls = stream;
         RH_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
@@ -393,6 +396,20 @@ public:
     }
 
     size_t lds_limit = 65536;  // the device's LDS per workgroup (queried at creation)
+    size_t lds_per_cu = 0;     // and per CU
+
+    // The move pass (pass 1 of k_push_move) runs with at most MOVE_WG_PER_CU
+    // workgroups per CU, set through its dynamic LDS request: with fewer
+    // tiles' scattered stores in flight per CU it runs faster (the 128x96x96
+    // box: 500 us per launch at the 7 workgroups its registers allow, 483 at
+    // 5 or 6, 474 at 4, 526 at 3).  The push pass keeps the full occupancy.
+    static constexpr int MOVE_WG_PER_CU = 4;
+    size_t move_lds_bytes() const {
+        const size_t need = push_move_lds_bytes<Real>(g);
+        if (lds_per_cu == 0) return need;
+        const size_t cap = lds_per_cu / MOVE_WG_PER_CU - 1024;  // below the share, so MOVE_WG_PER_CU fit
+        return std::min(lds_limit, std::max(need, cap));
+    }
 
     // LDS of the largest per-tile kernels for a tile shape
     size_t tile_lds_bytes(const int* t) const {
@@ -799,7 +816,7 @@ public:
                N_XCD * cdiv(cdiv((uint32_t)g.ntiles * (uint32_t)g.tc, (uint32_t)SLOT_THREADS), (uint32_t)N_XCD),
                SLOT_THREADS, 0, g, S_.cnt_tile.p, (const uint8_t*)S_.live_src.p, S_.sub_new.p, dcount.p);
         scan(S_.sub_new.p, g.nkeys, S_.sub_new.p, S_.scan_lv, 0);
-        launch(move_name, k_push_move<Real>, tile_grid(), PM_THREADS, push_move_lds_bytes<Real>(g), g, S_.pp,
+        launch(move_name, k_push_move<Real>, tile_grid(), PM_THREADS, move_lds_bytes(), g, S_.pp,
                (const uint32_t*)pf_tile.p,
                S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.appendix(), S_.cnt_tile.p,
                (const uint32_t*)S_.sub_new.p, S_.pcount.p, kbits, 1, (uint32_t)S_.cap);
