@@ -252,16 +252,21 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
     const uint32_t deadkey = (uint32_t)MAX_EXTK;
     // prefetch of the first chunk; an injected particle beyond the store's
     // capacity was never written (the injection counted the overflow)
-    Real nX = 0, nY = 0, nZ = 0, nvx = 0, nvy = 0, nvz = 0;
-    int nlc = 0;
-    {
-        const uint32_t u = tid;
+    // (the tile's particles in chunks of PM_THREADS, prefetched two chunks
+    // ahead into two register buffers used in turn: no copies between them,
+    // so a buffer's loads are waited for only where its chunk is used)
+    struct PBuf { Real X, Y, Z, vx, vy, vz; int lc; };
+    auto load_chunk = [&](PBuf& b, uint32_t u) {
         const uint32_t p = u < n0 ? p0 + u : a0 + (u - n0);
         if (u < ntot) {
-            if (p < cap) { nX = A.x[p]; nY = A.y[p]; nZ = A.z[p]; nvx = A.vx[p]; nvy = A.vy[p]; nvz = A.vz[p]; nlc = A.lc[p]; }
-            else nX = Real(-1);
+            if (p < cap) { b.X = A.x[p]; b.Y = A.y[p]; b.Z = A.z[p]; b.vx = A.vx[p]; b.vy = A.vy[p]; b.vz = A.vz[p]; b.lc = A.lc[p]; }
+            else b.X = Real(-1);
         }
-    }
+    };
+    PBuf bufA{Real(0), Real(0), Real(0), Real(0), Real(0), Real(0), 0};
+    PBuf bufB = bufA;
+    load_chunk(bufA, tid);
+    load_chunk(bufB, tid + PM_THREADS);
     // slot counters: the 8 octant slots of an extended cell are 8 consecutive
     // entries (16-byte vectors of counts, 32-byte vectors of subcell starts)
     uint4* cnt4 = reinterpret_cast<uint4*>(cnt);
@@ -300,23 +305,12 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
     }
     __syncthreads();
     uint32_t n_clamped = 0, n_dropped = 0;  // this lane's events (pass 0)
-    for (uint32_t ub = 0; ub < ntot; ub += PM_THREADS) {
+    auto chunk = [&](uint32_t ub, PBuf& buf) {
         const uint32_t u = ub + tid;
         const bool act = u < ntot;
-        Real X = nX, Y = nY, Z = nZ, vx = nvx, vy = nvy, vz = nvz;
-        const int lc = nlc;
-        {
-            const uint32_t un = u + PM_THREADS;
-            const uint32_t pn = un < n0 ? p0 + un : a0 + (un - n0);
-            if (un < ntot) {
-                if (pn < cap) {
-                    nX = A.x[pn]; nY = A.y[pn]; nZ = A.z[pn]; nvx = A.vx[pn]; nvy = A.vy[pn]; nvz = A.vz[pn];
-                    nlc = A.lc[pn];
-                } else {
-                    nX = Real(-1);
-                }
-            }
-        }
+        Real X = buf.X, Y = buf.Y, Z = buf.Z, vx = buf.vx, vy = buf.vy, vz = buf.vz;
+        const int lc = buf.lc;
+        load_chunk(buf, u + 2 * PM_THREADS);
         uint32_t key = deadkey;
         bool clamped = false, dropped = false;
         int di = 0, dj = 0, dk = 0;  // destination cell, local coordinates of the source tile
@@ -392,21 +386,13 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
                 dropped = true;
             }
         }
-        // The next chunk's loads were issued before this chunk's push: take
-        // their values here, before pass 1's stores.  The vector memory
-        // counter retires loads and stores in issue order, so a wait for those
-        // loads placed after the stores (at the loop's end) would also wait
-        // for the stores to complete, once per chunk.  (Here, before the
-        // passes part, so that neither path reaches the loop's end with the
-        // loads outstanding.)
-        asm volatile("" ::"v"(nX), "v"(nY), "v"(nZ), "v"(nvx), "v"(nvy), "v"(nvz), "v"(nlc));
         if (pass == 0) {
             // counts only: integer LDS atomics, so the order does not matter;
             // the events are counted once per tile
             if (key != deadkey) atomicAdd(&cnt[key], 1u);
             n_clamped += clamped;
             n_dropped += dropped;
-            continue;
+            return;
         }
         // tiles divide the grid, so a periodic wrap keeps the local index
         const uint8_t nlcell = (uint8_t)((di & (g.tx - 1)) + ((dj & (g.ty - 1)) << g.ltx) +
@@ -441,6 +427,10 @@ __global__ __launch_bounds__(PM_THREADS) void k_push_move(Geom g, PushParams pp,
             grp[w * g.extk + key] = 0;
         }
         __syncthreads();
+    };
+    for (uint32_t ub = 0; ub < ntot; ub += 2 * PM_THREADS) {
+        chunk(ub, bufA);
+        if (ub + PM_THREADS < ntot) chunk(ub + PM_THREADS, bufB);  // the same for the whole workgroup
     }
     __syncthreads();
     if (pass == 0) {
