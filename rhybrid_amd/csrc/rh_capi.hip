@@ -226,6 +226,9 @@ public:
         DevBuf<uint32_t> inj_cbeg, inj_off;
         DevBuf<unsigned long long> inj_total;
         bool inj_pending = false;  // injected particles wait past the store's end
+        // at least 4 particles per cell at the end of the last rh_step call
+        // (the move's occupancy cap pays for such species only)
+        bool dense = true;
         // log counters: the push's own counters (maxVi per species), per
         // boundary cell (escapes, sum v^2, impacts, sum v^2), per injection
         // place sum v^2; host carries of the accumulators when their layout
@@ -402,11 +405,14 @@ public:
     // workgroups per CU, set through its dynamic LDS request: with fewer
     // tiles' scattered stores in flight per CU it runs faster (the 128x96x96
     // box: 500 us per launch at the 7 workgroups its registers allow, 483 at
-    // 5 or 6, 474 at 4, 526 at 3).  The push pass keeps the full occupancy.
+    // 5 or 6, 474 at 4, 526 at 3).  The push pass keeps the full occupancy,
+    // and so does the move of a species with fewer than 4 particles per cell
+    // (at the end of the last rh_step call): its move is set by a few crowded
+    // tiles, which the cap only starts later.
     static constexpr int MOVE_WG_PER_CU = 4;
-    size_t move_lds_bytes() const {
+    size_t move_lds_bytes(const Species* S_ = nullptr) const {
         const size_t need = push_move_lds_bytes<Real>(g);
-        if (lds_per_cu == 0) return need;
+        if (lds_per_cu == 0 || (S_ && !S_->dense)) return need;
         const size_t cap = lds_per_cu / MOVE_WG_PER_CU - 1024;  // below the share, so MOVE_WG_PER_CU fit
         return std::min(lds_limit, std::max(need, cap));
     }
@@ -816,7 +822,7 @@ public:
                N_XCD * cdiv(cdiv((uint32_t)g.ntiles * (uint32_t)g.tc, (uint32_t)SLOT_THREADS), (uint32_t)N_XCD),
                SLOT_THREADS, 0, g, S_.cnt_tile.p, (const uint8_t*)S_.live_src.p, S_.sub_new.p, dcount.p);
         scan(S_.sub_new.p, g.nkeys, S_.sub_new.p, S_.scan_lv, 0);
-        launch(move_name, k_push_move<Real>, tile_grid(), PM_THREADS, move_lds_bytes(), g, S_.pp,
+        launch(move_name, k_push_move<Real>, tile_grid(), PM_THREADS, move_lds_bytes(&S_), g, S_.pp,
                (const uint32_t*)pf_tile.p,
                S_.A(), S_.Dst(), (const uint32_t*)S_.sub.p, S_.appendix(), S_.cnt_tile.p,
                (const uint32_t*)S_.sub_new.p, S_.pcount.p, kbits, 1, (uint32_t)S_.cap);
@@ -1133,6 +1139,15 @@ public:
             propagate_B_impl(it == n - 1);
         }
         check_overflow();
+        // each species' particle count (the sorted store's end; the stream has
+        // finished, check_overflow waited for it), for the next call's choice
+        // of the move's occupancy
+        for (auto& s_ : sp) {
+            uint32_t np = 0;
+            RH_HIP(hipMemcpyAsync(&np, s_->sub.p + g.nkeys, sizeof(np), hipMemcpyDeviceToHost, stream));
+            RH_HIP(hipStreamSynchronize(stream));
+            s_->dense = (double)np >= 4.0 * (double)g.nc;
+        }
     }
 
     // propagate, wait_particle_sends and accumulate of every species, each
